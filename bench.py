@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark of the short-range force path on the 64k-atom LJ liquid-vapour slab.
+
+One step = Langevin-style Gaussian perturbation of every position (std 0.01
+sigma, seeded) -> position refresh + per-wavefront skin test -> LJ force,
+energy and virial with the per-atom cutoff rc(z) and the per-atom tail
+correction.  The neighbor list is rebuilt every --rebuild-every steps; the
+skin flag recorded on the device is checked before every rebuild, so a run in
+which any atom moved more than skin/2 between rebuilds fails instead of
+reporting.  Steps between rebuilds are replayed as one hipGraph.
+
+Prints one JSON line:  value = pair evaluations per second (list entries
+processed by the force kernel per second of whole-job wall time, rebuilds
+included), ms_per_step, the FP64 form measured the same way, the force
+kernel's HBM roofline fraction, force/energy error of the mixed path against
+the FP64 form, and the CPU path (the FP64 oracle, OpenMP) timed on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "md.error_amd"))
+sys.path.insert(0, REPO)
+
+METRIC = "pair evals/s + ms/step, 64k LJ slab: mixed-prec vs FP64, max force err<=1e-5"
+HBM_PEAK_GBS = 8000.0
+PERTURB_SIGMA = 0.01
+SEED = 7
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(w, precision, steps, warmup, every, torch, Engine):
+    """Times `steps` perturb+update+force steps; returns a dict of figures."""
+    stream = torch.cuda.Stream()          # default priority
+    e = Engine(n_types=w.n_types, skin=w.skin, precision=precision,
+               stream=stream.cuda_stream)
+    w.setup(e, tail=True)
+    pos = torch.from_numpy(w.xyz.copy()).to("cuda")
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    e.set_perturbation(PERTURB_SIGMA, SEED, 0)
+
+    def segment_loop(n_steps):
+        done = 0
+        pairs = 0
+        rebuilds = 0
+        while done < n_steps:
+            n = min(every, n_steps - done)
+            nl, _, _ = e.neighbor_count()
+            e.graph_run(n, stats=True, tail=True, perturb=True)
+            pairs += nl * n
+            done += n
+            e.check_skin_flag()       # syncs; raises if the skin was exceeded
+            e.rebuild()
+            rebuilds += 1
+        return pairs, rebuilds
+
+    segment_loop(warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    pairs, rebuilds = segment_loop(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    # force kernel alone, back-to-back launches on the same stream (HIP events)
+    k_ms = e.time_compute(200, stats=True, tail=True)
+    nl, _, mx = e.neighbor_count()
+    e.compute(stats=True, tail=True)
+    nl, nin, mx = e.neighbor_count()
+    return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs,
+                rebuilds=rebuilds, force_ms=k_ms, n_list=nl, n_in_rc=nin, max_list=mx,
+                pair_evals_per_s=pairs / wall, ms_per_step=1e3 * wall / steps)
+
+
+def accuracy(w, torch, Engine):
+    """Mixed path vs the FP64 form at the same (initial) positions."""
+    import numpy as np
+    out = {}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        e = Engine(n_types=w.n_types, skin=w.skin, precision=prec)
+        w.setup(e, tail=True)
+        e.rebuild()
+        e.compute(stats=True, tail=True)
+        res[prec] = (e.forces(), e.stats()[0], e.atom_stats())
+        e.close()
+    F, S, A = res["mixed"]
+    F64, S64, A64 = res["fp64"]
+    rms = np.sqrt((F64 ** 2).sum(1).mean())
+    d = np.sqrt(((F - F64) ** 2).sum(1))
+    out["max_rel_force_err"] = float(d.max() / rms)
+    out["rms_rel_force_err"] = float(np.sqrt((d ** 2).mean()) / rms)
+    floor = np.maximum(np.sqrt((F64 ** 2).sum(1)), rms)
+    out["max_per_atom_rel_err_floored"] = float((d / floor).max())
+    out["energy_rel_err"] = float(abs(S[0] - S64[0]) / abs(S64[0]))
+    out["virial_rel_err"] = float(np.max(np.abs(S[1:] - S64[1:]) / np.abs(S64[1:])))
+    out["reference"] = "FP64 GPU form (itself checked against the FP64 oracle in tests/)"
+    return out
+
+
+def cpu_baseline(w, budget_s):
+    """The FP64 oracle (C++/OpenMP) on the host: list build + force evaluations."""
+    import numpy as np
+    from oracle import oracle as O
+    box = w.boxes[0]
+    rc, _ = O.assign_rc(w.xyz, box, w.rc_tables[0])
+    tail = O.tail_atoms(w.xyz, box, w.types, w.eps, w.sigma, w.rc_tables[0].ravel(), w.rho[0])
+    t0 = time.perf_counter()
+    off, j = O.build_pairs(w.xyz, box, rc, w.skin)
+    t_build = time.perf_counter() - t0
+    n_eval = 0
+    t0 = time.perf_counter()
+    while True:
+        O.forces(w.xyz, box, w.types, w.eps, w.sigma, rc, off, j, tail=tail)
+        n_eval += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n_eval >= 2000:
+            break
+    t_force = el / n_eval
+    L = int(off[-1])
+    return {"value": L / t_force, "unit": "pair evals/s", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"{n_eval} FP64 force evaluations of the 64k slab ({L} list entries, "
+                      f"{t_force * 1e3:.2f} ms each) + 1 list build ({t_build * 1e3:.1f} ms); "
+                      f"ms/step at the same rebuild period "
+                      f"{(t_force + t_build / 6) * 1e3:.2f}",
+            "ms_force": t_force * 1e3, "ms_build": t_build * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--rebuild-every", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=5.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
+              "multi-GPU layout of this path is described, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np  # noqa: F401
+    import torch
+    from sre import Engine, configs
+
+    if not torch.cuda.is_available():
+        log("bench.py needs a GPU")
+        sys.exit(1)
+    w = configs.slab()
+    log(f"[bench] workload {w.name}: N={w.n}, {w.notes}")
+    mixed = run_form(w, "mixed", args.steps, args.warmup, args.rebuild_every, torch, Engine)
+    log(f"[bench] mixed: {mixed['pair_evals_per_s']:.4g} pair evals/s, "
+        f"{mixed['ms_per_step']:.4f} ms/step, force kernel {mixed['force_ms'] * 1e3:.2f} us")
+    fp64 = None
+    if not args.mixed_only:
+        fp64 = run_form(w, "fp64", args.steps, args.warmup, args.rebuild_every, torch, Engine)
+        log(f"[bench] fp64:  {fp64['pair_evals_per_s']:.4g} pair evals/s, "
+            f"{fp64['ms_per_step']:.4f} ms/step, force kernel {fp64['force_ms'] * 1e3:.2f} us")
+    acc = accuracy(w, torch, Engine)
+    log(f"[bench] accuracy: {acc}")
+
+    L, N = mixed["n_list"], w.n
+    b_force = 4.0 * L + 52.0 * N
+    achieved = b_force / (mixed["force_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_force_mixed<T,stats,tail>",
+                "bytes_per_launch": b_force,
+                "bytes_def": "4*L + 52*N (int32 list entry per pair; pos 16 + rc 4 + count 4 "
+                             "+ tail index 4 + 3x int64 force 24 B per atom)",
+                "avg_launch_us": round(mixed["force_ms"] * 1e3, 3)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu['value']:.4g} pair evals/s on {cpu['cores']} threads")
+    result = {
+        "metric": METRIC,
+        "value": round(mixed["pair_evals_per_s"], 1),
+        "unit": "pair evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math, i64 fixed-point forces, f64 energy/virial",
+        "data": "synthetic (seeded random-sequential-addition slab)",
+        "config": {"workload": w.name, "n_atoms": N, "box": [30, 30, 120],
+                   "rho_liquid_vapour": [0.73, 0.02], "rc_z": "2.5->5.0, 256 bins",
+                   "skin": w.skin, "rebuild_every": args.rebuild_every,
+                   "perturbation_sigma": PERTURB_SIGMA, "tail": "per-atom, bin mode",
+                   "list_entries": L, "pairs_in_rc": mixed["n_in_rc"],
+                   "tuning": mixed["engine"].tuning()},
+        "force_kernel_us": round(mixed["force_ms"] * 1e3, 3),
+        "force_pair_evals_per_s": round(L / (mixed["force_ms"] * 1e-3), 1),
+        "roofline": roofline,
+        "accuracy_vs_fp64": acc,
+        "cpu_baseline": cpu,
+    }
+    if fp64 is not None:
+        result["fp64_form"] = {
+            "value": round(fp64["pair_evals_per_s"], 1), "ms_per_step": round(fp64["ms_per_step"], 5),
+            "force_kernel_us": round(fp64["force_ms"] * 1e3, 3)}
+        result["speedup_mixed_vs_fp64"] = {
+            "whole_step": round(mixed["pair_evals_per_s"] / fp64["pair_evals_per_s"], 3),
+            "force_kernel": round(fp64["force_ms"] / mixed["force_ms"], 3)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,263 @@
+/*
+ * sre.h -- C ABI of the MI355X short-range engine (SRE).
+ *
+ * The engine replaces the cell-list -> Verlet-list -> Lennard-Jones force,
+ * energy and virial path of the reference's adaptive-cutoff GPU MD engine
+ * (shortRange.inhomo/template/tools/simulation.hetero.cutoff, "H/" below),
+ * including its per-atom cutoff rc(r) taken from a tabulated lattice, and the
+ * per-bin mean error force of the force-correction engine
+ * (shortRange.inhomo/template/tools/simulation.fcorr, "FC/" below).
+ *
+ * The reference has no plugin ABI; it is a set of C++ objects that own device
+ * memory and run kernels on the default stream.  Every entry point below names
+ * the reference member function it stands in for.  All functions return an
+ * sre_status (0 = success); on failure sre_last_error() describes the cause.
+ * No exception crosses this boundary.
+ *
+ * Conventions
+ *  - One engine holds R >= 1 independent periodic systems ("replicas") that
+ *    share one force field.  Atoms of replica r occupy the caller-order index
+ *    range [atom_offset(r), atom_offset(r) + n_atoms[r]).  R = 1 is the
+ *    reference's single MDSystem.
+ *  - Positions are FP32 (x, y, z[, w]) like the reference's CoordType float4
+ *    (H/include/systemDefines.h:48).  Any position is accepted: periodic
+ *    wrapping is exact inside the engine.
+ *  - Forces are produced as 64-bit fixed point with scale 2^32 (resolution
+ *    2.3e-10 in LJ units), so they are bitwise independent of summation order.
+ *  - Energy and virial follow H/src/InteractionEngine_interface.cu:1276-1279:
+ *    per atom 1/2 sum_j u_ij and 1/2 sum_j F_ij,a * (x_j - x_i)_a (diagonal a).
+ *  - Every launch goes on the engine's HIP stream (sre_options.stream), so a
+ *    caller may capture the per-step calls into a hipGraph.
+ */
+#ifndef SRE_H
+#define SRE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sre_engine sre_engine;
+
+/* Status codes.  The first three mirror the reference's device error flags
+ * (H/include/error.h:19-26). */
+typedef enum {
+  SRE_OK = 0,
+  SRE_ERR_LIST_OVERFLOW = 1,   /* mdErrorShortNeighborList: only reported if regrowing failed */
+  SRE_ERR_CELL = 2,            /* mdErrorShortCellList: cell grid too large for the sort key */
+  SRE_ERR_OUT_OF_BOX = 3,      /* mdErrorOverFlowCellIdx: never raised (wrapping is exact); kept for ABI parity */
+  SRE_ERR_INVALID = 4,         /* bad argument or call order */
+  SRE_ERR_HIP = 5,             /* a HIP runtime call failed (cf. checkCUDAError, H/include/error.h:8-16) */
+  SRE_ERR_BOX_TOO_SMALL = 6,   /* rc_max + skin >= L/2 in some direction: minimum image is ambiguous */
+  SRE_ERR_FORCE_RANGE = 7,     /* a pair force component exceeded 2^19: fixed-point range */
+  SRE_ERR_SKIN = 8,            /* an atom moved more than skin/2 since the last rebuild (fixed-period mode) */
+  SRE_ERR_NOT_READY = 9        /* compute requested before the first rebuild */
+} sre_status;
+
+typedef enum {
+  SRE_PREC_MIXED = 0,  /* FP32 pair math, int64 fixed-point forces, FP64 energy/virial reduction */
+  SRE_PREC_FP64 = 1    /* FP64 pair math and accumulation ("plain FP64 GPU form") */
+} sre_precision;
+
+/* compute flags */
+#define SRE_STATS 1u        /* also energy and virial (the statistic kernel variant) */
+#define SRE_TAIL 2u         /* add the per-atom inhomogeneous tail correction */
+
+/* tail-correction evaluation modes (sre_set_tail_mode) */
+#define SRE_TAIL_BIN 0      /* value at the rc-bin centre, assigned per atom by bin (FC/src/AssignForceCorr.cu:95-128) */
+#define SRE_TAIL_EXACT 1    /* value evaluated at each atom's own z */
+
+typedef struct {
+  int n_types;          /* atom types (1..8) */
+  float skin;           /* Verlet skin ("rlistExten", H/app/lj.nhc-xtc.cu:44) */
+  int precision;        /* sre_precision */
+  void *stream;         /* hipStream_t to launch on; NULL = engine creates its own */
+  int device;           /* HIP device ordinal; -1 = current */
+} sre_options;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Creates an engine.  Replaces the constructors of MDSystem device data,
+ * CellList, NeighborList and InteractionEngine (H/src/MDSystem_interface.cu:146-171,
+ * H/include/CellList_interface.h:47-66, H/include/NeighborList_interface.h:61-80,
+ * H/include/InteractionEngine_interface.h:67-111). */
+sre_status sre_create(sre_engine **out, const sre_options *opt);
+void sre_destroy(sre_engine *e);
+const char *sre_last_error(void);
+const char *sre_version(void);
+
+/* ---- system description -------------------------------------------------- */
+
+/* Declares R replicas with n_atoms[r] atoms and box edge lengths box[3r..3r+2].
+ * Replaces MDSystem::initConfig/initDeviceData and RectangularBox setup
+ * (H/src/MDSystem_interface.cu:39-171, H/include/BoxGeometry.h:10-14). */
+sre_status sre_set_system(sre_engine *e, int n_replicas, const int *n_atoms,
+                          const float *box);
+
+/* Per-atom types (host array, caller order, values in [0, n_types)).
+ * Replaces MDSystem::initTopology (H/src/MDSystem_interface.cu:115-133). */
+sre_status sre_set_types(sre_engine *e, const int32_t *types);
+
+/* Type-pair LJ parameters, full symmetric n_types x n_types matrices (host).
+ * u(r) = 4 eps [(sig/r)^12 - (sig/r)^6] + shift.  Replaces
+ * SystemNonBondedInteraction::add + InteractionEngine::registNonBondedInteraction
+ * (H/src/SystemNonBondedInteraction.cu:235-330, H/src/InteractionEngine_interface.cu:92-166,
+ *  H/include/NonBondedInteraction.h:866-889).  shift may be NULL (0). */
+sre_status sre_set_pair_params(sre_engine *e, const float *eps,
+                               const float *sigma, const float *shift);
+
+/* Cutoff lattice for replica r: rc[iz + nz*(iy + ny*ix)], FP32.  Replaces
+ * AssignRCut::reinit/getRCut/uniform (H/src/AssignRCut.cu:17-63).  The lattice
+ * is assigned to atoms at every rebuild with the reference's bin rule
+ * (H/src/AssignRCut.cu:77-104). */
+sre_status sre_set_rc_table(sre_engine *e, int replica, int nx, int ny, int nz,
+                            const float *rc);
+
+/* Density profile rho_t(z) for replica r, t = 0..n_types-1, nz bins over
+ * [0, Lz) (host, FP64, layout rho[t*nz + iz]).  Input of the tail correction,
+ * the analogue of ForceCorr::calError's density input (FC/src/ForceCorr.cu:131-162). */
+sre_status sre_set_density_profile(sre_engine *e, int replica, int nz,
+                                   const double *rho);
+sre_status sre_set_tail_mode(sre_engine *e, int mode);
+
+/* ---- positions -------------------------------------------------------------- */
+
+/* Copies host positions (stride floats per atom, >= 3) into an engine-owned
+ * device array and binds it.  Replaces sys.ddata.coord uploads
+ * (H/src/MDSystem_interface.cu:146-171). */
+sre_status sre_upload_positions(sre_engine *e, const float *xyz, int stride);
+/* Binds a caller-owned device array (stride floats per atom).  The engine
+ * reads it on every sre_update_positions / sre_rebuild. */
+sre_status sre_bind_positions(sre_engine *e, const void *d_xyz, int stride);
+
+/* ---- the hot path ------------------------------------------------------------ */
+
+/* Assigns rc to every atom, bins atoms into cells with an atomic-free counting
+ * sort and builds the Verlet list j in L(i) <=> j != i and |d_ij|^2 < (rc_i + skin)^2.
+ * Replaces AssignRCut::assign + normalizeDeviceData + CellList::rebuild +
+ * NeighborList::rebuild + Displacement_max::recordCoord
+ * (H/app/lj.nhc-xtc.cu:170-181, H/src/NeighborList.cu:327-420,
+ *  U/src/NeighborList_interface.cu:361-491).  Regrows the list on overflow. */
+sre_status sre_rebuild(sre_engine *e);
+
+/* Refreshes device positions from the bound array and tests the skin (max
+ * displacement since the last rebuild vs skin/2, reduced per wavefront).
+ * If need_rebuild != NULL the call synchronises and reports the decision.
+ * Replaces Displacement_max::calMaxDisplacemant (H/src/Displacement_interface.cu:79-126). */
+sre_status sre_update_positions(sre_engine *e, int *need_rebuild);
+
+/* Forces (and with SRE_STATS energy/virial; with SRE_TAIL the tail term).
+ * Replaces InteractionEngine::clearInteraction + applyNonBondedInteraction
+ * (H/src/InteractionEngine_interface.cu:218-225, 240-382). */
+sre_status sre_compute(sre_engine *e, unsigned flags);
+
+/* update -> (rebuild if the skin test fires) -> compute; the reference loop
+ * body H/app/lj.nhc-xtc.cu:169-182.  *rebuilt (optional) reports a rebuild. */
+sre_status sre_step(sre_engine *e, unsigned flags, int *rebuilt);
+
+/* ---- results -------------------------------------------------------------- */
+
+/* Forces in caller order as double[N][3].  dst_on_device != 0: dst is a
+ * device pointer (no host sync).  Otherwise the call synchronises. */
+sre_status sre_get_forces(sre_engine *e, double *dst, int dst_on_device);
+/* Reference-compatible FP32 SoA accumulation forc{x,y,z}[i] += F_i (device
+ * pointers), as calNonBondedInteraction_neighbor writes them
+ * (H/src/InteractionEngine_interface.cu:1270-1273). */
+sre_status sre_add_forces_f32_soa(sre_engine *e, float *d_fx, float *d_fy,
+                                  float *d_fz);
+/* Raw fixed-point forces in caller order, int64[N][3] (scale 2^32), host. */
+sre_status sre_get_forces_fixed(sre_engine *e, int64_t *dst);
+
+/* Per replica: out[4r+0] = energy, out[4r+1..3] = virial xx, yy, zz (FP64),
+ * from the last SRE_STATS compute.  Replaces MDStatistic::updateHost +
+ * nonBondedEnergy (H/include/Statistic_interface.h:73-117). */
+sre_status sre_get_stats(sre_engine *e, double *out);
+/* Per-atom energy and virial diagonal (caller order, double[N][4]); the
+ * reference's per-atom statistic buffers (H/src/InteractionEngine_interface.cu:1276-1279). */
+sre_status sre_get_atom_stats(sre_engine *e, double *out);
+
+/* Sum of neighbor-list lengths (the reference's "tc" counter,
+ * NeighborList::calSumNeighbor, H/src/NeighborList_interface.cu:652-662),
+ * pairs inside rc_i at the last SRE_STATS compute, and the largest list. */
+sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list, int64_t *n_in_rc,
+                              int32_t *max_list);
+/* Neighbor lists as CSR in caller order with caller indices, rows sorted
+ * ascending: offsets[N+1], j[n_list].  Host arrays. */
+sre_status sre_export_pairs(sre_engine *e, int64_t *offsets, int32_t *j);
+/* Per-atom cutoff rc_i (caller order, host float[N]) as last assigned. */
+sre_status sre_get_rcut(sre_engine *e, float *rc);
+
+/* ---- adjoint (no reference counterpart) ----------------------------------- */
+
+/* Gradient of  L = sum_r dLdE[r] * E_r + sum_i dLdF_i . F_i  with respect to
+ * eps[a][b], sigma[a][b] (n_types^2 each, symmetric pairs summed into both
+ * entries) and every rc lattice value (concatenated over replicas).  Pair
+ * terms are recomputed from the neighbor list.  The cutoff mask is held fixed,
+ * so d/d rc acts through the tail correction only.  dLdF is a device pointer
+ * (double[N][3], caller order) or NULL; outputs are host arrays. */
+sre_status sre_adjoint(sre_engine *e, const double *dLdE, const double *d_dLdF,
+                       unsigned flags, double *g_eps, double *g_sigma,
+                       double *g_rc);
+
+/* ---- tuning, graphs, utilities ------------------------------------------- */
+
+typedef struct {
+  int force_lanes;      /* lanes per atom in the force kernel: 1,2,4,8,16 */
+  int build_lanes;      /* lanes per atom in the list build */
+  int block;            /* threads per block of the force kernel: 64..1024 */
+  float cell_xy;        /* cell edge in x,y as a multiple of (rc_min + skin) */
+  float cell_z;         /* cell edge in z as a multiple of (rc_min + skin) */
+} sre_tuning;
+sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
+/* Changing cell sizes or lanes takes effect at the next rebuild. */
+sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t);
+/* Times candidate tunings on the current system and keeps the fastest
+ * (forces are bitwise identical across candidates).  cache_path (optional)
+ * stores/reads results keyed by the system's signature. */
+sre_status sre_autotune(sre_engine *e, unsigned flags, const char *cache_path);
+
+/* Captures [perturb +] update_positions + compute(flags) on the engine stream
+ * into a hipGraph and replays it n times; the skin test only records
+ * violations (SRE_ERR_SKIN is returned by sre_check_skin_flag).  With
+ * SRE_PERTURB in flags every replay first applies the perturbation set by
+ * sre_set_perturbation, advancing its step counter on the device. */
+#define SRE_PERTURB 4u
+sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
+sre_status sre_set_perturbation(sre_engine *e, float sigma, uint64_t seed,
+                                uint64_t first_step);
+sre_status sre_check_skin_flag(sre_engine *e);
+
+/* Gaussian displacement of the bound positions (device), std sigma, seeded,
+ * counter-based: x += sigma * N(0,1).  The Langevin-style perturbation the
+ * benchmark applies between force evaluations. */
+sre_status sre_perturb_positions(sre_engine *e, float sigma, uint64_t seed,
+                                 uint64_t step);
+
+/* Average duration (ms) of n_iter back-to-back launches of the force kernel
+ * alone (no statistic reduction), from HIP events on the engine stream. */
+sre_status sre_time_compute(sre_engine *e, unsigned flags, int n_iter,
+                            double *ms_per_iter);
+/* Average duration (ms) of n_iter complete rebuilds (includes their host
+ * synchronisation for the list-capacity check). */
+sre_status sre_time_rebuild(sre_engine *e, int n_iter, double *ms_per_iter);
+sre_status sre_synchronize(sre_engine *e);
+
+/* ---- synthetic configurations (host only, no GPU use) --------------------- */
+
+/* FCC lattice of ncell^3 unit cells with lattice constant a, every coordinate
+ * jittered by U(-jitter, jitter).  xyz[4N] (w = 0). N = 4 ncell^3. */
+sre_status sre_gen_fcc(int ncell, double a, double jitter, uint64_t seed,
+                       float *xyz);
+/* Liquid-vapour slab: n atoms, box (lx, ly, lz), liquid of density rho_l in
+ * |z - lz/2| < h, vapour rho_v elsewhere, liquid count round(rho_l * lx*ly*2h),
+ * uniform random positions with a minimum pair distance dmin (random
+ * sequential addition), seeded.  xyz[4n]. */
+sre_status sre_gen_slab(int n, double lx, double ly, double lz, double h,
+                        double rho_l, double dmin, uint64_t seed, float *xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRE_H */

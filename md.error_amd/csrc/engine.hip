@@ -1,0 +1,1357 @@
+// engine.hip -- host side of the short-range engine and its C ABI.
+//
+// One sre_engine bundles what the reference spreads over MDSystem (device
+// data, H/include/MDSystem.h:43-78), AssignRCut (H/include/AssignRCut.h:10-42),
+// CellList (H/include/CellList_interface.h:18-73), NeighborList
+// (H/include/NeighborList_interface.h:20-82), InteractionEngine
+// (H/include/InteractionEngine_interface.h:17-155), MDStatistic
+// (H/include/Statistic_interface.h:30-117) and Displacement_max
+// (H/include/Displacement_interface.h:15-43).  Unlike the reference it keeps
+// no global state (no __constant__ tables, no texture bindings), so several
+// engines can coexist; each launches only on its own stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sre.h"
+#include "sre_internal.h"
+
+using namespace sre;
+
+// ------------------------------------------------------------ error state --
+
+static thread_local std::string g_err;
+
+static sre_status fail(sre_status s, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return s;
+}
+
+#define HIP_TRY(call)                                                      \
+  do {                                                                     \
+    hipError_t _e = (call);                                                \
+    if (_e != hipSuccess)                                                  \
+      return fail(SRE_ERR_HIP, "%s: %s (%s:%d)", #call,                    \
+                  hipGetErrorString(_e), __FILE__, __LINE__);              \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// ----------------------------------------------------------------- engine --
+
+struct sre_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int n_types = 1;
+  float skin = 0.3f;
+  int precision = SRE_PREC_MIXED;
+  sre_tuning tune{4, 8, 256, 0.5f, 0.5f};
+
+  // system
+  int n_rep = 0, n_atoms = 0;
+  std::vector<int> rep_n;
+  std::vector<float> box;
+  std::vector<RepInfo> reps;
+  RepInfo *d_reps = nullptr;
+  int *d_rep_of = nullptr;
+  int *d_types = nullptr;
+  bool types_set = false;
+
+  // force field
+  std::vector<float> eps, sig, shift;
+  PairParam *d_params = nullptr;
+  PairParamD *d_paramsd = nullptr;
+  bool params_set = false;
+
+  // rc lattices
+  std::vector<std::vector<float>> rc_tab;
+  std::vector<int> rc_dims;  // 3 per replica
+  float *d_rc = nullptr;
+  int rc_total = 0;
+  bool rc_dirty = true;
+
+  // density profiles and tail tables
+  std::vector<std::vector<double>> rho;
+  std::vector<int> rho_nz;
+  double *d_rho = nullptr;
+  double2 *d_tail = nullptr;
+  int *d_tail_entry_rep = nullptr;
+  int tail_bins = 0;
+  double2 *d_tail_atom = nullptr;
+  int tail_mode = SRE_TAIL_BIN;
+  bool tail_dirty = true;
+
+  // positions
+  const float *d_pos = nullptr;
+  int pos_stride = 4;
+  float *d_pos_own = nullptr;
+  size_t pos_own_floats = 0;
+
+  // cells and sort
+  SortBufs sb{};
+  size_t counts_cap = 0;
+  int key_bits = 1;
+  int n_cells = 0;
+  int *d_start = nullptr;
+  int start_cap = 0;
+  uint4 *d_pos4 = nullptr, *d_backup = nullptr;
+  float4 *d_res4 = nullptr;   // FP64 form only
+  float *d_rcut = nullptr;
+  int *d_tail_idx = nullptr;
+
+  // neighbor list
+  int cap = 0;
+  int list_lanes = 0;
+  size_t list_size = 0;
+  int *d_list = nullptr;
+  int *d_nnb = nullptr;
+  int *d_maxcount = nullptr;
+  int max_count = 0;
+  int64_t n_list = 0;
+
+  // outputs
+  int64_t *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;
+  double *d_fxd = nullptr, *d_fyd = nullptr, *d_fzd = nullptr;
+  double *d_wave_part = nullptr;
+  size_t wave_part_cap = 0;
+  double *d_stats = nullptr;
+  double *d_atom_stats = nullptr;
+  double *d_tmp_out = nullptr;
+  unsigned *d_flags = nullptr;  // [0] max displacement^2 bits, [1] error bits
+  uint64_t *d_step_ctr = nullptr;  // perturbation step counter (graph mode)
+  float pert_sigma = 0.f;
+  uint64_t pert_seed = 0;
+  bool stats_valid = false;
+  bool atom_stats_valid = false;
+  bool forces_valid = false;
+
+  bool built = false;
+  bool system_set = false;
+  int graph_epoch = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  unsigned graph_flags = 0;
+  int graph_epoch_built = -1;
+
+  ~sre_engine();
+};
+
+sre_engine::~sre_engine() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  dfree(d_reps); dfree(d_rep_of); dfree(d_types); dfree(d_params);
+  dfree(d_paramsd); dfree(d_rc); dfree(d_rho); dfree(d_tail);
+  dfree(d_tail_entry_rep); dfree(d_tail_atom); dfree(d_pos_own);
+  dfree(sb.key_a); dfree(sb.val_a); dfree(sb.key_b); dfree(sb.val_b);
+  dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
+  dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
+  dfree(d_rcut); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
+  dfree(d_maxcount); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
+  dfree(d_fyd); dfree(d_fzd); dfree(d_wave_part); dfree(d_stats);
+  dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+static bool pow2_lanes(int t) {
+  return t == 1 || t == 2 || t == 4 || t == 8 || t == 16;
+}
+
+// ------------------------------------------------------------- lifetime --
+
+extern "C" const char *sre_last_error(void) { return g_err.c_str(); }
+extern "C" const char *sre_version(void) { return "sre 0.1 gfx950"; }
+
+extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
+  if (!out || !opt) return fail(SRE_ERR_INVALID, "sre_create: null argument");
+  *out = nullptr;
+  if (opt->n_types < 1 || opt->n_types > kMaxTypes)
+    return fail(SRE_ERR_INVALID, "n_types must be in [1, %d]", kMaxTypes);
+  if (!(opt->skin >= 0.f))
+    return fail(SRE_ERR_INVALID, "skin must be >= 0");
+  if (opt->precision != SRE_PREC_MIXED && opt->precision != SRE_PREC_FP64)
+    return fail(SRE_ERR_INVALID, "unknown precision %d", opt->precision);
+  sre_engine *e = new sre_engine();
+  e->n_types = opt->n_types;
+  e->skin = opt->skin;
+  e->precision = opt->precision;
+  if (opt->device >= 0) {
+    hipError_t r = hipSetDevice(opt->device);
+    if (r != hipSuccess) {
+      delete e;
+      return fail(SRE_ERR_HIP, "hipSetDevice(%d): %s", opt->device,
+                  hipGetErrorString(r));
+    }
+  }
+  if (hipGetDevice(&e->device) != hipSuccess) {
+    delete e;
+    return fail(SRE_ERR_HIP, "hipGetDevice failed (no GPU?)");
+  }
+  if (opt->stream) {
+    e->stream = (hipStream_t)opt->stream;
+  } else {
+    hipError_t r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (r != hipSuccess) {
+      delete e;
+      return fail(SRE_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(r));
+    }
+    e->own_stream = true;
+  }
+  e->eps.assign(e->n_types * e->n_types, 1.f);
+  e->sig.assign(e->n_types * e->n_types, 1.f);
+  e->shift.assign(e->n_types * e->n_types, 0.f);
+  hipError_t r = dalloc(&e->d_flags, 4);
+  if (r == hipSuccess) r = hipMemsetAsync(e->d_flags, 0, 4 * sizeof(unsigned), e->stream);
+  if (r == hipSuccess) r = dalloc(&e->d_maxcount, 1);
+  if (r == hipSuccess) r = dalloc(&e->d_params, kMaxPairTypes);
+  if (r == hipSuccess) r = dalloc(&e->d_paramsd, kMaxPairTypes);
+  if (r != hipSuccess) {
+    delete e;
+    return fail(SRE_ERR_HIP, "allocation failed: %s", hipGetErrorString(r));
+  }
+  *out = e;
+  return SRE_OK;
+}
+
+extern "C" void sre_destroy(sre_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  delete e;
+}
+
+// ------------------------------------------------------ system description --
+
+extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
+                                     const int *n_atoms, const float *box) {
+  if (!e || n_replicas < 1 || !n_atoms || !box)
+    return fail(SRE_ERR_INVALID, "sre_set_system: bad arguments");
+  HIP_TRY(hipSetDevice(e->device));
+  long long total = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    if (n_atoms[r] < (n_replicas > 1 ? kWave : 1))
+      return fail(SRE_ERR_INVALID,
+                  "replica %d has %d atoms; batched replicas need >= %d", r,
+                  n_atoms[r], kWave);
+    for (int d = 0; d < 3; ++d)
+      if (!(box[3 * r + d] > 0.f))
+        return fail(SRE_ERR_INVALID, "replica %d: box edge %d must be > 0", r, d);
+    total += n_atoms[r];
+  }
+  if (total > (1ll << 30))
+    return fail(SRE_ERR_INVALID, "too many atoms (%lld)", total);
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->n_rep = n_replicas;
+  e->n_atoms = (int)total;
+  e->rep_n.assign(n_atoms, n_atoms + n_replicas);
+  e->box.assign(box, box + 3 * n_replicas);
+  e->reps.assign(n_replicas, RepInfo{});
+  int off = 0;
+  for (int r = 0; r < n_replicas; ++r) {
+    RepInfo &R = e->reps[r];
+    for (int d = 0; d < 3; ++d) {
+      R.L[d] = box[3 * r + d];
+      R.q[d] = ldexpf(R.L[d], -32);
+      R.invq[d] = 4294967296.0 / (double)R.L[d];
+      R.qd[d] = ldexp((double)R.L[d], -32);
+    }
+    R.atom_off = off;
+    R.n_atoms = n_atoms[r];
+    R.tail_off = -1;
+    R.rho_off = -1;
+    off += n_atoms[r];
+  }
+  // host-side replica map, uploaded once
+  std::vector<int> rep_of(e->n_atoms);
+  for (int r = 0; r < n_replicas; ++r)
+    for (int i = 0; i < n_atoms[r]; ++i) rep_of[e->reps[r].atom_off + i] = r;
+  const int N = e->n_atoms;
+  dfree(e->d_rep_of); dfree(e->d_types); dfree(e->d_reps);
+  dfree(e->d_pos4); dfree(e->d_backup); dfree(e->d_rcut); dfree(e->d_tail_idx);
+  dfree(e->d_res4);
+  dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
+  dfree(e->d_fxd); dfree(e->d_fyd); dfree(e->d_fzd); dfree(e->d_atom_stats);
+  dfree(e->d_stats); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
+  dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
+  dfree(e->sb.tkey); dfree(e->sb.tval);
+  HIP_TRY(dalloc(&e->d_rep_of, N));
+  HIP_TRY(dalloc(&e->d_types, N));
+  HIP_TRY(dalloc(&e->d_reps, n_replicas));
+  HIP_TRY(dalloc(&e->d_pos4, N));
+  HIP_TRY(dalloc(&e->d_backup, N));
+  HIP_TRY(dalloc(&e->d_rcut, N));
+  HIP_TRY(dalloc(&e->d_tail_idx, N));
+  HIP_TRY(dalloc(&e->d_nnb, N));
+  if (e->precision == SRE_PREC_MIXED) {
+    HIP_TRY(dalloc(&e->d_fx, N));
+    HIP_TRY(dalloc(&e->d_fy, N));
+    HIP_TRY(dalloc(&e->d_fz, N));
+  } else {
+    HIP_TRY(dalloc(&e->d_res4, N));
+    HIP_TRY(dalloc(&e->d_fxd, N));
+    HIP_TRY(dalloc(&e->d_fyd, N));
+    HIP_TRY(dalloc(&e->d_fzd, N));
+  }
+  HIP_TRY(dalloc(&e->d_atom_stats, (size_t)4 * N));
+  HIP_TRY(dalloc(&e->d_stats, (size_t)5 * n_replicas));
+  HIP_TRY(dalloc(&e->d_tmp_out, (size_t)4 * N));
+  HIP_TRY(dalloc(&e->sb.key_a, N));
+  HIP_TRY(dalloc(&e->sb.val_a, N));
+  HIP_TRY(dalloc(&e->sb.key_b, N));
+  HIP_TRY(dalloc(&e->sb.val_b, N));
+  HIP_TRY(dalloc(&e->sb.tkey, N));
+  HIP_TRY(dalloc(&e->sb.tval, N));
+  HIP_TRY(hipMemcpy(e->d_rep_of, rep_of.data(), sizeof(int) * N,
+                    hipMemcpyHostToDevice));
+  // defaults: every atom type 0, uniform rc lattice of 1 bin (set later)
+  e->rc_tab.assign(n_replicas, std::vector<float>());
+  e->rc_dims.assign(3 * n_replicas, 0);
+  e->rho.assign(n_replicas, std::vector<double>());
+  e->rho_nz.assign(n_replicas, 0);
+  e->types_set = false;
+  e->rc_dirty = true;
+  e->tail_dirty = true;
+  e->built = false;
+  e->system_set = true;
+  e->d_pos = nullptr;
+  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  e->graph_epoch++;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_types(sre_engine *e, const int32_t *types) {
+  if (!e || !types || !e->system_set)
+    return fail(SRE_ERR_INVALID, "sre_set_types: system not set");
+  for (int i = 0; i < e->n_atoms; ++i)
+    if (types[i] < 0 || types[i] >= e->n_types)
+      return fail(SRE_ERR_INVALID, "type of atom %d is %d, outside [0,%d)", i,
+                  types[i], e->n_types);
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipMemcpy(e->d_types, types, sizeof(int) * e->n_atoms,
+                    hipMemcpyHostToDevice));
+  e->types_set = true;
+  e->built = false;
+  return SRE_OK;
+}
+
+static sre_status upload_params(sre_engine *e) {
+  const int nt = e->n_types;
+  std::vector<PairParam> p(nt * nt);
+  std::vector<PairParamD> pd(nt * nt);
+  for (int a = 0; a < nt * nt; ++a) {
+    const double ep = e->eps[a], sg = e->sig[a], sh = e->shift[a];
+    const double s6 = sg * sg * sg * sg * sg * sg, s12 = s6 * s6;
+    const double A = 4.0 * ep * s12, B = 4.0 * ep * s6;
+    pd[a] = PairParamD{12.0 * A, 6.0 * B, A, B, sh, ep, sg, 0.0};
+    p[a] = PairParam{(float)(12.0 * A), (float)(6.0 * B), (float)A, (float)B,
+                     (float)sh, (float)ep, (float)sg, 0.f};
+  }
+  HIP_TRY(hipMemcpy(e->d_params, p.data(), sizeof(PairParam) * nt * nt,
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(e->d_paramsd, pd.data(), sizeof(PairParamD) * nt * nt,
+                    hipMemcpyHostToDevice));
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_pair_params(sre_engine *e, const float *eps,
+                                          const float *sigma,
+                                          const float *shift) {
+  if (!e || !eps || !sigma)
+    return fail(SRE_ERR_INVALID, "sre_set_pair_params: null argument");
+  const int nt = e->n_types;
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const int ab = a * nt + b, ba = b * nt + a;
+      if (eps[ab] != eps[ba] || sigma[ab] != sigma[ba] ||
+          (shift && shift[ab] != shift[ba]))
+        return fail(SRE_ERR_INVALID, "pair parameters must be symmetric (%d,%d)", a, b);
+      if (!(sigma[ab] > 0.f) || !(eps[ab] >= 0.f))
+        return fail(SRE_ERR_INVALID, "need sigma > 0 and eps >= 0 (%d,%d)", a, b);
+    }
+  HIP_TRY(hipSetDevice(e->device));
+  e->eps.assign(eps, eps + nt * nt);
+  e->sig.assign(sigma, sigma + nt * nt);
+  if (shift) e->shift.assign(shift, shift + nt * nt);
+  else e->shift.assign(nt * nt, 0.f);
+  sre_status s = upload_params(e);
+  if (s != SRE_OK) return s;
+  e->params_set = true;
+  e->tail_dirty = true;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_rc_table(sre_engine *e, int replica, int nx,
+                                       int ny, int nz, const float *rc) {
+  if (!e || !e->system_set || replica < 0 || replica >= e->n_rep || !rc ||
+      nx < 1 || ny < 1 || nz < 1)
+    return fail(SRE_ERR_INVALID, "sre_set_rc_table: bad arguments");
+  const size_t m = (size_t)nx * ny * nz;
+  for (size_t k = 0; k < m; ++k)
+    if (!(rc[k] > 0.f))
+      return fail(SRE_ERR_INVALID, "rc lattice entry %zu must be > 0", k);
+  e->rc_tab[replica].assign(rc, rc + m);
+  e->rc_dims[3 * replica + 0] = nx;
+  e->rc_dims[3 * replica + 1] = ny;
+  e->rc_dims[3 * replica + 2] = nz;
+  e->rc_dirty = true;
+  e->tail_dirty = true;
+  e->built = false;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_density_profile(sre_engine *e, int replica,
+                                              int nz, const double *rho) {
+  if (!e || !e->system_set || replica < 0 || replica >= e->n_rep)
+    return fail(SRE_ERR_INVALID, "sre_set_density_profile: bad arguments");
+  if (!rho || nz <= 0) {
+    e->rho[replica].clear();
+    e->rho_nz[replica] = 0;
+  } else {
+    e->rho[replica].assign(rho, rho + (size_t)nz * e->n_types);
+    e->rho_nz[replica] = nz;
+  }
+  e->tail_dirty = true;
+  e->built = false;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_tail_mode(sre_engine *e, int mode) {
+  if (!e || (mode != SRE_TAIL_BIN && mode != SRE_TAIL_EXACT))
+    return fail(SRE_ERR_INVALID, "sre_set_tail_mode: bad mode");
+  e->tail_mode = mode;
+  e->tail_dirty = true;
+  e->built = false;
+  return SRE_OK;
+}
+
+// --------------------------------------------------------------- positions --
+
+extern "C" sre_status sre_upload_positions(sre_engine *e, const float *xyz,
+                                           int stride) {
+  if (!e || !e->system_set || !xyz || stride < 3)
+    return fail(SRE_ERR_INVALID, "sre_upload_positions: bad arguments");
+  HIP_TRY(hipSetDevice(e->device));
+  const size_t nf = (size_t)e->n_atoms * stride;
+  if (e->pos_own_floats < nf) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    dfree(e->d_pos_own);
+    HIP_TRY(dalloc(&e->d_pos_own, nf));
+    e->pos_own_floats = nf;
+    e->graph_epoch++;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_pos_own, xyz, nf * sizeof(float),
+                         hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->d_pos != e->d_pos_own || e->pos_stride != stride) e->graph_epoch++;
+  e->d_pos = e->d_pos_own;
+  e->pos_stride = stride;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_bind_positions(sre_engine *e, const void *d_xyz,
+                                         int stride) {
+  if (!e || !e->system_set || !d_xyz || stride < 3)
+    return fail(SRE_ERR_INVALID, "sre_bind_positions: bad arguments");
+  if (e->d_pos != d_xyz || e->pos_stride != stride) e->graph_epoch++;
+  e->d_pos = (const float *)d_xyz;
+  e->pos_stride = stride;
+  return SRE_OK;
+}
+
+// ---------------------------------------------------------------- rebuild --
+
+static int bitlen(unsigned v) {
+  int b = 0;
+  while (v) { ++b; v >>= 1; }
+  return b;
+}
+
+// Uploads rc lattices, chooses cell grids, uploads RepInfo.
+static sre_status prepare_tables(sre_engine *e) {
+  // rc lattices
+  if (e->rc_dirty) {
+    int total = 0;
+    for (int r = 0; r < e->n_rep; ++r) {
+      if (e->rc_tab[r].empty())
+        return fail(SRE_ERR_INVALID, "replica %d has no rc table", r);
+      total += (int)e->rc_tab[r].size();
+    }
+    std::vector<float> all;
+    all.reserve(total);
+    for (int r = 0; r < e->n_rep; ++r) {
+      RepInfo &R = e->reps[r];
+      R.rc_off = (int)all.size();
+      for (int d = 0; d < 3; ++d) R.rc_n[d] = e->rc_dims[3 * r + d];
+      R.rc_min = *std::min_element(e->rc_tab[r].begin(), e->rc_tab[r].end());
+      R.rc_max = *std::max_element(e->rc_tab[r].begin(), e->rc_tab[r].end());
+      all.insert(all.end(), e->rc_tab[r].begin(), e->rc_tab[r].end());
+    }
+    if (total > e->rc_total) {
+      dfree(e->d_rc);
+      HIP_TRY(dalloc(&e->d_rc, total));
+      e->graph_epoch++;
+    }
+    e->rc_total = total;
+    HIP_TRY(hipMemcpy(e->d_rc, all.data(), sizeof(float) * total,
+                      hipMemcpyHostToDevice));
+    e->rc_dirty = false;
+  }
+  // cell grids
+  long long cells = 0;
+  for (int r = 0; r < e->n_rep; ++r) {
+    RepInfo &R = e->reps[r];
+    const float rlmax = R.rc_max + e->skin;
+    for (int d = 0; d < 3; ++d)
+      if (!(2.f * rlmax < R.L[d]))
+        return fail(SRE_ERR_BOX_TOO_SMALL,
+                    "replica %d: rc_max + skin = %g is not below half the box "
+                    "edge %d (%g)", r, rlmax, d, R.L[d]);
+    const double rlmin = R.rc_min + e->skin;
+    double hxy = std::max(0.05, (double)e->tune.cell_xy) * rlmin;
+    double hz = std::max(0.05, (double)e->tune.cell_z) * rlmin;
+    int nc[3];
+    for (int tries = 0;; ++tries) {
+      nc[0] = std::max(1, (int)std::floor(R.L[0] / hxy));
+      nc[1] = std::max(1, (int)std::floor(R.L[1] / hxy));
+      nc[2] = std::max(1, (int)std::floor(R.L[2] / hz));
+      const long long c = (long long)nc[0] * nc[1] * nc[2];
+      if (c <= 8ll * R.n_atoms + 4096 || tries > 40) break;
+      hxy *= 1.1;
+      hz *= 1.1;
+    }
+    for (int d = 0; d < 3; ++d) {
+      R.nc[d] = nc[d];
+      R.h[d] = R.L[d] / (float)nc[d];
+    }
+    R.cell_off = (int)cells;
+    cells += (long long)nc[0] * nc[1] * nc[2];
+  }
+  if (cells >= (1ll << 31) - 2)
+    return fail(SRE_ERR_CELL, "cell grid too large (%lld cells)", cells);
+  e->n_cells = (int)cells;
+  e->key_bits = std::max(1, bitlen((unsigned)(cells - 1)));
+  if (e->key_bits > 32) return fail(SRE_ERR_CELL, "cell key exceeds 32 bits");
+  if (e->start_cap < e->n_cells + 1) {
+    dfree(e->d_start);
+    HIP_TRY(dalloc(&e->d_start, e->n_cells + 1));
+    e->start_cap = e->n_cells + 1;
+    e->graph_epoch++;
+  }
+  const size_t cc = radix_counts_capacity(e->n_atoms, e->key_bits);
+  if (cc > e->counts_cap) {
+    dfree(e->sb.counts);
+    dfree(e->sb.offsets);
+    dfree(e->sb.scan_tmp);
+    HIP_TRY(dalloc(&e->sb.counts, cc));
+    HIP_TRY(dalloc(&e->sb.offsets, cc));
+    HIP_TRY(dalloc(&e->sb.scan_tmp, cc / 4096 + 8));
+    e->counts_cap = cc;
+  }
+  // density profiles / tail tables
+  if (e->tail_dirty) {
+    std::vector<double> rho_all;
+    std::vector<int> entry_rep;
+    int bins = 0;
+    for (int r = 0; r < e->n_rep; ++r) {
+      RepInfo &R = e->reps[r];
+      R.tail_off = -1;
+      R.rho_off = -1;
+      R.rho_nz = 0;
+      if (e->rho_nz[r] > 0) {
+        R.rho_off = (int)rho_all.size();
+        R.rho_nz = e->rho_nz[r];
+        rho_all.insert(rho_all.end(), e->rho[r].begin(), e->rho[r].end());
+        if (R.rc_n[0] == 1 && R.rc_n[1] == 1) {
+          R.tail_off = bins;
+          for (int b = 0; b < R.rc_n[2]; ++b) entry_rep.push_back(r);
+          bins += R.rc_n[2];
+        } else {
+          return fail(SRE_ERR_INVALID,
+                      "replica %d: the tail correction needs an rc(z) table "
+                      "(nx = ny = 1)", r);
+        }
+      }
+    }
+    dfree(e->d_rho);
+    dfree(e->d_tail);
+    dfree(e->d_tail_entry_rep);
+    e->tail_bins = bins;
+    if (bins > 0) {
+      HIP_TRY(dalloc(&e->d_rho, rho_all.size()));
+      HIP_TRY(hipMemcpy(e->d_rho, rho_all.data(), sizeof(double) * rho_all.size(),
+                        hipMemcpyHostToDevice));
+      HIP_TRY(dalloc(&e->d_tail, (size_t)bins * e->n_types));
+      HIP_TRY(dalloc(&e->d_tail_entry_rep, bins));
+      HIP_TRY(hipMemcpy(e->d_tail_entry_rep, entry_rep.data(), sizeof(int) * bins,
+                        hipMemcpyHostToDevice));
+    }
+    e->graph_epoch++;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_reps, e->reps.data(), sizeof(RepInfo) * e->n_rep,
+                         hipMemcpyHostToDevice, e->stream));
+  if (e->tail_dirty) {
+    if (e->tail_bins > 0)
+      launch_tail_table(e->tail_bins * e->n_types, e->d_reps,
+                        e->d_tail_entry_rep, e->n_types, e->d_paramsd, e->d_rc,
+                        e->d_rho, e->d_tail, e->stream);
+    e->tail_dirty = false;
+  }
+  return SRE_OK;
+}
+
+static sre_status ensure_list(sre_engine *e, int cap) {
+  const int T = e->tune.force_lanes;
+  const int A = kWave / T;
+  const size_t waves = (size_t)(e->n_atoms + A - 1) / A;
+  const size_t need = waves * (size_t)cap * A;
+  if (need > e->list_size) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    dfree(e->d_list);
+    HIP_TRY(dalloc(&e->d_list, need));
+    e->list_size = need;
+    e->graph_epoch++;
+  }
+  e->cap = cap;
+  e->list_lanes = T;
+  return SRE_OK;
+}
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+static sre_status do_rebuild(sre_engine *e) {
+  if (!e->system_set) return fail(SRE_ERR_INVALID, "rebuild: system not set");
+  if (!e->types_set) return fail(SRE_ERR_INVALID, "rebuild: types not set");
+  if (!e->params_set) return fail(SRE_ERR_INVALID, "rebuild: pair parameters not set");
+  if (!e->d_pos) return fail(SRE_ERR_INVALID, "rebuild: positions not bound");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = prepare_tables(e);
+  if (s != SRE_OK) return s;
+  const int N = e->n_atoms;
+  hipStream_t st = e->stream;
+  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps, e->sb.key_a,
+              e->sb.val_a, st);
+  if (launch_radix_sort(N, e->key_bits, e->sb, st) != 0)
+    return fail(SRE_ERR_CELL, "radix sort: key of %d bits unsupported", e->key_bits);
+  launch_cell_start(e->n_cells, N, e->sb.key_a, e->d_start, st);
+  launch_gather_sorted(N, e->sb.val_a, e->d_pos, e->pos_stride, e->d_types,
+                       e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->d_pos4,
+                       e->d_backup, e->d_res4, e->d_rcut, e->d_tail_idx, st);
+  if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) {
+    if (!e->d_tail_atom) HIP_TRY(dalloc(&e->d_tail_atom, N));
+    launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
+                      e->d_paramsd, e->d_rho, e->d_tail_atom, st);
+  }
+  // list capacity: first guess from the densest possible shell, then exact
+  int cap = e->cap;
+  if (cap <= 0 || e->list_lanes != e->tune.force_lanes) {
+    double guess = 0;
+    for (const RepInfo &R : e->reps) {
+      const double rl = R.rc_max + e->skin;
+      const double rho = R.n_atoms / ((double)R.L[0] * R.L[1] * R.L[2]);
+      guess = std::max(guess, 4.18879 * rl * rl * rl * rho * 2.0 + 32.0);
+    }
+    cap = round_up((int)guess, e->tune.force_lanes);
+  }
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    s = ensure_list(e, cap);
+    if (s != SRE_OK) return s;
+    HIP_TRY(hipMemsetAsync(e->d_maxcount, 0, sizeof(int), st));
+    launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
+                      e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
+                      e->d_list, e->d_nnb, e->d_maxcount, st);
+    int mc = 0;
+    HIP_TRY(hipMemcpyAsync(&mc, e->d_maxcount, sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    e->max_count = mc;
+    if (mc <= e->cap) break;
+    cap = round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes);
+    if (attempt == 3)
+      return fail(SRE_ERR_LIST_OVERFLOW, "neighbor list overflow (%d > %d)", mc,
+                  e->cap);
+  }
+  HIP_TRY(hipMemsetAsync(e->d_flags, 0, sizeof(unsigned), st));
+  e->built = true;
+  e->n_list = -1;
+  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_rebuild(sre_engine *e) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  return do_rebuild(e);
+}
+
+extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "update before the first rebuild");
+  HIP_TRY(hipSetDevice(e->device));
+  launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
+                          e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                          e->d_flags, nullptr, e->stream);
+  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  if (need_rebuild) {
+    unsigned bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    float d2;
+    memcpy(&d2, &bits, sizeof d2);
+    const float half = 0.5f * e->skin;
+    *need_rebuild = d2 > half * half ? 1 : 0;
+  }
+  return SRE_OK;
+}
+
+// ---------------------------------------------------------------- compute --
+
+static ForceArgs force_args(sre_engine *e, bool atom_stats) {
+  ForceArgs a{};
+  a.n = e->n_atoms;
+  a.n_types = e->n_types;
+  a.cap = e->cap;
+  a.pos4 = e->d_pos4;
+  a.res4 = e->d_res4;
+  a.rcut = e->d_rcut;
+  a.nnb = e->d_nnb;
+  a.list = e->d_list;
+  a.reps = e->d_reps;
+  a.params = e->d_params;
+  a.paramsd = e->d_paramsd;
+  a.tail = e->d_tail;
+  a.tail_idx = e->d_tail_idx;
+  a.tail_atom = (e->tail_mode == SRE_TAIL_EXACT) ? e->d_tail_atom : nullptr;
+  a.fx = e->d_fx; a.fy = e->d_fy; a.fz = e->d_fz;
+  a.fxd = e->d_fxd; a.fyd = e->d_fyd; a.fzd = e->d_fzd;
+  a.wave_part = e->d_wave_part;
+  a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
+  a.err_flag = e->d_flags + 1;
+  return a;
+}
+
+static sre_status launch_compute(sre_engine *e, unsigned flags) {
+  const bool stats = flags & SRE_STATS;
+  const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
+  const int T = e->list_lanes;
+  if (stats) {
+    const size_t need = (size_t)force_num_waves(e->n_atoms, T) * 12;
+    if (need > e->wave_part_cap) {
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      dfree(e->d_wave_part);
+      HIP_TRY(dalloc(&e->d_wave_part, need));
+      e->wave_part_cap = need;
+      e->graph_epoch++;
+    }
+  }
+  ForceArgs a = force_args(e, stats);
+  launch_force(e->precision, T, e->tune.block, stats, tail, a, e->stream);
+  if (stats)
+    launch_stats_reduce(e->n_rep, e->d_reps, T, e->d_wave_part, e->d_stats,
+                        e->stream);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_compute(sre_engine *e, unsigned flags) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "compute before the first rebuild");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = launch_compute(e, flags);
+  if (s != SRE_OK) return s;
+  e->forces_valid = true;
+  e->stats_valid = (flags & SRE_STATS) != 0;
+  e->atom_stats_valid = e->stats_valid;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_step(sre_engine *e, unsigned flags, int *rebuilt) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  int need = 1;
+  sre_status s;
+  if (e->built) {
+    s = sre_update_positions(e, &need);
+    if (s != SRE_OK) return s;
+  }
+  if (need) {
+    s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+  }
+  if (rebuilt) *rebuilt = need;
+  return sre_compute(e, flags);
+}
+
+// ---------------------------------------------------------------- results --
+
+static sre_status check_errors(sre_engine *e) {
+  unsigned f = 0;
+  HIP_TRY(hipMemcpyAsync(&f, e->d_flags + 1, sizeof(unsigned),
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (f & 1u)
+    return fail(SRE_ERR_FORCE_RANGE,
+                "a pair force component exceeded 2^19 (overlapping atoms?)");
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_get_forces(sre_engine *e, double *dst,
+                                     int dst_on_device) {
+  if (!e || !dst) return fail(SRE_ERR_INVALID, "sre_get_forces: null argument");
+  if (!e->forces_valid) return fail(SRE_ERR_NOT_READY, "no forces computed");
+  HIP_TRY(hipSetDevice(e->device));
+  const int N = e->n_atoms;
+  double *out = dst_on_device ? dst : e->d_tmp_out;
+  launch_export_forces(N, e->sb.val_a, e->d_fx, e->d_fy, e->d_fz, e->d_fxd,
+                       e->d_fyd, e->d_fzd, out, nullptr, e->stream);
+  if (!dst_on_device) {
+    HIP_TRY(hipMemcpyAsync(dst, e->d_tmp_out, sizeof(double) * 3 * N,
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_errors(e);
+  }
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_add_forces_f32_soa(sre_engine *e, float *d_fx,
+                                             float *d_fy, float *d_fz) {
+  if (!e || !d_fx || !d_fy || !d_fz)
+    return fail(SRE_ERR_INVALID, "sre_add_forces_f32_soa: null argument");
+  if (!e->forces_valid) return fail(SRE_ERR_NOT_READY, "no forces computed");
+  HIP_TRY(hipSetDevice(e->device));
+  launch_add_forces_f32(e->n_atoms, e->sb.val_a, e->d_fx, e->d_fy, e->d_fz,
+                        e->d_fxd, e->d_fyd, e->d_fzd, d_fx, d_fy, d_fz,
+                        e->stream);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_get_forces_fixed(sre_engine *e, int64_t *dst) {
+  if (!e || !dst) return fail(SRE_ERR_INVALID, "sre_get_forces_fixed: null argument");
+  if (!e->forces_valid) return fail(SRE_ERR_NOT_READY, "no forces computed");
+  if (e->precision != SRE_PREC_MIXED)
+    return fail(SRE_ERR_INVALID, "fixed-point forces exist only in mixed precision");
+  HIP_TRY(hipSetDevice(e->device));
+  const int N = e->n_atoms;
+  int64_t *tmp = (int64_t *)e->d_tmp_out;
+  launch_export_forces(N, e->sb.val_a, e->d_fx, e->d_fy, e->d_fz, nullptr,
+                       nullptr, nullptr, nullptr, tmp, e->stream);
+  HIP_TRY(hipMemcpyAsync(dst, tmp, sizeof(int64_t) * 3 * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return check_errors(e);
+}
+
+extern "C" sre_status sre_get_stats(sre_engine *e, double *out) {
+  if (!e || !out) return fail(SRE_ERR_INVALID, "sre_get_stats: null argument");
+  if (!e->stats_valid) return fail(SRE_ERR_NOT_READY, "no SRE_STATS compute yet");
+  HIP_TRY(hipSetDevice(e->device));
+  std::vector<double> st(5 * (size_t)e->n_rep);
+  HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int r = 0; r < e->n_rep; ++r)
+    for (int c = 0; c < 4; ++c) out[4 * r + c] = st[5 * r + c];
+  return check_errors(e);
+}
+
+extern "C" sre_status sre_get_atom_stats(sre_engine *e, double *out) {
+  if (!e || !out) return fail(SRE_ERR_INVALID, "sre_get_atom_stats: null argument");
+  if (!e->atom_stats_valid) return fail(SRE_ERR_NOT_READY, "no SRE_STATS compute yet");
+  HIP_TRY(hipSetDevice(e->device));
+  const int N = e->n_atoms;
+  launch_export_atom_stats(N, e->sb.val_a, e->d_atom_stats, e->d_tmp_out,
+                           e->stream);
+  HIP_TRY(hipMemcpyAsync(out, e->d_tmp_out, sizeof(double) * 4 * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list,
+                                         int64_t *n_in_rc, int32_t *max_list) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "no neighbor list");
+  HIP_TRY(hipSetDevice(e->device));
+  if (n_list) {
+    if (e->n_list < 0) {
+      std::vector<int> nn(e->n_atoms);
+      HIP_TRY(hipMemcpyAsync(nn.data(), e->d_nnb, sizeof(int) * e->n_atoms,
+                             hipMemcpyDeviceToHost, e->stream));
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      int64_t t = 0;
+      for (int v : nn) t += v;
+      e->n_list = t;
+    }
+    *n_list = e->n_list;
+  }
+  if (n_in_rc) {
+    *n_in_rc = -1;
+    if (e->stats_valid) {
+      std::vector<double> st(5 * (size_t)e->n_rep);
+      HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
+                             hipMemcpyDeviceToHost, e->stream));
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      double t = 0;
+      for (int r = 0; r < e->n_rep; ++r) t += st[5 * r + 4];
+      *n_in_rc = (int64_t)llround(t);
+    }
+  }
+  if (max_list) *max_list = e->max_count;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_export_pairs(sre_engine *e, int64_t *offsets,
+                                       int32_t *j) {
+  if (!e || !offsets) return fail(SRE_ERR_INVALID, "sre_export_pairs: null argument");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "no neighbor list");
+  HIP_TRY(hipSetDevice(e->device));
+  const int N = e->n_atoms;
+  std::vector<int> nn(N), list(e->list_size);
+  std::vector<uint32_t> perm(N);
+  HIP_TRY(hipMemcpyAsync(nn.data(), e->d_nnb, sizeof(int) * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(perm.data(), e->sb.val_a, sizeof(uint32_t) * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(list.data(), e->d_list, sizeof(int) * e->list_size,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  std::vector<int64_t> cnt(N + 1, 0);
+  for (int s = 0; s < N; ++s) cnt[perm[s] + 1] = nn[s];
+  for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+  for (int i = 0; i <= N; ++i) offsets[i] = cnt[i];
+  if (j) {
+    for (int s = 0; s < N; ++s) {
+      const int i = (int)perm[s];
+      int32_t *row = j + cnt[i];
+      for (int k = 0; k < nn[s]; ++k)
+        row[k] = (int32_t)perm[list[list_slot(s, k, e->list_lanes, e->cap)]];
+      std::sort(row, row + nn[s]);
+    }
+  }
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_get_rcut(sre_engine *e, float *rc) {
+  if (!e || !rc) return fail(SRE_ERR_INVALID, "sre_get_rcut: null argument");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "no rc assigned yet");
+  HIP_TRY(hipSetDevice(e->device));
+  const int N = e->n_atoms;
+  std::vector<float> r(N);
+  std::vector<uint32_t> perm(N);
+  HIP_TRY(hipMemcpyAsync(r.data(), e->d_rcut, sizeof(float) * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(perm.data(), e->sb.val_a, sizeof(uint32_t) * N,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int s = 0; s < N; ++s) rc[perm[s]] = r[s];
+  return SRE_OK;
+}
+
+// ---------------------------------------------------------------- adjoint --
+
+extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
+                                  const double *d_dLdF, unsigned flags,
+                                  double *g_eps, double *g_sigma,
+                                  double *g_rc) {
+  if (!e || !g_eps || !g_sigma)
+    return fail(SRE_ERR_INVALID, "sre_adjoint: null argument");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "adjoint before the first rebuild");
+  const bool use_tail = (flags & SRE_TAIL) && e->tail_bins > 0;
+  if (use_tail && e->tail_mode != SRE_TAIL_BIN)
+    return fail(SRE_ERR_INVALID, "adjoint of the tail term needs SRE_TAIL_BIN");
+  HIP_TRY(hipSetDevice(e->device));
+  const int nt = e->n_types, npt = nt * nt;
+  const int gw = adjoint_grid_waves(e->n_atoms, e->list_lanes);
+  const int n_entries = std::max(1, e->tail_bins * nt);
+  const int rc_n = std::max(1, e->rc_total);
+  double *d_wp = nullptr, *d_dLdE = nullptr, *d_out = nullptr,
+         *d_tout = nullptr, *d_contrib = nullptr, *d_grc = nullptr;
+  int64_t *d_tw = nullptr;
+  sre_status s = SRE_OK;
+  hipError_t r = dalloc(&d_wp, (size_t)gw * 2 * npt);
+  if (r == hipSuccess) r = dalloc(&d_dLdE, e->n_rep);
+  if (r == hipSuccess) r = dalloc(&d_out, 2 * npt);
+  if (r == hipSuccess) r = dalloc(&d_tout, 2 * npt);
+  if (r == hipSuccess) r = dalloc(&d_contrib, (size_t)std::max(1, e->tail_bins) * 2 * npt);
+  if (r == hipSuccess) r = dalloc(&d_grc, rc_n);
+  if (r == hipSuccess) r = dalloc(&d_tw, (size_t)2 * n_entries);
+  std::vector<double> hE(e->n_rep, 0.0);
+  if (dLdE) hE.assign(dLdE, dLdE + e->n_rep);
+  std::vector<double> zero(std::max(rc_n, 2 * npt), 0.0);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(d_dLdE, hE.data(), sizeof(double) * e->n_rep,
+                       hipMemcpyHostToDevice, e->stream);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(d_grc, zero.data(), sizeof(double) * rc_n,
+                       hipMemcpyHostToDevice, e->stream);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(d_tout, zero.data(), sizeof(double) * 2 * npt,
+                       hipMemcpyHostToDevice, e->stream);
+  std::vector<double> out(2 * npt), tout(2 * npt), grc(rc_n);
+  if (r == hipSuccess) {
+    launch_fill_i64(d_tw, 0, (size_t)2 * n_entries, e->stream);
+    AdjointArgs a{};
+    a.n = e->n_atoms;
+    a.n_types = nt;
+    a.cap = e->cap;
+    a.n_rep = e->n_rep;
+    a.pos4 = e->d_pos4;
+    a.rcut = e->d_rcut;
+    a.nnb = e->d_nnb;
+    a.list = e->d_list;
+    a.perm = e->sb.val_a;
+    a.reps = e->d_reps;
+    a.params = e->d_params;
+    a.dLdE = d_dLdE;
+    a.dLdF = d_dLdF;
+    a.tail_idx = e->d_tail_idx;
+    a.use_tail = use_tail ? 1 : 0;
+    a.wave_part = d_wp;
+    a.tail_w = d_tw;
+    launch_adjoint_pairs(a, e->precision, e->list_lanes, d_out, e->stream);
+    if (use_tail)
+      launch_adjoint_tail(e->tail_bins, e->d_tail_entry_rep, e->d_reps, nt,
+                          e->d_paramsd, e->d_rc, e->d_rho, d_tw, d_contrib,
+                          d_grc, d_tout, e->stream);
+    r = hipMemcpyAsync(out.data(), d_out, sizeof(double) * 2 * npt,
+                       hipMemcpyDeviceToHost, e->stream);
+  }
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(tout.data(), d_tout, sizeof(double) * 2 * npt,
+                       hipMemcpyDeviceToHost, e->stream);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(grc.data(), d_grc, sizeof(double) * rc_n,
+                       hipMemcpyDeviceToHost, e->stream);
+  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+  if (r != hipSuccess) s = fail(SRE_ERR_HIP, "adjoint: %s", hipGetErrorString(r));
+  dfree(d_wp); dfree(d_dLdE); dfree(d_out); dfree(d_tout); dfree(d_contrib);
+  dfree(d_grc); dfree(d_tw);
+  if (s != SRE_OK) return s;
+  // Values are per ordered type pair; a symmetric parameter (a,b) = (b,a)
+  // collects both orders.
+  for (int pa = 0; pa < nt; ++pa)
+    for (int pb = 0; pb < nt; ++pb) {
+      const int ab = pa * nt + pb, ba = pb * nt + pa;
+      double ge = out[ab] + tout[ab], gs = out[npt + ab] + tout[npt + ab];
+      if (ab != ba) {
+        ge += out[ba] + tout[ba];
+        gs += out[npt + ba] + tout[npt + ba];
+      }
+      g_eps[ab] = ge;
+      g_sigma[ab] = gs;
+    }
+  if (g_rc)
+    for (int k = 0; k < e->rc_total; ++k) g_rc[k] = grc[k];
+  return SRE_OK;
+}
+
+// ----------------------------------------------------------------- tuning --
+
+extern "C" sre_status sre_get_tuning(sre_engine *e, sre_tuning *t) {
+  if (!e || !t) return fail(SRE_ERR_INVALID, "null argument");
+  *t = e->tune;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
+  if (!e || !t) return fail(SRE_ERR_INVALID, "null argument");
+  if (!pow2_lanes(t->force_lanes) || !pow2_lanes(t->build_lanes))
+    return fail(SRE_ERR_INVALID, "lanes must be 1, 2, 4, 8 or 16");
+  if (t->block < 64 || t->block > 1024 || t->block % 64)
+    return fail(SRE_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+  if (!(t->cell_xy >= 0.05f && t->cell_xy <= 4.f && t->cell_z >= 0.05f &&
+        t->cell_z <= 4.f))
+    return fail(SRE_ERR_INVALID, "cell sizes must be in [0.05, 4]");
+  const bool layout = t->force_lanes != e->tune.force_lanes ||
+                      t->build_lanes != e->tune.build_lanes ||
+                      t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
+  e->tune = *t;
+  if (layout) {
+    e->built = false;
+    e->cap = 0;
+    e->graph_epoch++;
+  }
+  return SRE_OK;
+}
+
+static sre_status time_region(sre_engine *e, int n_iter, double *ms,
+                              sre_status (*body)(sre_engine *, unsigned),
+                              unsigned flags) {
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  HIP_TRY(hipEventRecord(a, e->stream));
+  sre_status s = SRE_OK;
+  for (int i = 0; i < n_iter && s == SRE_OK; ++i) s = body(e, flags);
+  HIP_TRY(hipEventRecord(b, e->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = (double)t / n_iter;
+  return s;
+}
+
+// the force kernel alone (no statistic reduction), for per-launch timing
+static sre_status body_compute(sre_engine *e, unsigned flags) {
+  const bool stats = flags & SRE_STATS;
+  if (stats) {
+    const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
+    if (need > e->wave_part_cap) {
+      HIP_TRY(hipStreamSynchronize(e->stream));
+      dfree(e->d_wave_part);
+      HIP_TRY(dalloc(&e->d_wave_part, need));
+      e->wave_part_cap = need;
+      e->graph_epoch++;
+    }
+  }
+  ForceArgs a = force_args(e, stats);
+  launch_force(e->precision, e->list_lanes, e->tune.block, stats,
+               (flags & SRE_TAIL) && e->tail_bins > 0, a, e->stream);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_time_compute(sre_engine *e, unsigned flags,
+                                       int n_iter, double *ms_per_iter) {
+  if (!e || !ms_per_iter || n_iter < 1) return fail(SRE_ERR_INVALID, "bad arguments");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "compute before the first rebuild");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = time_region(e, n_iter, ms_per_iter, body_compute, flags);
+  if (s == SRE_OK) {
+    e->forces_valid = true;
+    e->stats_valid = false;  // the statistic reduction was not run
+    e->atom_stats_valid = (flags & SRE_STATS) != 0;
+  }
+  return s;
+}
+
+static sre_status body_rebuild(sre_engine *e, unsigned) { return do_rebuild(e); }
+
+extern "C" sre_status sre_time_rebuild(sre_engine *e, int n_iter,
+                                       double *ms_per_iter) {
+  if (!e || !ms_per_iter || n_iter < 1) return fail(SRE_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(e->device));
+  return time_region(e, n_iter, ms_per_iter, body_rebuild, 0);
+}
+
+extern "C" sre_status sre_synchronize(sre_engine *e) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
+                                   const char *cache_path) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  HIP_TRY(hipSetDevice(e->device));
+  char sig[512];
+  {
+    double lsum = 0, rcs = 0;
+    for (int r = 0; r < e->n_rep; ++r) {
+      lsum += (double)e->box[3 * r] + 3.0 * e->box[3 * r + 1] + 7.0 * e->box[3 * r + 2];
+      for (float v : e->rc_tab[r]) rcs += v;
+    }
+    snprintf(sig, sizeof sig, "N=%d R=%d T=%d prec=%d flags=%u skin=%.4f L=%.6g rc=%.6g",
+             e->n_atoms, e->n_rep, e->n_types, e->precision, flags,
+             (double)e->skin, lsum, rcs);
+  }
+  if (cache_path) {
+    FILE *f = fopen(cache_path, "r");
+    if (f) {
+      char line[1024];
+      while (fgets(line, sizeof line, f)) {
+        char *bar = strchr(line, '|');
+        if (!bar) continue;
+        *bar = 0;
+        if (strcmp(line, sig) == 0) {
+          sre_tuning t;
+          if (sscanf(bar + 1, "%d %d %d %f %f", &t.force_lanes, &t.build_lanes,
+                     &t.block, &t.cell_xy, &t.cell_z) == 5) {
+            fclose(f);
+            sre_status s = sre_set_tuning(e, &t);
+            if (s != SRE_OK) return s;
+            return do_rebuild(e);
+          }
+        }
+      }
+      fclose(f);
+    }
+  }
+  // phase 1: binning + list build
+  const float cells_xy[] = {0.35f, 0.5f, 0.7f, 1.0f};
+  const float cells_z[] = {0.25f, 0.35f, 0.5f, 1.0f};
+  const int build_lanes[] = {4, 8, 16};
+  sre_tuning best = e->tune;
+  double best_ms = 1e30;
+  for (float cxy : cells_xy)
+    for (float cz : cells_z)
+      for (int bl : build_lanes) {
+        sre_tuning t = e->tune;
+        t.cell_xy = cxy;
+        t.cell_z = cz;
+        t.build_lanes = bl;
+        sre_status s = sre_set_tuning(e, &t);
+        if (s != SRE_OK) return s;
+        s = do_rebuild(e);
+        if (s != SRE_OK) return s;
+        double ms;
+        s = sre_time_rebuild(e, 3, &ms);
+        if (s != SRE_OK) return s;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+  // phase 2: force kernel lanes and block
+  const int force_lanes[] = {1, 2, 4, 8, 16};
+  const int blocks[] = {64, 128, 256, 512};
+  double best_f = 1e30;
+  sre_tuning bestf = best;
+  std::vector<int64_t> ref, cur;
+  for (int fl : force_lanes)
+    for (int bk : blocks) {
+      sre_tuning t = best;
+      t.force_lanes = fl;
+      t.block = bk;
+      sre_status s = sre_set_tuning(e, &t);
+      if (s != SRE_OK) return s;
+      s = do_rebuild(e);
+      if (s != SRE_OK) return s;
+      double ms;
+      s = sre_time_compute(e, flags, 2, &ms);
+      if (s != SRE_OK) return s;
+      s = sre_time_compute(e, flags, 10, &ms);
+      if (s != SRE_OK) return s;
+      if (e->precision == SRE_PREC_MIXED) {
+        cur.resize((size_t)3 * e->n_atoms);
+        s = sre_get_forces_fixed(e, cur.data());
+        if (s != SRE_OK) return s;
+        if (ref.empty()) ref = cur;
+        else if (ref != cur)
+          return fail(SRE_ERR_INVALID,
+                      "autotune: forces differ between lane counts (bug)");
+      }
+      if (ms < best_f) { best_f = ms; bestf = t; }
+    }
+  sre_status s = sre_set_tuning(e, &bestf);
+  if (s != SRE_OK) return s;
+  s = do_rebuild(e);
+  if (s != SRE_OK) return s;
+  if (cache_path) {
+    FILE *f = fopen(cache_path, "a");
+    if (f) {
+      fprintf(f, "%s|%d %d %d %.4f %.4f\n", sig, bestf.force_lanes,
+              bestf.build_lanes, bestf.block, (double)bestf.cell_xy,
+              (double)bestf.cell_z);
+      fclose(f);
+    }
+  }
+  return SRE_OK;
+}
+
+// ----------------------------------------------------------------- graphs --
+
+extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) {
+  if (!e || n_steps < 0) return fail(SRE_ERR_INVALID, "bad arguments");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
+  HIP_TRY(hipSetDevice(e->device));
+  if (flags & SRE_STATS) {
+    // make sure the wave partial buffer exists before capture
+    const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
+    if (need > e->wave_part_cap) {
+      dfree(e->d_wave_part);
+      HIP_TRY(dalloc(&e->d_wave_part, need));
+      e->wave_part_cap = need;
+      e->graph_epoch++;
+    }
+  }
+  if (!e->graph_exec || e->graph_flags != flags ||
+      e->graph_epoch_built != e->graph_epoch) {
+    if (e->graph_exec) {
+      (void)hipGraphExecDestroy(e->graph_exec);
+      e->graph_exec = nullptr;
+    }
+    hipGraph_t g;
+    const bool pert = (flags & SRE_PERTURB) != 0;
+    if (pert && !e->d_step_ctr)
+      return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    if (pert)
+      launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), e->pos_stride,
+                     e->pert_sigma, e->pert_seed, 0, e->d_step_ctr, e->stream);
+    launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
+                            e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                            e->d_flags, pert ? e->d_step_ctr : nullptr,
+                            e->stream);
+    sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL));
+    hipError_t r = hipStreamEndCapture(e->stream, &g);
+    if (s != SRE_OK) return s;
+    if (r != hipSuccess)
+      return fail(SRE_ERR_HIP, "graph capture: %s", hipGetErrorString(r));
+    r = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess)
+      return fail(SRE_ERR_HIP, "graph instantiate: %s", hipGetErrorString(r));
+    e->graph_flags = flags;
+    e->graph_epoch_built = e->graph_epoch;
+  }
+  for (int i = 0; i < n_steps; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+  e->forces_valid = true;
+  e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_check_skin_flag(sre_engine *e) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  HIP_TRY(hipSetDevice(e->device));
+  unsigned bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  float d2;
+  memcpy(&d2, &bits, sizeof d2);
+  const float half = 0.5f * e->skin;
+  if (d2 > half * half)
+    return fail(SRE_ERR_SKIN, "max displacement %g exceeds skin/2 = %g",
+                sqrt((double)d2), (double)half);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_perturb_positions(sre_engine *e, float sigma,
+                                            uint64_t seed, uint64_t step) {
+  if (!e || !e->d_pos) return fail(SRE_ERR_INVALID, "no positions bound");
+  HIP_TRY(hipSetDevice(e->device));
+  launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), e->pos_stride,
+                 sigma, seed, step, nullptr, e->stream);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_perturbation(sre_engine *e, float sigma,
+                                           uint64_t seed, uint64_t first_step) {
+  if (!e || !(sigma >= 0.f)) return fail(SRE_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(e->device));
+  if (!e->d_step_ctr) {
+    HIP_TRY(dalloc(&e->d_step_ctr, 1));
+    e->graph_epoch++;
+  }
+  HIP_TRY(hipMemcpyAsync(e->d_step_ctr, &first_step, sizeof(uint64_t),
+                         hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->pert_sigma != sigma || e->pert_seed != seed) e->graph_epoch++;
+  e->pert_sigma = sigma;
+  e->pert_seed = seed;
+  return SRE_OK;
+}

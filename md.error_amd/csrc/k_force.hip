@@ -1,0 +1,483 @@
+// k_force.hip -- per-step kernels: position refresh + skin test, the LJ
+// force/energy/virial kernels (mixed and FP64), the statistic reduction and
+// the force export.
+//
+// Force semantics (H/src/InteractionEngine_interface.cu:1204-1281 with
+// H/include/NonBondedInteraction.h:866-889):
+//   for k < Nneighbor[i]: d = minimg(x_j - x_i); if |d|^2 < rc_i^2:
+//       F_i += d * (-4 eps (12 s^12 - 6 s^6) / r^2)
+//       E_i += 4 eps (s^12 - s^6) + shift ;  W_i,aa += F_a d_a
+//   stored per atom: E_i / 2 and W_i / 2.
+// The list is full (no Newton's third law), so every atom owns its sums and no
+// atomics touch forces.  A group of T lanes shares one atom's list; entry k of
+// the atom is read by lane k % T at row k / T of the wave's tile, so each list
+// row is one coalesced 256-byte load.
+#include "sre_internal.h"
+
+namespace sre {
+
+// ----------------------------------------------------- update + skin test --
+
+__global__ __launch_bounds__(256) void k_update_positions(
+    int n, const uint32_t *__restrict__ perm, const float *__restrict__ pos,
+    int stride, const RepInfo *__restrict__ reps, uint4 *__restrict__ pos4,
+    const uint4 *__restrict__ backup, float4 *__restrict__ res4,
+    unsigned *skin_flag, uint64_t *step_ctr) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (step_ctr && s == 0) *step_ctr += 1;  // the perturbation ran before us
+  float d2 = 0.f;
+  if (s < n) {
+    const uint4 b = backup[s];
+    const RepInfo &R = reps[b.w >> 8];
+    const float *p = pos + (size_t)perm[s] * stride;
+    const uint32_t ux = quantize(p[0], R.invq[0]);
+    const uint32_t uy = quantize(p[1], R.invq[1]);
+    const uint32_t uz = quantize(p[2], R.invq[2]);
+    pos4[s] = make_uint4(ux, uy, uz, b.w);
+    if (res4)
+      res4[s] = make_float4(quant_residual(p[0], R.invq[0], R.qd[0]),
+                            quant_residual(p[1], R.invq[1], R.qd[1]),
+                            quant_residual(p[2], R.invq[2], R.qd[2]), 0.f);
+    d2 = dist2(disp(ux, b.x, R.q[0]), disp(uy, b.y, R.q[1]),
+               disp(uz, b.z, R.q[2]));
+  }
+  // per-wavefront maximum, one atomic per wave (max is order independent)
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0 && d2 > 0.f)
+    atomicMax(skin_flag, __float_as_uint(d2));
+}
+
+void launch_update_positions(int n, const uint32_t *perm, const float *pos,
+                             int stride, const RepInfo *reps, uint4 *pos4,
+                             const uint4 *backup, float4 *res4,
+                             unsigned *skin_flag, uint64_t *step_ctr,
+                             hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_positions, dim3((n + 255) / 256), dim3(256), 0,
+                     st, n, perm, pos, stride, reps, pos4, backup, res4,
+                     skin_flag, step_ctr);
+}
+
+// ------------------------------------------------------ wave reductions --
+
+template <typename V>
+__device__ __forceinline__ V group_sum(V v, int T) {
+  for (int o = T >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Writes the two replica slots of one wave's statistic partials.
+// vals[5] are this lane's group totals (only lanes with l == 0 contribute).
+__device__ __forceinline__ void write_wave_stats(double *wp, int wave,
+                                                 int rep, int rep0, bool own,
+                                                 const double vals[5]) {
+  double out0[5], out1[5];
+  const bool in0 = own && rep == rep0, in1 = own && rep != rep0;
+  int r1 = in1 ? rep : -1;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    out0[c] = wave_sum(in0 ? vals[c] : 0.0);
+    out1[c] = wave_sum(in1 ? vals[c] : 0.0);
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) r1 = max(r1, __shfl_xor(r1, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    double *d = wp + (size_t)wave * 12;
+    d[0] = (double)rep0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) d[1 + c] = out0[c];
+    d[6] = (double)r1;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) d[7 + c] = out1[c];
+  }
+}
+
+// -------------------------------------------------------------- mixed ----
+
+template <int T, bool STATS, bool TAIL, bool MULTI>
+__global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
+  __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
+  if (MULTI) {
+    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
+      sp[t] = a.params[t];
+    __syncthreads();
+  }
+  constexpr int A = kWave / T;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l = lane % T;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int s = wave * A + lane / T;
+  const bool act = s < a.n;
+  const int s0 = wave * A;
+  if (s0 >= a.n) return;  // whole wave beyond the atoms
+
+  uint64_t ax = 0, ay = 0, az = 0;
+  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fmx = 0.f;
+  int nin = 0, iters = 0, rep = 0;
+  if (act) {
+    const uint4 pi = a.pos4[s];
+    rep = (int)(pi.w >> 8);
+    const RepInfo &R = a.reps[rep];
+    const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+    const float rc = a.rcut[s];
+    const float rc2 = rc * rc;
+    const int nn = a.nnb[s];
+    const int ti = (int)(pi.w & 0xffu) * a.n_types;
+    iters = (nn + T - 1) / T;
+    PairParam P = a.params[0];
+    const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
+    for (int m = 0; m < iters; ++m) {
+      const int k = m * T + l;
+      const bool v = k < nn;
+      const int j = v ? lp[(size_t)m * kWave] : s;
+      const uint4 pj = a.pos4[j];
+      const float dx = disp(pj.x, pi.x, qx);
+      const float dy = disp(pj.y, pi.y, qy);
+      const float dz = disp(pj.z, pi.z, qz);
+      const float d2 = dist2(dx, dy, dz);
+      const bool in = v && (d2 < rc2);
+      if (MULTI) P = sp[ti + (int)(pj.w & 0xffu)];
+      const float ri2 = __builtin_amdgcn_rcpf(d2);
+      const float ri6 = ri2 * ri2 * ri2;
+      float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
+      fs = in ? fs : 0.f;
+      const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
+      ax += fix_bits((double)fpx);
+      ay += fix_bits((double)fpy);
+      az += fix_bits((double)fpz);
+      fmx = fmaxf(fmx, fmaxf(fabsf(fpx), fmaxf(fabsf(fpy), fabsf(fpz))));
+      if (STATS) {
+        const float ep = __builtin_fmaf(ri6, __builtin_fmaf(P.A, ri6, -P.B), P.shift);
+        e += in ? ep : 0.f;
+        wx = __builtin_fmaf(fpx, dx, wx);
+        wy = __builtin_fmaf(fpy, dy, wy);
+        wz = __builtin_fmaf(fpz, dz, wz);
+        nin += in ? 1 : 0;
+      }
+    }
+  }
+  const uint64_t mb = (uint64_t)iters * magic_bits();
+  int64_t Fx = (int64_t)(ax - mb), Fy = (int64_t)(ay - mb), Fz = (int64_t)(az - mb);
+  Fx = group_sum(Fx, T);
+  Fy = group_sum(Fy, T);
+  Fz = group_sum(Fz, T);
+  if (__ballot(fmx >= kPairForceLimit) && lane == 0) atomicOr(a.err_flag, 1u);
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0, Nin = 0.0;
+  if (STATS) {
+    E = group_sum((double)e, T) * 0.5;
+    Wx = group_sum((double)wx, T) * 0.5;
+    Wy = group_sum((double)wy, T) * 0.5;
+    Wz = group_sum((double)wz, T) * 0.5;
+    Nin = (double)group_sum(nin, T);
+  }
+  const bool own = act && l == 0;
+  if (own) {
+    if (TAIL) {
+      double2 tv = make_double2(0.0, 0.0);
+      if (a.tail_atom) {
+        tv = a.tail_atom[s];
+      } else {
+        const int ti = a.tail_idx[s];
+        if (ti >= 0) tv = a.tail[ti];
+      }
+      Fz += to_fix(tv.x);
+      E += tv.y;
+    }
+    a.fx[s] = Fx;
+    a.fy[s] = Fy;
+    a.fz[s] = Fz;
+    if (STATS && a.atom_stats) {
+      double *d = a.atom_stats + (size_t)s * 4;
+      d[0] = E; d[1] = Wx; d[2] = Wy; d[3] = Wz;
+    }
+  }
+  if (STATS) {
+    const int rep0 = (int)(a.pos4[s0].w >> 8);
+    const double vals[5] = {E, Wx, Wy, Wz, Nin};
+    write_wave_stats(a.wave_part, wave, rep, rep0, own, vals);
+  }
+}
+
+// --------------------------------------------------------------- FP64 ----
+
+template <int T, bool STATS, bool TAIL, bool MULTI>
+__global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
+  __shared__ PairParamD sp[MULTI ? kMaxPairTypes : 1];
+  if (MULTI) {
+    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
+      sp[t] = a.paramsd[t];
+    __syncthreads();
+  }
+  constexpr int A = kWave / T;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l = lane % T;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int s = wave * A + lane / T;
+  const bool act = s < a.n;
+  const int s0 = wave * A;
+  if (s0 >= a.n) return;
+
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  int nin = 0, rep = 0;
+  if (act) {
+    const uint4 pi = a.pos4[s];
+    rep = (int)(pi.w >> 8);
+    const RepInfo &R = a.reps[rep];
+    const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+    const double qdx = R.qd[0], qdy = R.qd[1], qdz = R.qd[2];
+    const float4 ri = a.res4[s];
+    const float rc = a.rcut[s];
+    const float rc2 = rc * rc;
+    const int nn = a.nnb[s];
+    const int ti = (int)(pi.w & 0xffu) * a.n_types;
+    const int iters = (nn + T - 1) / T;
+    PairParamD P = a.paramsd[0];
+    const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
+    for (int m = 0; m < iters; ++m) {
+      const int k = m * T + l;
+      const bool v = k < nn;
+      const int j = v ? lp[(size_t)m * kWave] : s;
+      const uint4 pj = a.pos4[j];
+      // the pair set is decided by the FP32 predicate shared with the mixed path
+      const float d2f = dist2(disp(pj.x, pi.x, qx), disp(pj.y, pi.y, qy),
+                              disp(pj.z, pi.z, qz));
+      const bool in = v && (d2f < rc2);
+      if (MULTI) P = sp[ti + (int)(pj.w & 0xffu)];
+      const float4 rj = a.res4[j];
+      const double dx = (double)(int32_t)(pj.x - pi.x) * qdx + ((double)rj.x - (double)ri.x);
+      const double dy = (double)(int32_t)(pj.y - pi.y) * qdy + ((double)rj.y - (double)ri.y);
+      const double dz = (double)(int32_t)(pj.z - pi.z) * qdz + ((double)rj.z - (double)ri.z);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double ri2 = in ? 1.0 / d2 : 0.0;
+      const double ri6 = ri2 * ri2 * ri2;
+      const double fs = ri2 * ri6 * (P.B6 - P.A12 * ri6);
+      fx += fs * dx;
+      fy += fs * dy;
+      fz += fs * dz;
+      if (STATS) {
+        e += in ? ri6 * (P.A * ri6 - P.B) + P.shift : 0.0;
+        wx += fs * dx * dx;
+        wy += fs * dy * dy;
+        wz += fs * dz * dz;
+        nin += in ? 1 : 0;
+      }
+    }
+  }
+  fx = group_sum(fx, T);
+  fy = group_sum(fy, T);
+  fz = group_sum(fz, T);
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0, Nin = 0.0;
+  if (STATS) {
+    E = group_sum(e, T) * 0.5;
+    Wx = group_sum(wx, T) * 0.5;
+    Wy = group_sum(wy, T) * 0.5;
+    Wz = group_sum(wz, T) * 0.5;
+    Nin = (double)group_sum(nin, T);
+  }
+  const bool own = act && l == 0;
+  if (own) {
+    if (TAIL) {
+      double2 tv = make_double2(0.0, 0.0);
+      if (a.tail_atom) {
+        tv = a.tail_atom[s];
+      } else {
+        const int ti = a.tail_idx[s];
+        if (ti >= 0) tv = a.tail[ti];
+      }
+      fz += tv.x;
+      E += tv.y;
+    }
+    a.fxd[s] = fx;
+    a.fyd[s] = fy;
+    a.fzd[s] = fz;
+    if (STATS && a.atom_stats) {
+      double *d = a.atom_stats + (size_t)s * 4;
+      d[0] = E; d[1] = Wx; d[2] = Wy; d[3] = Wz;
+    }
+  }
+  if (STATS) {
+    const int rep0 = (int)(a.pos4[s0].w >> 8);
+    const double vals[5] = {E, Wx, Wy, Wz, Nin};
+    write_wave_stats(a.wave_part, wave, rep, rep0, own, vals);
+  }
+}
+
+int force_num_waves(int n, int lanes) {
+  const int A = kWave / lanes;
+  return (n + A - 1) / A;
+}
+
+template <int T, bool STATS, bool TAIL>
+static void launch_force_t(int precision, int block, const ForceArgs &a,
+                           hipStream_t st) {
+  const int waves = force_num_waves(a.n, T);
+  const int grid = (waves * kWave + block - 1) / block;
+  const bool multi = a.n_types > 1;
+  if (precision == 0) {
+    if (multi)
+      hipLaunchKernelGGL((k_force_mixed<T, STATS, TAIL, true>), dim3(grid),
+                         dim3(block), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_force_mixed<T, STATS, TAIL, false>), dim3(grid),
+                         dim3(block), 0, st, a);
+  } else {
+    if (multi)
+      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, true>), dim3(grid),
+                         dim3(block), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, false>), dim3(grid),
+                         dim3(block), 0, st, a);
+  }
+}
+
+template <int T>
+static void launch_force_lanes(int precision, int block, bool stats, bool tail,
+                               const ForceArgs &a, hipStream_t st) {
+  if (stats) {
+    if (tail) launch_force_t<T, true, true>(precision, block, a, st);
+    else launch_force_t<T, true, false>(precision, block, a, st);
+  } else {
+    if (tail) launch_force_t<T, false, true>(precision, block, a, st);
+    else launch_force_t<T, false, false>(precision, block, a, st);
+  }
+}
+
+void launch_force(int precision, int lanes, int block, bool stats, bool tail,
+                  const ForceArgs &a, hipStream_t st) {
+  if (a.n <= 0) return;
+  switch (lanes) {
+    case 1: launch_force_lanes<1>(precision, block, stats, tail, a, st); break;
+    case 2: launch_force_lanes<2>(precision, block, stats, tail, a, st); break;
+    case 4: launch_force_lanes<4>(precision, block, stats, tail, a, st); break;
+    case 8: launch_force_lanes<8>(precision, block, stats, tail, a, st); break;
+    default: launch_force_lanes<16>(precision, block, stats, tail, a, st); break;
+  }
+}
+
+// ------------------------------------------------------ statistic reduce --
+
+// One block per replica sums the wave partials in a fixed order.
+__global__ __launch_bounds__(256) void k_stats_reduce(const RepInfo *reps,
+                                                      int A,
+                                                      const double *wave_part,
+                                                      double *stats) {
+  __shared__ double red[256][5];
+  const int r = blockIdx.x;
+  const RepInfo &R = reps[r];
+  const int w_lo = R.atom_off / A;
+  const int w_hi = (R.atom_off + R.n_atoms - 1) / A;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int w = w_lo + (int)threadIdx.x; w <= w_hi; w += blockDim.x) {
+    const double *d = wave_part + (size_t)w * 12;
+    if ((int)d[0] == r)
+      for (int c = 0; c < 5; ++c) acc[c] += d[1 + c];
+    if ((int)d[6] == r)
+      for (int c = 0; c < 5; ++c) acc[c] += d[7 + c];
+  }
+  for (int c = 0; c < 5; ++c) red[threadIdx.x][c] = acc[c];
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int c = 0; c < 5; ++c) red[threadIdx.x][c] += red[threadIdx.x + o][c];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 5; ++c) stats[(size_t)r * 5 + c] = red[0][c];
+}
+
+void launch_stats_reduce(int n_rep, const RepInfo *reps, int lanes,
+                         const double *wave_part, double *stats,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_stats_reduce, dim3(n_rep), dim3(256), 0, st, reps,
+                     kWave / lanes, wave_part, stats);
+}
+
+// ----------------------------------------------------------------- export --
+
+__global__ __launch_bounds__(256) void k_export_forces(
+    int n, const uint32_t *perm, const int64_t *fx, const int64_t *fy,
+    const int64_t *fz, const double *fxd, const double *fyd, const double *fzd,
+    double *out, int64_t *out_fix) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const size_t i = (size_t)perm[s] * 3;
+  if (fx) {
+    const int64_t X = fx[s], Y = fy[s], Z = fz[s];
+    if (out) {
+      out[i] = (double)X * kFixInv;
+      out[i + 1] = (double)Y * kFixInv;
+      out[i + 2] = (double)Z * kFixInv;
+    }
+    if (out_fix) {
+      out_fix[i] = X; out_fix[i + 1] = Y; out_fix[i + 2] = Z;
+    }
+  } else if (out) {
+    out[i] = fxd[s];
+    out[i + 1] = fyd[s];
+    out[i + 2] = fzd[s];
+  }
+}
+
+void launch_export_forces(int n, const uint32_t *perm, const int64_t *fx,
+                          const int64_t *fy, const int64_t *fz,
+                          const double *fxd, const double *fyd,
+                          const double *fzd, double *out, int64_t *out_fix,
+                          hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_export_forces, dim3((n + 255) / 256), dim3(256), 0, st,
+                     n, perm, fx, fy, fz, fxd, fyd, fzd, out, out_fix);
+}
+
+__global__ __launch_bounds__(256) void k_add_forces_f32(
+    int n, const uint32_t *perm, const int64_t *fx, const int64_t *fy,
+    const int64_t *fz, const double *fxd, const double *fyd, const double *fzd,
+    float *ox, float *oy, float *oz) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = perm[s];
+  if (fx) {
+    ox[i] += (float)((double)fx[s] * kFixInv);
+    oy[i] += (float)((double)fy[s] * kFixInv);
+    oz[i] += (float)((double)fz[s] * kFixInv);
+  } else {
+    ox[i] += (float)fxd[s];
+    oy[i] += (float)fyd[s];
+    oz[i] += (float)fzd[s];
+  }
+}
+
+void launch_add_forces_f32(int n, const uint32_t *perm, const int64_t *fx,
+                           const int64_t *fy, const int64_t *fz,
+                           const double *fxd, const double *fyd,
+                           const double *fzd, float *ox, float *oy, float *oz,
+                           hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add_forces_f32, dim3((n + 255) / 256), dim3(256), 0, st,
+                     n, perm, fx, fy, fz, fxd, fyd, fzd, ox, oy, oz);
+}
+
+__global__ __launch_bounds__(256) void k_export_atom_stats(int n,
+                                                           const uint32_t *perm,
+                                                           const double *src,
+                                                           double *dst) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const size_t i = (size_t)perm[s] * 4;
+  for (int c = 0; c < 4; ++c) dst[i + c] = src[(size_t)s * 4 + c];
+}
+
+void launch_export_atom_stats(int n, const uint32_t *perm, const double *src,
+                              double *dst, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_export_atom_stats, dim3((n + 255) / 256), dim3(256), 0,
+                     st, n, perm, src, dst);
+}
+
+}  // namespace sre

@@ -1,0 +1,368 @@
+// k_sort.hip -- cell keys, atomic-free LSD radix sort, cell offsets, and the
+// gather of per-atom data into cell order.
+//
+// The reference bins atoms with fixed-capacity cells and atomicInc slot
+// insertion (H/src/NeighborList.cu:84-167), whose in-cell order depends on
+// scheduling.  Here binning is a stable counting sort built from
+//   (1) a per-block bitonic sort of (digit, local index) in LDS,
+//   (2) a digit-major exclusive scan of per-block digit counts,
+//   (3) a stable scatter,
+// repeated over 1-3 digits of the cell key.  No atomics are used, so the cell
+// order (and everything derived from it) is a pure function of the input.
+#include "sre_internal.h"
+
+namespace sre {
+
+constexpr int kSortE = 2048;      // elements per sort block
+constexpr int kSortT = 1024;      // threads per sort block
+constexpr int kMaxDigitBits = 12;
+constexpr int kScanTile = 4096;   // elements per scan block (1024 threads x 4)
+
+// ------------------------------------------------------------------ keys --
+
+__global__ __launch_bounds__(256) void k_keys(int n, const float *pos,
+                                              int stride, const int *rep_of,
+                                              const RepInfo *reps,
+                                              uint32_t *key, uint32_t *val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const RepInfo &R = reps[rep_of[i]];
+  const float *p = pos + (size_t)i * stride;
+  uint32_t ux = quantize(p[0], R.invq[0]);
+  uint32_t uy = quantize(p[1], R.invq[1]);
+  uint32_t uz = quantize(p[2], R.invq[2]);
+  int cx = cell_of(ux, R.nc[0]);
+  int cy = cell_of(uy, R.nc[1]);
+  int cz = cell_of(uz, R.nc[2]);
+  key[i] = (uint32_t)(R.cell_off + (cx * R.nc[1] + cy) * R.nc[2] + cz);
+  val[i] = (uint32_t)i;
+}
+
+void launch_keys(int n, const float *pos, int stride, const int *rep_of,
+                 const RepInfo *reps, uint32_t *key, uint32_t *val,
+                 hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, pos,
+                     stride, rep_of, reps, key, val);
+}
+
+// ------------------------------------------------------- block scan 1024 --
+
+// Exclusive scan of one value per thread over a 1024-thread block.
+__device__ uint32_t block_exscan_1024(uint32_t v, uint32_t *tmp,
+                                      uint32_t *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int i = 0; i < kSortT / 64; ++i) {
+      uint32_t t = tmp[i];
+      tmp[i] = s;
+      s += t;
+    }
+    tmp[kSortT / 64] = s;
+  }
+  __syncthreads();
+  uint32_t r = x - v + tmp[w];
+  if (total) *total = tmp[kSortT / 64];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ sort block --
+
+__global__ __launch_bounds__(kSortT) void k_sort_block(
+    int n, int shift, int bits, int nblocks, const uint32_t *key,
+    const uint32_t *val, uint32_t *tkey, uint32_t *tval, uint32_t *counts) {
+  __shared__ uint32_t sk[kSortE];
+  __shared__ int run_start[1 << kMaxDigitBits];
+  __shared__ uint32_t cnt[1 << kMaxDigitBits];
+  const int b = blockIdx.x;
+  const int base = b * kSortE;
+  const int nv = min(kSortE, n - base);
+  const uint32_t R = 1u << bits, mask = R - 1u;
+  for (int t = threadIdx.x; t < kSortE; t += kSortT)
+    sk[t] = (t < nv) ? ((((key[base + t] >> shift) & mask) << 11) | (uint32_t)t)
+                     : 0xFFFFFFFFu;
+  for (uint32_t d = threadIdx.x; d < R; d += kSortT) cnt[d] = 0;
+  __syncthreads();
+  // bitonic sort of kSortE unique 32-bit keys, one compare-swap per thread
+  for (int k = 2; k <= kSortE; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int t = threadIdx.x;
+      const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+      const int p = i | j;
+      uint32_t a = sk[i], c = sk[p];
+      const bool asc = (i & k) == 0;
+      if ((a > c) == asc) {
+        sk[i] = c;
+        sk[p] = a;
+      }
+      __syncthreads();
+    }
+  }
+  for (int p = threadIdx.x; p < nv; p += kSortT) {
+    uint32_t d = sk[p] >> 11;
+    if (p == 0 || (sk[p - 1] >> 11) != d) run_start[d] = p;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < nv; p += kSortT) {
+    uint32_t d = sk[p] >> 11;
+    if (p == nv - 1 || (sk[p + 1] >> 11) != d)
+      cnt[d] = (uint32_t)(p + 1 - run_start[d]);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < R; d += kSortT)
+    counts[(size_t)d * nblocks + b] = cnt[d];
+  for (int p = threadIdx.x; p < nv; p += kSortT) {
+    int li = (int)(sk[p] & 2047u);
+    tkey[base + p] = key[base + li];
+    tval[base + p] = val[base + li];
+  }
+}
+
+__global__ __launch_bounds__(kSortT) void k_sort_scatter(
+    int n, int shift, int bits, int nblocks, const uint32_t *tkey,
+    const uint32_t *tval, const uint32_t *counts, const uint32_t *offsets,
+    uint32_t *okey, uint32_t *oval) {
+  __shared__ uint32_t lstart[1 << kMaxDigitBits];
+  __shared__ uint32_t goff[1 << kMaxDigitBits];
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const int b = blockIdx.x;
+  const int base = b * kSortE;
+  const int nv = min(kSortE, n - base);
+  const int R = 1 << bits;
+  const uint32_t mask = (uint32_t)R - 1u;
+  const int per = (R + kSortT - 1) / kSortT;
+  const int lo = min(R, (int)threadIdx.x * per), hi = min(R, lo + per);
+  uint32_t local = 0;
+  for (int d = lo; d < hi; ++d) local += counts[(size_t)d * nblocks + b];
+  uint32_t run = block_exscan_1024(local, tmp, nullptr);
+  for (int d = lo; d < hi; ++d) {
+    lstart[d] = run;
+    run += counts[(size_t)d * nblocks + b];
+    goff[d] = offsets[(size_t)d * nblocks + b];
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < nv; p += kSortT) {
+    uint32_t k = tkey[base + p];
+    uint32_t d = (k >> shift) & mask;
+    uint32_t dst = goff[d] + ((uint32_t)p - lstart[d]);
+    okey[dst] = k;
+    oval[dst] = tval[base + p];
+  }
+}
+
+// ------------------------------------------------------------ device scan --
+
+__global__ __launch_bounds__(kSortT) void k_scan_reduce(size_t m,
+                                                        const uint32_t *in,
+                                                        uint32_t *sums) {
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const size_t base = (size_t)blockIdx.x * kScanTile;
+  uint32_t v = 0;
+  for (int k = 0; k < 4; ++k) {
+    size_t idx = base + (size_t)threadIdx.x * 4 + k;
+    if (idx < m) v += in[idx];
+  }
+  uint32_t total;
+  block_exscan_1024(v, tmp, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// single-block exclusive scan of m elements (in place allowed)
+__global__ __launch_bounds__(kSortT) void k_scan_single(size_t m,
+                                                        const uint32_t *in,
+                                                        uint32_t *out) {
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const size_t per = (m + kSortT - 1) / kSortT;
+  const size_t lo = min(m, (size_t)threadIdx.x * per), hi = min(m, lo + per);
+  uint32_t local = 0;
+  for (size_t i = lo; i < hi; ++i) local += in[i];
+  uint32_t run = block_exscan_1024(local, tmp, nullptr);
+  for (size_t i = lo; i < hi; ++i) {
+    uint32_t v = in[i];
+    out[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortT) void k_scan_down(size_t m,
+                                                      const uint32_t *in,
+                                                      const uint32_t *sums,
+                                                      uint32_t *out) {
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * 4;
+  uint32_t v[4];
+  uint32_t local = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < m) ? in[base + k] : 0u;
+    local += v[k];
+  }
+  uint32_t run = block_exscan_1024(local, tmp, nullptr) + sums[blockIdx.x];
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < m) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+static void device_exscan(size_t m, const uint32_t *in, uint32_t *out,
+                          uint32_t *tmp, hipStream_t st) {
+  if (m <= 65536) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSortT), 0, st, m, in,
+                       out);
+    return;
+  }
+  size_t nt = (m + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nt), dim3(kSortT), 0, st,
+                     m, in, tmp);
+  hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSortT), 0, st, nt, tmp,
+                     tmp);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nt), dim3(kSortT), 0, st, m,
+                     in, tmp, out);
+}
+
+static void digit_plan(int key_bits, int *np, int bits[3]) {
+  if (key_bits < 1) key_bits = 1;
+  int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  int rem = key_bits;
+  for (int p = 0; p < passes; ++p) {
+    bits[p] = (rem + (passes - p) - 1) / (passes - p);
+    rem -= bits[p];
+  }
+  *np = passes;
+}
+
+size_t radix_counts_capacity(int n, int key_bits) {
+  int np, bits[3];
+  digit_plan(key_bits, &np, bits);
+  int mb = 0;
+  for (int p = 0; p < np; ++p) mb = bits[p] > mb ? bits[p] : mb;
+  size_t nblocks = (size_t)(n + kSortE - 1) / kSortE;
+  return ((size_t)1 << mb) * (nblocks > 0 ? nblocks : 1);
+}
+
+int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
+  if (n <= 0) return 0;
+  if (key_bits > 3 * kMaxDigitBits) return -1;
+  int np, bits[3];
+  digit_plan(key_bits, &np, bits);
+  const int nblocks = (n + kSortE - 1) / kSortE;
+  uint32_t *kin = b.key_a, *vin = b.val_a, *kout = b.key_b, *vout = b.val_b;
+  int shift = 0;
+  for (int p = 0; p < np; ++p) {
+    const size_t m = ((size_t)1 << bits[p]) * nblocks;
+    hipLaunchKernelGGL(k_sort_block, dim3(nblocks), dim3(kSortT), 0, st, n,
+                       shift, bits[p], nblocks, kin, vin, b.tkey, b.tval,
+                       b.counts);
+    device_exscan(m, b.counts, b.offsets, b.scan_tmp, st);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortT), 0, st, n,
+                       shift, bits[p], nblocks, b.tkey, b.tval, b.counts,
+                       b.offsets, kout, vout);
+    shift += bits[p];
+    uint32_t *t;
+    t = kin; kin = kout; kout = t;
+    t = vin; vin = vout; vout = t;
+  }
+  if (kin != b.key_a) {
+    // odd number of passes ended in key_b/val_b: swap roles for the caller
+    uint32_t *t;
+    t = b.key_a; b.key_a = b.key_b; b.key_b = t;
+    t = b.val_a; b.val_a = b.val_b; b.val_b = t;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------- cell start --
+
+__global__ __launch_bounds__(256) void k_cell_start(int n_cells, int n,
+                                                    const uint32_t *keys,
+                                                    int *start) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_cells) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (keys[mid] < (uint32_t)c) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = lo;
+}
+
+void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_start, dim3((n_cells + 1 + 255) / 256), dim3(256),
+                     0, st, n_cells, n, keys, start);
+}
+
+// ---------------------------------------------------------- gather sorted --
+
+__global__ __launch_bounds__(256) void k_gather_sorted(
+    int n, const uint32_t *perm, const float *pos, int stride,
+    const int *types, const int *rep_of, const RepInfo *reps,
+    const float *rc_table, int n_types, uint4 *pos4, uint4 *backup,
+    float4 *res4, float *rcut, int *tail_idx) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = (int)perm[s];
+  const int r = rep_of[i];
+  const RepInfo &R = reps[r];
+  const float *p = pos + (size_t)i * stride;
+  const float x = p[0], y = p[1], z = p[2];
+  const int ty = types[i];
+  uint4 v = make_uint4(quantize(x, R.invq[0]), quantize(y, R.invq[1]),
+                       quantize(z, R.invq[2]),
+                       (uint32_t)ty | ((uint32_t)r << 8));
+  pos4[s] = v;
+  backup[s] = v;
+  if (res4)
+    res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
+                          quant_residual(y, R.invq[1], R.qd[1]),
+                          quant_residual(z, R.invq[2], R.qd[2]), 0.f);
+  const int ix = rc_bin(x, R.L[0], R.rc_n[0]);
+  const int iy = rc_bin(y, R.L[1], R.rc_n[1]);
+  const int iz = rc_bin(z, R.L[2], R.rc_n[2]);
+  rcut[s] = rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
+  tail_idx[s] = (R.tail_off >= 0) ? (R.tail_off + iz) * n_types + ty : -1;
+}
+
+void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
+                          int stride, const int *types, const int *rep_of,
+                          const RepInfo *reps, const float *rc_table,
+                          int n_types, uint4 *pos4, uint4 *backup, float4 *res4,
+                          float *rcut, int *tail_idx, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_sorted, dim3((n + 255) / 256), dim3(256), 0, st,
+                     n, perm, pos, stride, types, rep_of, reps, rc_table,
+                     n_types, pos4, backup, res4, rcut, tail_idx);
+}
+
+// ------------------------------------------------------------------ fills --
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i64(int64_t *p, int64_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, st, p, v, n);
+}
+void launch_fill_i64(int64_t *p, int64_t v, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, st, p, v, n);
+}
+
+}  // namespace sre

@@ -1,0 +1,243 @@
+// sre_internal.h -- types and device helpers shared by the SRE HIP sources.
+//
+// Numerical definitions used by every kernel (the oracle restates them):
+//
+//  * Fixed-point lattice coordinate of an FP32 position x in a box edge L:
+//        u = (floor(double(x) * (2^32 / double(L)) + 0.5) mod 2^32)
+//    evaluated in binary64.  Minimum image is then plain int32 wraparound of
+//    u_j - u_i, and the displacement is  d = float(int32(u_j - u_i)) * q,
+//    q = L * 2^-32 (exact in FP32).  The reference computes
+//    d -= floorf(d/L + 0.5) L in FP32 (H/include/BoxGeometry.h:171-186); both
+//    pick the nearest image, ours without rounding in the wrap.
+//  * Squared distance: d2 = fmaf(dz, dz, fmaf(dy, dy, dx*dx)) (FP32, exactly
+//    this operation order; the library is compiled with -ffp-contract=off).
+//  * List predicate  j != i && d2 < rl*rl,  rl = rc_i + skin  (FP32), the
+//    reference's H/src/NeighborList.cu:391.  Force predicate d2 < rc_i*rc_i
+//    (H/src/InteractionEngine_interface.cu:1224-1245).
+//  * rc bin of an atom: one-image wrap, ix = int((x*nx)/L) in FP32 with an
+//    IEEE division (H/src/AssignRCut.cu:92-101), clamped to [0, nx-1].
+//  * Forces: int64 fixed point, scale 2^32, each pair component rounded to
+//    nearest-even before the integer sum (order-independent).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sre {
+
+constexpr int kWave = 64;
+constexpr int kMaxTypes = 8;
+constexpr int kMaxPairTypes = kMaxTypes * kMaxTypes;
+constexpr double kFixScale = 4294967296.0;          // 2^32
+constexpr double kFixInv = 1.0 / 4294967296.0;
+constexpr double kMagic = 6755399441055744.0;       // 1.5 * 2^52
+constexpr float kPairForceLimit = 524288.0f;        // 2^19: |pair component| limit of the magic rounding
+
+// Geometry and table offsets of one replica (device-resident array).
+struct RepInfo {
+  float L[3];       // box edges (FP32, as given)
+  float q[3];       // L * 2^-32
+  double invq[3];   // 2^32 / L
+  double qd[3];     // L * 2^-32 in FP64
+  int nc[3];        // cell grid
+  float h[3];       // cell edges L / nc
+  int cell_off;     // first global cell id of this replica
+  int atom_off;     // first atom (caller order and sorted order)
+  int n_atoms;
+  int rc_off;       // offset of the rc lattice in the concatenated table
+  int rc_n[3];      // rc lattice dims nx, ny, nz
+  int tail_off;     // offset (in entries of n_types) of the tail table, -1 if none
+  int rho_off;      // offset of rho[n_types][rho_nz], -1 if none
+  int rho_nz;
+  float rc_min, rc_max;
+};
+
+// LJ pair constants in the form used by the kernels:
+//   u = ri6*(A*ri6 - B) + shift,  F_i = d * ri2*ri6*(B6 - A12*ri6)
+// with A = 4 eps sig^12, B = 4 eps sig^6, A12 = 12 A, B6 = 6 B.
+struct PairParam {
+  float A12, B6, A, B;
+  float shift, eps, sig, pad;
+};
+struct PairParamD {
+  double A12, B6, A, B;
+  double shift, eps, sig, pad;
+};
+
+// ---------------------------------------------------------------- helpers --
+
+__host__ __device__ inline uint32_t quantize(float x, double invq) {
+  double t = (double)x * invq;
+  double f = floor(t + 0.5);
+  double w = f - floor(f * 0x1p-32) * 0x1p32;
+  return (uint32_t)w;
+}
+
+// Sub-lattice residual r = x - u*q (mod L) of the same coordinate, in FP64
+// length units (|r| <= q/2); the FP64 form adds r_j - r_i to the lattice
+// displacement so that it does not inherit the 2^-32 quantization.
+__host__ __device__ inline float quant_residual(float x, double invq, double qd) {
+  double t = (double)x * invq;
+  double f = floor(t + 0.5);
+  return (float)((t - f) * qd);
+}
+
+__host__ __device__ inline int cell_of(uint32_t u, int nc) {
+  return (int)(((uint64_t)u * (uint64_t)nc) >> 32);
+}
+
+__host__ __device__ inline int rc_bin(float x, float L, int n) {
+  if (x < 0.f) x = x + L;
+  else if (x >= L) x = x - L;
+  int ix = (int)((x * (float)n) / L);
+  ix = ix < 0 ? 0 : ix;
+  ix = ix > n - 1 ? n - 1 : ix;
+  return ix;
+}
+
+__device__ __forceinline__ float disp(uint32_t uj, uint32_t ui, float q) {
+  return (float)(int32_t)(uj - ui) * q;
+}
+
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) {
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Round v*2^32 to the nearest integer (ties to even) and return the bit
+// pattern of (v*2^32 + 1.5*2^52); subtracting bits(1.5*2^52) recovers the
+// integer.  Valid for |v| < 2^19.
+__device__ __forceinline__ uint64_t fix_bits(double v) {
+  double t = __builtin_fma(v, kFixScale, kMagic);
+  return (uint64_t)__double_as_longlong(t);
+}
+__device__ __forceinline__ uint64_t magic_bits() {
+  return (uint64_t)__double_as_longlong(kMagic);
+}
+// Exact conversion of an FP64 value (|v| < 2^19) to fixed point.
+__device__ __forceinline__ int64_t to_fix(double v) {
+  return (int64_t)(fix_bits(v) - magic_bits());
+}
+
+// list slot of entry k of sorted atom s: waves of 64/T atoms, entry rows of 64
+__host__ __device__ inline size_t list_slot(int s, int k, int T, int cap) {
+  int A = kWave / T;
+  int w = s / A, a = s - w * A;
+  return (size_t)w * (size_t)cap * (size_t)A + (size_t)(k / T) * kWave +
+         (size_t)(a * T + (k % T));
+}
+
+// ------------------------------------------------------------ launchers ----
+// Implemented in the k_*.hip files; called from engine.hip.
+
+struct SortBufs {
+  uint32_t *key_a, *val_a, *key_b, *val_b, *tkey, *tval;
+  uint32_t *counts, *offsets, *scan_tmp;
+};
+
+void launch_keys(int n, const float *pos, int stride, const int *rep_of,
+                 const RepInfo *reps, uint32_t *key, uint32_t *val,
+                 hipStream_t st);
+// Sorts (key, val) ascending by key (stable), key < 2^key_bits.  Result in
+// key_a/val_a of bufs.  Returns 0 on success.
+int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st);
+size_t radix_counts_capacity(int n, int key_bits);
+void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
+                       hipStream_t st);
+void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
+                          int stride, const int *types, const int *rep_of,
+                          const RepInfo *reps, const float *rc_table,
+                          int n_types, uint4 *pos4, uint4 *backup, float4 *res4,
+                          float *rcut, int *tail_idx, hipStream_t st);
+void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
+                       const float *rcut, const RepInfo *reps, const int *start,
+                       float skin, int cap, int *list, int *nnb, int *maxcount,
+                       hipStream_t st);
+// step_ctr (optional) is advanced by one thread after the positions are read
+void launch_update_positions(int n, const uint32_t *perm, const float *pos,
+                             int stride, const RepInfo *reps, uint4 *pos4,
+                             const uint4 *backup, float4 *res4,
+                             unsigned *skin_flag, uint64_t *step_ctr,
+                             hipStream_t st);
+
+struct ForceArgs {
+  int n;
+  int n_types;
+  int cap;
+  const uint4 *pos4;
+  const float4 *res4;          // FP64 form: sub-lattice residuals (sorted order)
+  const float *rcut;
+  const int *nnb;
+  const int *list;
+  const RepInfo *reps;
+  const PairParam *params;     // n_types^2
+  const PairParamD *paramsd;   // n_types^2 (FP64 form)
+  const double2 *tail;         // per-atom-index table (F_z, E) or NULL
+  const int *tail_idx;         // per sorted atom, -1 = none
+  const double2 *tail_atom;    // exact mode: per sorted atom (F_z, E) or NULL
+  int64_t *fx, *fy, *fz;       // fixed-point output (sorted order)
+  double *fxd, *fyd, *fzd;     // FP64 output (sorted order)
+  double *wave_part;           // per wave: 2 slots x 5 (E, Wxx, Wyy, Wzz, n_in)
+  double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
+  unsigned *err_flag;
+};
+void launch_force(int precision, int lanes, int block, bool stats, bool tail,
+                  const ForceArgs &a, hipStream_t st);
+int force_num_waves(int n, int lanes);
+void launch_stats_reduce(int n_rep, const RepInfo *reps, int lanes,
+                         const double *wave_part, double *stats,
+                         hipStream_t st);
+void launch_export_forces(int n, const uint32_t *perm, const int64_t *fx,
+                          const int64_t *fy, const int64_t *fz,
+                          const double *fxd, const double *fyd,
+                          const double *fzd, double *out, int64_t *out_fix,
+                          hipStream_t st);
+void launch_add_forces_f32(int n, const uint32_t *perm, const int64_t *fx,
+                           const int64_t *fy, const int64_t *fz,
+                           const double *fxd, const double *fyd,
+                           const double *fzd, float *ox, float *oy, float *oz,
+                           hipStream_t st);
+void launch_export_atom_stats(int n, const uint32_t *perm, const double *src,
+                              double *dst, hipStream_t st);
+
+void launch_tail_table(int n_entries, const RepInfo *reps,
+                       const int *entry_rep, int n_types,
+                       const PairParamD *params, const float *rc_table,
+                       const double *rho, double2 *tail, hipStream_t st);
+void launch_tail_exact(int n, int n_types, const uint4 *pos4,
+                       const float *rcut, const RepInfo *reps,
+                       const PairParamD *params, const double *rho,
+                       double2 *out, hipStream_t st);
+
+struct AdjointArgs {
+  int n, n_types, cap, n_rep;
+  const uint4 *pos4;
+  const float *rcut;
+  const int *nnb;
+  const int *list;
+  const uint32_t *perm;
+  const RepInfo *reps;
+  const PairParam *params;
+  const double *dLdE;      // per replica (device)
+  const double *dLdF;      // caller order [N][3] or NULL
+  const int *tail_idx;
+  int use_tail;
+  double *wave_part;       // per launched wave: 2 * n_types^2 doubles
+  int64_t *tail_w;         // per tail entry: (sum g_z, sum dLdE), scale 2^32
+};
+int adjoint_grid_waves(int n, int lanes);
+// pair part: out[2*npt] (d/d eps then d/d sigma per ordered type pair)
+void launch_adjoint_pairs(const AdjointArgs &a, int precision, int lanes,
+                          double *out, hipStream_t st);
+// tail part: g_rc per rc lattice entry, out[2*npt] as above
+void launch_adjoint_tail(int n_bins, const int *entry_rep, const RepInfo *reps,
+                         int n_types, const PairParamD *params,
+                         const float *rc_table, const double *rho,
+                         const int64_t *tail_w, double *contrib, double *g_rc,
+                         double *out, hipStream_t st);
+
+void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+void launch_fill_i64(int64_t *p, int64_t v, size_t n, hipStream_t st);
+// step_ctr != NULL: the step index is read from device memory
+void launch_perturb(int n, float *pos, int stride, float sigma, uint64_t seed,
+                    uint64_t step, const uint64_t *step_ctr, hipStream_t st);
+
+}  // namespace sre

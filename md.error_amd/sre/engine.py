@@ -1,0 +1,314 @@
+"""Python binding over the C ABI of libsre.so.
+
+``Engine`` is a thin object wrapper; every method is one C call.  Errors
+become exceptions named after the reference's (MDExcptCuda for device/runtime
+failures, H/include/MDException.h:11-29; the list/cell/box flags of
+H/include/error.h:19-26).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+
+
+class SREError(RuntimeError):
+    """Base of all engine errors; ``status`` is the sre_status code."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"[sre status {status}] {msg}")
+        self.status = status
+
+
+class MDExcptCuda(SREError):
+    """Runtime (HIP) failure, as checkCUDAError raises MDExcptCuda."""
+
+
+class ShortNeighborList(SREError):
+    """mdErrorShortNeighborList: the list could not be regrown."""
+
+
+class ShortCellList(SREError):
+    """mdErrorShortCellList: the cell grid does not fit the sort key."""
+
+
+class BoxTooSmall(SREError):
+    """rc_max + skin is not below half of a box edge."""
+
+
+class ForceRange(SREError):
+    """A pair force left the fixed-point range (overlapping atoms)."""
+
+
+class SkinExceeded(SREError):
+    """An atom moved more than skin/2 without a rebuild (graph mode)."""
+
+
+class NotReady(SREError):
+    """Call order error (e.g. compute before rebuild)."""
+
+
+class InvalidArgument(SREError, ValueError):
+    pass
+
+
+_EXC = {
+    _lib.ERR_LIST_OVERFLOW: ShortNeighborList,
+    _lib.ERR_CELL: ShortCellList,
+    _lib.ERR_INVALID: InvalidArgument,
+    _lib.ERR_HIP: MDExcptCuda,
+    _lib.ERR_BOX_TOO_SMALL: BoxTooSmall,
+    _lib.ERR_FORCE_RANGE: ForceRange,
+    _lib.ERR_SKIN: SkinExceeded,
+    _lib.ERR_NOT_READY: NotReady,
+}
+
+
+def check(status):
+    if status != _lib.OK:
+        msg = lib().sre_last_error().decode(errors="replace")
+        raise _EXC.get(status, SREError)(status, msg)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _flags(stats, tail):
+    return (_lib.STATS if stats else 0) | (_lib.TAIL if tail else 0)
+
+
+class Engine:
+    """One engine: R >= 1 periodic systems sharing one LJ force field."""
+
+    def __init__(self, n_types=1, skin=0.3, precision="mixed", stream=None, device=-1):
+        prec = {"mixed": _lib.PREC_MIXED, "fp64": _lib.PREC_FP64}[precision]
+        opt = _lib.Options(int(n_types), float(skin), prec,
+                           C.c_void_p(stream) if stream else None, int(device))
+        h = C.c_void_p()
+        check(lib().sre_create(C.byref(h), C.byref(opt)))
+        self._h = h
+        self.n_types = int(n_types)
+        self.skin = float(skin)
+        self.precision = precision
+        self.n_atoms = 0
+        self.n_replicas = 0
+        self._pos_ref = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sre_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- description -------------------------------------------------------
+    def set_system(self, n_atoms, boxes):
+        n = np.ascontiguousarray(np.atleast_1d(n_atoms), dtype=np.int32)
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.float32).reshape(-1, 3))
+        assert b.shape[0] == n.size
+        check(lib().sre_set_system(self._h, n.size, _ptr(n, C.c_int32), _ptr(b, C.c_float)))
+        self.n_atoms = int(n.sum())
+        self.n_replicas = int(n.size)
+
+    def set_types(self, types):
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        assert t.size == self.n_atoms
+        check(lib().sre_set_types(self._h, _ptr(t, C.c_int32)))
+
+    def set_pair_params(self, eps, sigma, shift=None):
+        nt = self.n_types
+        e = np.ascontiguousarray(np.asarray(eps, np.float32).reshape(nt, nt))
+        s = np.ascontiguousarray(np.asarray(sigma, np.float32).reshape(nt, nt))
+        sh = (np.ascontiguousarray(np.asarray(shift, np.float32).reshape(nt, nt))
+              if shift is not None else None)
+        check(lib().sre_set_pair_params(self._h, _ptr(e, C.c_float), _ptr(s, C.c_float),
+                                        _ptr(sh, C.c_float) if sh is not None else None))
+
+    def set_rc_table(self, table, replica=0):
+        t = np.ascontiguousarray(np.asarray(table, np.float32))
+        if t.ndim == 1:
+            t = t.reshape(1, 1, -1)
+        nx, ny, nz = t.shape
+        check(lib().sre_set_rc_table(self._h, int(replica), nx, ny, nz, _ptr(t, C.c_float)))
+
+    def set_density_profile(self, rho, replica=0):
+        if rho is None:
+            check(lib().sre_set_density_profile(self._h, int(replica), 0, None))
+            return
+        r = np.ascontiguousarray(np.asarray(rho, np.float64).reshape(self.n_types, -1))
+        check(lib().sre_set_density_profile(self._h, int(replica), r.shape[1],
+                                            _ptr(r, C.c_double)))
+
+    def set_tail_mode(self, exact=False):
+        check(lib().sre_set_tail_mode(self._h, _lib.TAIL_EXACT if exact else _lib.TAIL_BIN))
+
+    # -- positions ---------------------------------------------------------
+    def upload_positions(self, xyz):
+        x = np.ascontiguousarray(np.asarray(xyz, np.float32))
+        assert x.shape[0] == self.n_atoms and x.shape[1] in (3, 4)
+        check(lib().sre_upload_positions(self._h, _ptr(x, C.c_float), x.shape[1]))
+
+    def bind_positions(self, tensor_or_ptr, stride=4):
+        """Binds a device array: a torch CUDA tensor (N, stride) or a raw pointer."""
+        if hasattr(tensor_or_ptr, "data_ptr"):
+            t = tensor_or_ptr
+            assert t.is_cuda and t.dtype.itemsize == 4 and t.is_contiguous()
+            stride = t.shape[1]
+            self._pos_ref = t
+            ptr = t.data_ptr()
+        else:
+            ptr = int(tensor_or_ptr)
+        check(lib().sre_bind_positions(self._h, C.c_void_p(ptr), int(stride)))
+
+    # -- hot path ----------------------------------------------------------
+    def rebuild(self):
+        check(lib().sre_rebuild(self._h))
+
+    def update_positions(self, check_skin=True):
+        need = C.c_int(0)
+        check(lib().sre_update_positions(self._h, C.byref(need) if check_skin else None))
+        return bool(need.value)
+
+    def compute(self, stats=False, tail=False):
+        check(lib().sre_compute(self._h, _flags(stats, tail)))
+
+    def step(self, stats=False, tail=False):
+        rebuilt = C.c_int(0)
+        check(lib().sre_step(self._h, _flags(stats, tail), C.byref(rebuilt)))
+        return bool(rebuilt.value)
+
+    # -- results -----------------------------------------------------------
+    def forces(self):
+        out = np.empty((self.n_atoms, 3), np.float64)
+        check(lib().sre_get_forces(self._h, _ptr(out, C.c_double), 0))
+        return out
+
+    def forces_to_device(self, tensor):
+        assert tensor.is_cuda and tensor.dtype.itemsize == 8
+        check(lib().sre_get_forces(self._h, C.cast(C.c_void_p(tensor.data_ptr()),
+                                                   C.POINTER(C.c_double)), 1))
+
+    def add_forces_f32_soa(self, fx, fy, fz):
+        check(lib().sre_add_forces_f32_soa(self._h, C.c_void_p(fx.data_ptr()),
+                                           C.c_void_p(fy.data_ptr()),
+                                           C.c_void_p(fz.data_ptr())))
+
+    def forces_fixed(self):
+        out = np.empty((self.n_atoms, 3), np.int64)
+        check(lib().sre_get_forces_fixed(self._h, _ptr(out, C.c_int64)))
+        return out
+
+    def stats(self):
+        out = np.empty((self.n_replicas, 4), np.float64)
+        check(lib().sre_get_stats(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def atom_stats(self):
+        out = np.empty((self.n_atoms, 4), np.float64)
+        check(lib().sre_get_atom_stats(self._h, _ptr(out, C.c_double)))
+        return out
+
+    def neighbor_count(self):
+        nl, ni, mx = C.c_int64(), C.c_int64(), C.c_int32()
+        check(lib().sre_neighbor_count(self._h, C.byref(nl), C.byref(ni), C.byref(mx)))
+        return nl.value, ni.value, mx.value
+
+    def export_pairs(self):
+        off = np.empty(self.n_atoms + 1, np.int64)
+        check(lib().sre_export_pairs(self._h, _ptr(off, C.c_int64), None))
+        j = np.empty(max(int(off[-1]), 1), np.int32)
+        check(lib().sre_export_pairs(self._h, _ptr(off, C.c_int64), _ptr(j, C.c_int32)))
+        return off, j[:int(off[-1])]
+
+    def rcut(self):
+        out = np.empty(self.n_atoms, np.float32)
+        check(lib().sre_get_rcut(self._h, _ptr(out, C.c_float)))
+        return out
+
+    def adjoint(self, dLdE=None, dLdF=None, tail=False, n_rc=None):
+        """Gradient of sum_r dLdE[r] E_r + sum_i dLdF_i . F_i.
+
+        dLdF: a CUDA float64 tensor (N, 3) in caller order, or None.
+        Returns (g_eps (nt, nt), g_sigma (nt, nt), g_rc (total rc lattice size,)).
+        """
+        nt = self.n_types
+        ge = np.zeros(nt * nt)
+        gs = np.zeros(nt * nt)
+        grc = np.zeros(max(int(n_rc or 0), 1))
+        dE = np.ascontiguousarray(np.broadcast_to(
+            np.asarray(0.0 if dLdE is None else dLdE, np.float64), (self.n_replicas,)))
+        ptr = None
+        if dLdF is not None:
+            assert dLdF.is_cuda and dLdF.dtype.itemsize == 8 and dLdF.is_contiguous()
+            ptr = C.c_void_p(dLdF.data_ptr())
+        check(lib().sre_adjoint(self._h, _ptr(dE, C.c_double), ptr,
+                                _lib.TAIL if tail else 0, _ptr(ge, C.c_double),
+                                _ptr(gs, C.c_double),
+                                _ptr(grc, C.c_double) if n_rc else None))
+        return ge.reshape(nt, nt), gs.reshape(nt, nt), (grc if n_rc else None)
+
+    # -- tuning, graphs, timing ---------------------------------------------
+    def tuning(self):
+        t = _lib.Tuning()
+        check(lib().sre_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _lib.Tuning()
+        check(lib().sre_get_tuning(self._h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().sre_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, stats=True, tail=False, cache_path=None):
+        check(lib().sre_autotune(self._h, _flags(stats, tail),
+                                 cache_path.encode() if cache_path else None))
+        return self.tuning()
+
+    def graph_run(self, n_steps, stats=False, tail=False, perturb=False):
+        """Replays [perturb +] update + compute n_steps times as one hipGraph."""
+        f = _flags(stats, tail) | (_lib.PERTURB if perturb else 0)
+        check(lib().sre_graph_run(self._h, f, int(n_steps)))
+
+    def set_perturbation(self, sigma, seed, first_step=0):
+        check(lib().sre_set_perturbation(self._h, float(sigma), int(seed), int(first_step)))
+
+    def check_skin_flag(self):
+        check(lib().sre_check_skin_flag(self._h))
+
+    def perturb(self, sigma, seed, step):
+        check(lib().sre_perturb_positions(self._h, float(sigma), int(seed), int(step)))
+
+    def time_compute(self, n_iter=10, stats=False, tail=False):
+        ms = C.c_double()
+        check(lib().sre_time_compute(self._h, _flags(stats, tail), int(n_iter), C.byref(ms)))
+        return ms.value
+
+    def time_rebuild(self, n_iter=3):
+        ms = C.c_double()
+        check(lib().sre_time_rebuild(self._h, int(n_iter), C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(lib().sre_synchronize(self._h))
+
+
+def gen_fcc(ncell, a, jitter, seed):
+    n = 4 * ncell ** 3
+    out = np.empty((n, 4), np.float32)
+    check(lib().sre_gen_fcc(int(ncell), float(a), float(jitter), int(seed),
+                            _ptr(out, C.c_float)))
+    return out
+
+
+def gen_slab(n, box, h, rho_l, dmin, seed):
+    out = np.empty((n, 4), np.float32)
+    check(lib().sre_gen_slab(int(n), float(box[0]), float(box[1]), float(box[2]), float(h),
+                             float(rho_l), float(dmin), int(seed), _ptr(out, C.c_float)))
+    return out
