@@ -99,7 +99,8 @@ struct sre_engine {
   int tail_bins = 0;
   double2 *d_tail_atom = nullptr;
   int tail_mode = SRE_TAIL_BIN;
-  bool tail_dirty = true;
+  bool tail_dirty = true;          // profile/rc layout changed: rebuild tables
+  bool tail_values_dirty = false;  // only the LJ parameters changed
 
   // positions
   const float *d_pos = nullptr;
@@ -360,8 +361,10 @@ static sre_status upload_params(sre_engine *e) {
     const double s6 = sg * sg * sg * sg * sg * sg, s12 = s6 * s6;
     const double A = 4.0 * ep * s12, B = 4.0 * ep * s6;
     pd[a] = PairParamD{12.0 * A, 6.0 * B, A, B, sh, ep, sg, 0.0};
-    p[a] = PairParam{(float)(12.0 * A), (float)(6.0 * B), (float)A, (float)B,
-                     (float)sh, (float)ep, (float)sg, 0.f};
+    // the mixed kernel's force constants carry the fixed-point scale 2^32
+    // (an exact power-of-two factor), so pair components come out pre-scaled
+    p[a] = PairParam{(float)(12.0 * A * kFixScale), (float)(6.0 * B * kFixScale),
+                     (float)A, (float)B, (float)sh, (float)ep, (float)sg, 0.f};
   }
   HIP_TRY(hipMemcpy(e->d_params, p.data(), sizeof(PairParam) * nt * nt,
                     hipMemcpyHostToDevice));
@@ -393,7 +396,7 @@ extern "C" sre_status sre_set_pair_params(sre_engine *e, const float *eps,
   sre_status s = upload_params(e);
   if (s != SRE_OK) return s;
   e->params_set = true;
-  e->tail_dirty = true;
+  e->tail_values_dirty = true;
   return SRE_OK;
 }
 
@@ -606,21 +609,37 @@ static sre_status prepare_tables(sre_engine *e) {
   }
   HIP_TRY(hipMemcpyAsync(e->d_reps, e->reps.data(), sizeof(RepInfo) * e->n_rep,
                          hipMemcpyHostToDevice, e->stream));
-  if (e->tail_dirty) {
+  if (e->tail_dirty || e->tail_values_dirty) {
     if (e->tail_bins > 0)
       launch_tail_table(e->tail_bins * e->n_types, e->d_reps,
                         e->d_tail_entry_rep, e->n_types, e->d_paramsd, e->d_rc,
                         e->d_rho, e->d_tail, e->stream);
     e->tail_dirty = false;
+    e->tail_values_dirty = false;
   }
   return SRE_OK;
+}
+
+// New LJ parameters after the last rebuild: recompute the tail values in place.
+static void refresh_tail_values(sre_engine *e) {
+  if (!e->tail_values_dirty) return;
+  if (e->tail_bins > 0) {
+    launch_tail_table(e->tail_bins * e->n_types, e->d_reps, e->d_tail_entry_rep,
+                      e->n_types, e->d_paramsd, e->d_rc, e->d_rho, e->d_tail,
+                      e->stream);
+    if (e->tail_mode == SRE_TAIL_EXACT && e->d_tail_atom)
+      launch_tail_exact(e->n_atoms, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
+                        e->d_paramsd, e->d_rho, e->d_tail_atom, e->stream);
+  }
+  e->tail_values_dirty = false;
 }
 
 static sre_status ensure_list(sre_engine *e, int cap) {
   const int T = e->tune.force_lanes;
   const int A = kWave / T;
   const size_t waves = (size_t)(e->n_atoms + A - 1) / A;
-  const size_t need = waves * (size_t)cap * A;
+  // + padding rows: the force kernel prefetches rows past a tile's end
+  const size_t need = waves * (size_t)cap * A + (size_t)kListPadRows * kWave;
   if (need > e->list_size) {
     HIP_TRY(hipStreamSynchronize(e->stream));
     dfree(e->d_list);
@@ -676,6 +695,7 @@ static sre_status do_rebuild(sre_engine *e) {
     launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
                       e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
                       e->d_list, e->d_nnb, e->d_maxcount, st);
+    HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
     int mc = 0;
     HIP_TRY(hipMemcpyAsync(&mc, e->d_maxcount, sizeof(int),
                            hipMemcpyDeviceToHost, st));
@@ -706,6 +726,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
                           e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
                           e->d_flags, nullptr, e->stream);
+  HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
     unsigned bits = 0;
@@ -760,11 +781,13 @@ static sre_status launch_compute(sre_engine *e, unsigned flags) {
       e->graph_epoch++;
     }
   }
+  refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
   launch_force(e->precision, T, e->tune.block, stats, tail, a, e->stream);
   if (stats)
     launch_stats_reduce(e->n_rep, e->d_reps, T, e->d_wave_part, e->d_stats,
                         e->stream);
+  HIP_TRY(hipGetLastError());  // launch-configuration errors
   return SRE_OK;
 }
 
@@ -1071,8 +1094,9 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
   if (!e || !t) return fail(SRE_ERR_INVALID, "null argument");
   if (!pow2_lanes(t->force_lanes) || !pow2_lanes(t->build_lanes))
     return fail(SRE_ERR_INVALID, "lanes must be 1, 2, 4, 8 or 16");
-  if (t->block < 64 || t->block > 1024 || t->block % 64)
-    return fail(SRE_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+  // the force kernels are compiled with __launch_bounds__(256)
+  if (t->block != 64 && t->block != 128 && t->block != 256)
+    return fail(SRE_ERR_INVALID, "block must be 64, 128 or 256");
   if (!(t->cell_xy >= 0.05f && t->cell_xy <= 4.f && t->cell_z >= 0.05f &&
         t->cell_z <= 4.f))
     return fail(SRE_ERR_INVALID, "cell sizes must be in [0.05, 4]");
@@ -1120,9 +1144,11 @@ static sre_status body_compute(sre_engine *e, unsigned flags) {
       e->graph_epoch++;
     }
   }
+  refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
   launch_force(e->precision, e->list_lanes, e->tune.block, stats,
                (flags & SRE_TAIL) && e->tail_bins > 0, a, e->stream);
+  HIP_TRY(hipGetLastError());
   return SRE_OK;
 }
 
@@ -1217,7 +1243,7 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
       }
   // phase 2: force kernel lanes and block
   const int force_lanes[] = {1, 2, 4, 8, 16};
-  const int blocks[] = {64, 128, 256, 512};
+  const int blocks[] = {64, 128, 256};
   double best_f = 1e30;
   sre_tuning bestf = best;
   std::vector<int64_t> ref, cur;
@@ -1278,6 +1304,7 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
       e->graph_epoch++;
     }
   }
+  refresh_tail_values(e);
   if (!e->graph_exec || e->graph_flags != flags ||
       e->graph_epoch_built != e->graph_epoch) {
     if (e->graph_exec) {

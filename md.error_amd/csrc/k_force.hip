@@ -101,8 +101,26 @@ __device__ __forceinline__ void write_wave_stats(double *wp, int wave,
 
 // -------------------------------------------------------------- mixed ----
 
+// Wave-wide maximum of an int (butterfly over the 64 lanes).
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// The loop runs a wave-uniform number of groups of U list rows (the wave's
+// longest list / (T U)); a lane whose entry k is past its atom's list
+// evaluates its own atom (d = 0) and masks the result.  Each group first
+// prefetches the next group's U list rows, then issues its U position gathers
+// together, then evaluates the U pairs, so one memory latency is exposed per
+// U pairs per wave.  Rows past the end of a wave's tile read the next tile or
+// the list's tail padding; those values are never used.
+// P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
+// pair component c is already c*2^32 in FP32 and its fixed-point value is the
+// bit pattern of double(c*2^32) + 1.5*2^52.
 template <int T, bool STATS, bool TAIL, bool MULTI>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
+  constexpr int U = kForceUnroll;
   __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -118,41 +136,56 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   const int s0 = wave * A;
   if (s0 >= a.n) return;  // whole wave beyond the atoms
 
+  const int si = act ? s : s0;
+  const uint4 pi = a.pos4[si];
+  const int rep = (int)(pi.w >> 8);
+  const RepInfo &R = a.reps[rep];
+  const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  const float rc = a.rcut[si];
+  const float rc2 = rc * rc;
+  const int nn = act ? a.nnb[si] : 0;
+  const int ti = (int)(pi.w & 0xffu) * a.n_types;
+  const int groups = __builtin_amdgcn_readfirstlane(
+      wave_max_int((nn + T * U - 1) / (T * U)));
+  PairParam P = a.params[0];
+
   uint64_t ax = 0, ay = 0, az = 0;
-  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fmx = 0.f;
-  int nin = 0, iters = 0, rep = 0;
-  if (act) {
-    const uint4 pi = a.pos4[s];
-    rep = (int)(pi.w >> 8);
-    const RepInfo &R = a.reps[rep];
-    const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
-    const float rc = a.rcut[s];
-    const float rc2 = rc * rc;
-    const int nn = a.nnb[s];
-    const int ti = (int)(pi.w & 0xffu) * a.n_types;
-    iters = (nn + T - 1) / T;
-    PairParam P = a.params[0];
-    const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
-    for (int m = 0; m < iters; ++m) {
-      const int k = m * T + l;
-      const bool v = k < nn;
-      const int j = v ? lp[(size_t)m * kWave] : s;
-      const uint4 pj = a.pos4[j];
-      const float dx = disp(pj.x, pi.x, qx);
-      const float dy = disp(pj.y, pi.y, qy);
-      const float dz = disp(pj.z, pi.z, qz);
+  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
+  int nin = 0;
+  const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)u * kWave];
+  for (int g = 0; g < groups; ++g) {
+    int jc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = (g * U + u) * T + l;
+      jc[u] = (k < nn) ? jn[u] : si;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)((g + 1) * U + u) * kWave];
+    uint4 pj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) pj[u] = a.pos4[jc[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool v = (g * U + u) * T + l < nn;
+      const float dx = disp(pj[u].x, pi.x, qx);
+      const float dy = disp(pj[u].y, pi.y, qy);
+      const float dz = disp(pj[u].z, pi.z, qz);
       const float d2 = dist2(dx, dy, dz);
       const bool in = v && (d2 < rc2);
-      if (MULTI) P = sp[ti + (int)(pj.w & 0xffu)];
+      if (MULTI) P = sp[ti + (int)(pj[u].w & 0xffu)];
       const float ri2 = __builtin_amdgcn_rcpf(d2);
       const float ri6 = ri2 * ri2 * ri2;
       float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
       fs = in ? fs : 0.f;
+      fsmax = fmaxf(fsmax, fabsf(fs));
       const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
-      ax += fix_bits((double)fpx);
-      ay += fix_bits((double)fpy);
-      az += fix_bits((double)fpz);
-      fmx = fmaxf(fmx, fmaxf(fabsf(fpx), fmaxf(fabsf(fpy), fabsf(fpz))));
+      ax += (uint64_t)__double_as_longlong((double)fpx + kMagic);
+      ay += (uint64_t)__double_as_longlong((double)fpy + kMagic);
+      az += (uint64_t)__double_as_longlong((double)fpz + kMagic);
       if (STATS) {
         const float ep = __builtin_fmaf(ri6, __builtin_fmaf(P.A, ri6, -P.B), P.shift);
         e += in ? ep : 0.f;
@@ -163,18 +196,21 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       }
     }
   }
+  const int iters = groups * U;
   const uint64_t mb = (uint64_t)iters * magic_bits();
   int64_t Fx = (int64_t)(ax - mb), Fy = (int64_t)(ay - mb), Fz = (int64_t)(az - mb);
   Fx = group_sum(Fx, T);
   Fy = group_sum(Fy, T);
   Fz = group_sum(Fz, T);
-  if (__ballot(fmx >= kPairForceLimit) && lane == 0) atomicOr(a.err_flag, 1u);
+  // |pair component| * 2^32 must stay below 2^51 (exact rounding by the magic add)
+  if (__ballot(act && fsmax * rc >= 2251799813685248.0f) && lane == 0)
+    atomicOr(a.err_flag, 1u);
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0, Nin = 0.0;
   if (STATS) {
     E = group_sum((double)e, T) * 0.5;
-    Wx = group_sum((double)wx, T) * 0.5;
-    Wy = group_sum((double)wy, T) * 0.5;
-    Wz = group_sum((double)wz, T) * 0.5;
+    Wx = group_sum((double)wx, T) * (0.5 * kFixInv);
+    Wy = group_sum((double)wy, T) * (0.5 * kFixInv);
+    Wz = group_sum((double)wz, T) * (0.5 * kFixInv);
     Nin = (double)group_sum(nin, T);
   }
   const bool own = act && l == 0;
@@ -184,8 +220,8 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       if (a.tail_atom) {
         tv = a.tail_atom[s];
       } else {
-        const int ti = a.tail_idx[s];
-        if (ti >= 0) tv = a.tail[ti];
+        const int tix = a.tail_idx[s];
+        if (tix >= 0) tv = a.tail[tix];
       }
       Fz += to_fix(tv.x);
       E += tv.y;

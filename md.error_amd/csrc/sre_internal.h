@@ -31,6 +31,10 @@ constexpr double kFixScale = 4294967296.0;          // 2^32
 constexpr double kFixInv = 1.0 / 4294967296.0;
 constexpr double kMagic = 6755399441055744.0;       // 1.5 * 2^52
 constexpr float kPairForceLimit = 524288.0f;        // 2^19: |pair component| limit of the magic rounding
+constexpr int kForceUnroll = 4;   // list rows per group in the mixed force kernel
+// list rows allocated past the last tile: the force kernel prefetches up to
+// 2 * kForceUnroll rows beyond a tile's end
+constexpr int kListPadRows = 2 * kForceUnroll + 2;
 
 // Geometry and table offsets of one replica (device-resident array).
 struct RepInfo {
@@ -55,7 +59,7 @@ struct RepInfo {
 //   u = ri6*(A*ri6 - B) + shift,  F_i = d * ri2*ri6*(B6 - A12*ri6)
 // with A = 4 eps sig^12, B = 4 eps sig^6, A12 = 12 A, B6 = 6 B.
 struct PairParam {
-  float A12, B6, A, B;
+  float A12, B6, A, B;   // A12 and B6 are stored multiplied by 2^32
   float shift, eps, sig, pad;
 };
 struct PairParamD {

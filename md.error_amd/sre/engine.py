@@ -115,6 +115,7 @@ class Engine:
         check(lib().sre_set_system(self._h, n.size, _ptr(n, C.c_int32), _ptr(b, C.c_float)))
         self.n_atoms = int(n.sum())
         self.n_replicas = int(n.size)
+        self._rc_layout = [None] * self.n_replicas
 
     def set_types(self, types):
         t = np.ascontiguousarray(types, dtype=np.int32)
@@ -136,6 +137,12 @@ class Engine:
             t = t.reshape(1, 1, -1)
         nx, ny, nz = t.shape
         check(lib().sre_set_rc_table(self._h, int(replica), nx, ny, nz, _ptr(t, C.c_float)))
+        self._rc_layout[int(replica)] = t.shape
+
+    @property
+    def rc_size(self):
+        """Total number of rc lattice values over all replicas."""
+        return int(sum(int(np.prod(s)) for s in self._rc_layout if s is not None))
 
     def set_density_profile(self, rho, replica=0):
         if rho is None:

@@ -1,0 +1,256 @@
+"""The reference's object API for the short-range path, over one Engine.
+
+Class and method names follow the reference's C++ interface so that a driver
+written against it (H/app/lj.nhc-xtc.cu:72-113, 169-188) reads the same:
+
+    sys = MDSystem(); sys.initConfig("conf.gro")
+    ljparam = LennardJones6_12Parameter(1., 1., 0., rcmax)
+    sysNbInter = SystemNonBondedInteraction(); sysNbInter.add(0, 0, ljparam)
+    sys.initTopology(types); sys.initDeviceData()
+    assign = AssignRCut(); assign.reinit(sys, (1, 1, nz)); assign.getRCut(table)
+    assign.assign(sys)
+    clist = CellList(sys, rlist); nlist = NeighborList(sysNbInter, sys, rlist, skin)
+    nlist.rebuild(sys, clist)
+    inter = InteractionEngine(sys); inter.registNonBondedInteraction(sysNbInter)
+    st = MDStatistic(sys); inter.clearInteraction(sys)
+    inter.applyNonBondedInteraction(sys, nlist, st)
+    st.updateHost(); st.nonBondedEnergy()
+
+Semantics kept: forces accumulate (+=) into FP32 SoA device arrays
+``sys.forcx/forcy/forcz`` after ``clearInteraction`` zeroes them
+(H/src/InteractionEngine_interface.cu:218-225, 1270-1273); energy and the
+virial diagonal accumulate into the statistic (H/include/Statistic_interface.h).
+Differences: the cell list needs no separate rebuild (binning is part of
+``NeighborList.rebuild``), ``normalizeDeviceData`` is a no-op (wrapping is
+exact), and rc assignment happens at every list rebuild.
+"""
+import numpy as np
+
+from .engine import Engine
+
+
+def read_gro(path):
+    """GROMACS .gro: positions (columns 21-44) and the box (last line)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].split()[0])
+    xyz = np.empty((n, 3), np.float32)
+    for i in range(n):
+        ln = lines[2 + i]
+        xyz[i] = [float(ln[20:28]), float(ln[28:36]), float(ln[36:44])]
+    box = np.array([float(v) for v in lines[2 + n].split()[:3]], np.float32)
+    return xyz, box
+
+
+class LennardJones6_12Parameter:
+    """{epsilon, sigma, shift, rcut} (H/include/NonBondedInteraction.h:19-42)."""
+
+    def __init__(self, epsilon=1.0, sigma=1.0, shift=0.0, rcut=2.5):
+        self.reinit(epsilon, sigma, shift, rcut)
+
+    def reinit(self, epsilon, sigma, shift, rcut):
+        self.epsilon, self.sigma, self.shift, self.rc = (float(epsilon), float(sigma),
+                                                        float(shift), float(rcut))
+
+    def rcut(self):
+        return self.rc
+
+
+class SystemNonBondedInteraction:
+    """Type-pair table (H/src/SystemNonBondedInteraction.cu:125-330)."""
+
+    def __init__(self):
+        self.table = {}
+
+    def add(self, i, j, param):
+        self.table[(min(i, j), max(i, j))] = param
+
+    def numAtomTypes(self):
+        return 1 + max(max(k) for k in self.table) if self.table else 0
+
+    def maxRcut(self):
+        return max(p.rcut() for p in self.table.values())
+
+    def matrices(self):
+        nt = self.numAtomTypes()
+        eps = np.zeros((nt, nt), np.float32)
+        sig = np.ones((nt, nt), np.float32)
+        sh = np.zeros((nt, nt), np.float32)
+        for (a, b), p in self.table.items():
+            for x, y in ((a, b), (b, a)):
+                eps[x, y], sig[x, y], sh[x, y] = p.epsilon, p.sigma, p.shift
+        return eps, sig, sh
+
+
+class MDSystem:
+    """Positions, types and box; owns the engine (H/include/MDSystem_interface.h)."""
+
+    def __init__(self):
+        self.xyz = None
+        self.box = None
+        self.types = None
+        self.engine = None
+        self._skin = 0.3
+        self._rc = None
+        self._nb = None
+        self.forcx = self.forcy = self.forcz = None
+
+    @property
+    def numAtom(self):
+        return 0 if self.xyz is None else len(self.xyz)
+
+    def initConfig(self, filename_or_xyz, box=None):
+        if isinstance(filename_or_xyz, str):
+            self.xyz, self.box = read_gro(filename_or_xyz)
+        else:
+            self.xyz = np.ascontiguousarray(filename_or_xyz, np.float32)[:, :3]
+            self.box = np.asarray(box, np.float32)
+
+    def initTopology(self, types=None):
+        self.types = (np.zeros(self.numAtom, np.int32) if types is None
+                      else np.ascontiguousarray(types, np.int32))
+
+    def initDeviceData(self):
+        import torch
+        self._pos = torch.zeros((self.numAtom, 4), dtype=torch.float32, device="cuda")
+        self._pos[:, :3] = torch.from_numpy(self.xyz).cuda()
+        self.forcx = torch.zeros(self.numAtom, dtype=torch.float32, device="cuda")
+        self.forcy = torch.zeros_like(self.forcx)
+        self.forcz = torch.zeros_like(self.forcx)
+        torch.cuda.synchronize()
+
+    @property
+    def coord(self):
+        """Device positions (N, 4) float32, like sys.ddata.coord."""
+        return self._pos
+
+    def normalizeDeviceData(self):
+        pass
+
+    def _engine(self, n_types, skin):
+        if self.engine is None:
+            self.engine = Engine(n_types=n_types, skin=skin)
+            self.engine.set_system([self.numAtom], self.box.reshape(1, 3))
+            self.engine.set_types(self.types)
+            self.engine.bind_positions(self._pos)
+        return self.engine
+
+
+class AssignRCut:
+    """rc lattice assigned per atom (H/include/AssignRCut.h:29-37)."""
+
+    def __init__(self):
+        self.dims = (1, 1, 1)
+        self.table = None
+
+    def reinit(self, sys, dims, NThread=96):
+        self.dims = tuple(int(v) for v in dims)
+        self.sys = sys
+
+    def getRCut(self, table):
+        self.table = np.asarray(table, np.float32).reshape(self.dims)
+        self.sys._rc = self.table
+
+    def uniform(self, rc):
+        self.getRCut(np.full(self.dims, rc, np.float32))
+
+    def assign(self, sys):
+        sys._rc = self.table
+        if sys.engine is not None:
+            sys.engine.set_rc_table(self.table)
+
+
+class CellList:
+    """Kept for interface parity; binning runs inside NeighborList.rebuild."""
+
+    def __init__(self, sys, cellSize, NTcell=96, NTatom=96, bdir=7, divide=1):
+        self.cellSize = float(cellSize)
+
+    def rebuild(self, sys, timer=None):
+        pass
+
+
+class NeighborList:
+    """Verlet list with the per-atom radius rc_i + skin (H/include/NeighborList_interface.h)."""
+
+    def __init__(self, sysNbInter, sys, rlist, rlistExten, NThread=96, expansion=4.0):
+        self.nb = sysNbInter
+        self.skin = float(rlistExten)
+        self.sys = sys
+        sys._skin = self.skin
+
+    def rebuild(self, sys, clist=None, timer=None):
+        e = sys._engine(max(1, self.nb.numAtomTypes()), self.skin)
+        eps, sig, sh = self.nb.matrices()
+        e.set_pair_params(eps, sig, sh)
+        e.set_rc_table(sys._rc)
+        e.rebuild()
+
+    def calSumNeighbor(self):
+        return self.sys.engine.neighbor_count()[0]
+
+
+class MDStatistic:
+    """Energy and virial diagonal (H/include/Statistic_interface.h:30-117)."""
+
+    def __init__(self, sys):
+        self.sys = sys
+        self.clearDevice()
+
+    def clearDevice(self):
+        self.ddata = np.zeros(4)
+        self.hdata = np.zeros(4)
+
+    def _add(self, stats):
+        self.ddata += stats
+
+    def updateHost(self):
+        self.hdata = self.ddata.copy()
+
+    def nonBondedEnergy(self):
+        return float(self.hdata[0])
+
+    def virial(self):
+        return self.hdata[1:].copy()
+
+    def pressureXX(self, box, kinetic_xx=0.0):
+        V = float(np.prod(box))
+        return 2.0 / V * (kinetic_xx - 0.5 * self.hdata[1])
+
+
+class InteractionEngine:
+    """Force/energy/virial of the registered LJ table (H/include/InteractionEngine_interface.h)."""
+
+    def __init__(self, sys, NThread=96):
+        self.sys = sys
+        self.tail = False
+
+    def registNonBondedInteraction(self, sysNbInter):
+        self.nb = sysNbInter
+
+    def clearInteraction(self, sys):
+        sys.forcx.zero_()
+        sys.forcy.zero_()
+        sys.forcz.zero_()
+
+    def applyNonBondedInteraction(self, sys, nlist, st=None, excl=None, timer=None):
+        e = sys.engine
+        e.update_positions(check_skin=False)
+        e.compute(stats=st is not None, tail=self.tail)
+        e.add_forces_f32_soa(sys.forcx, sys.forcy, sys.forcz)
+        if st is not None:
+            st._add(e.stats()[0])
+
+
+class Displacement_max:
+    """Largest displacement since recordCoord (H/include/Displacement_interface.h:15-43)."""
+
+    def __init__(self, sys, NThread=96):
+        self.sys = sys
+
+    def recordCoord(self, sys):
+        pass  # the engine records positions at every list rebuild
+
+    def calMaxDisplacemant(self, sys, timer=None):
+        """Returns True when max |dr| > skin/2 (the reference compares on the host)."""
+        return sys.engine.update_positions(check_skin=True)

@@ -1,0 +1,233 @@
+"""GPU checks of the rows around the core path: tail correction, batched
+replicas, position refresh and skin test, hipGraph replay, adjoint, autotuner
+and the reference-style float accumulation output.  Everything goes through
+the C ABI; the FP64 oracle is the checker."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sre import Engine, SkinExceeded, configs
+
+pytestmark = pytest.mark.gpu
+
+TOL_F_MIXED = 1e-5
+
+
+def rel_force_err(F, ref):
+    rms = np.sqrt((ref ** 2).sum(1).mean())
+    return np.sqrt(((F - ref) ** 2).sum(1)).max() / rms
+
+
+def oracle_system(w, r=0, xyz=None, tail=False, exact=False):
+    sl = w.replica_slice(r)
+    x = w.xyz[sl] if xyz is None else xyz
+    box, types = w.boxes[r], w.types[sl]
+    rc, _ = O.assign_rc(x, box, w.rc_tables[r])
+    off, j = O.build_pairs(x, box, rc, w.skin)
+    tl = (O.tail_atoms(x, box, types, w.eps, w.sigma, w.rc_tables[r].ravel(), w.rho[r],
+                       exact=exact) if tail else None)
+    F, E, W, nin = O.forces(x, box, types, w.eps, w.sigma, rc, off, j, tail=tl)
+    return dict(rc=rc, off=off, j=j, F=F, E=E, W=W, tail=tl)
+
+
+@pytest.fixture(scope="module")
+def slab():
+    return configs.slab(seed=31, n=12000, box=(18.0, 18.0, 64.0), nz=96, name="slab_12k")
+
+
+@pytest.fixture(scope="module")
+def binary():
+    return configs.slab(seed=32, n=12000, box=(18.0, 18.0, 64.0), nz=96, binary=True,
+                        name="binary_12k")
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+@pytest.mark.parametrize("exact", [False, True])
+def test_tail_correction_matches_oracle(torch_cuda, slab, prec, exact):
+    e = Engine(skin=slab.skin, precision=prec)
+    slab.setup(e, tail=True)
+    e.set_tail_mode(exact=exact)
+    e.rebuild()
+    e.compute(stats=True, tail=True)
+    ref = oracle_system(slab, tail=True, exact=exact)
+    F = e.forces()
+    tol = TOL_F_MIXED if prec == "mixed" else 1e-11
+    assert rel_force_err(F, ref["F"]) <= tol
+    a = e.atom_stats()
+    np.testing.assert_allclose(a[:, 0], ref["E"], rtol=1e-4 if prec == "mixed" else 1e-10,
+                               atol=1e-5 if prec == "mixed" else 1e-11)
+    # the tail term is really there: it moves the z forces by the oracle's amount
+    e.compute(stats=True, tail=False)
+    F0 = e.forces()
+    dz = F[:, 2] - F0[:, 2]
+    np.testing.assert_allclose(dz, ref["tail"][:, 0], rtol=1e-6, atol=1e-9)
+
+
+def test_binary_mixture_matches_oracle(torch_cuda, binary):
+    for prec, tol in (("mixed", TOL_F_MIXED), ("fp64", 1e-11)):
+        e = Engine(n_types=2, skin=binary.skin, precision=prec)
+        binary.setup(e, tail=True)
+        e.rebuild()
+        e.compute(stats=True, tail=True)
+        ref = oracle_system(binary, tail=True)
+        goff, gj = e.export_pairs()
+        assert np.array_equal(goff, ref["off"]) and np.array_equal(gj, ref["j"])
+        assert rel_force_err(e.forces(), ref["F"]) <= tol
+        st = e.stats()[0]
+        assert abs(st[0] - ref["E"].sum()) / np.abs(ref["E"]).sum() <= 1e-6
+
+
+def test_batched_replicas_match_oracle_per_replica(torch_cuda):
+    w = configs.replicas(seed=9, count=12, n_lo=2000, n_hi=6000)
+    e = Engine(skin=w.skin)
+    w.setup(e, tail=True)
+    e.rebuild()
+    e.compute(stats=True, tail=True)
+    F = e.forces()
+    st = e.stats()
+    goff, gj = e.export_pairs()
+    for r in range(len(w.n_atoms)):
+        sl = w.replica_slice(r)
+        ref = oracle_system(w, r, tail=True)
+        o = goff[sl.start:sl.stop + 1] - goff[sl.start]
+        jj = gj[goff[sl.start]:goff[sl.stop]] - sl.start
+        assert np.array_equal(o, ref["off"]) and np.array_equal(jj, ref["j"]), r
+        assert rel_force_err(F[sl], ref["F"]) <= TOL_F_MIXED, r
+        assert abs(st[r, 0] - ref["E"].sum()) / np.abs(ref["E"]).sum() <= 1e-6, r
+        np.testing.assert_allclose(st[r, 1:], ref["W"].sum(0), rtol=1e-6)
+
+
+def test_update_positions_and_skin(torch_cuda, slab):
+    import torch
+    e = Engine(skin=slab.skin)
+    slab.setup(e, tail=False)
+    pos = torch.from_numpy(slab.xyz.copy()).cuda()
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    # small moves: no rebuild needed, forces at the new positions with the old list
+    rng = np.random.default_rng(5)
+    moved = slab.xyz.copy()
+    moved[:, :3] += rng.normal(scale=0.02, size=(slab.n, 3)).astype(np.float32)
+    pos.copy_(torch.from_numpy(moved))
+    torch.cuda.synchronize()
+    assert e.update_positions() is False
+    e.compute(stats=True)
+    ref = oracle_system(slab, xyz=moved)
+    # the list was built at the old positions; with the skin it still holds every
+    # pair inside rc at the new ones, so the force sets agree
+    assert rel_force_err(e.forces(), ref["F"]) <= TOL_F_MIXED
+    # a large move trips the skin test, and step() then rebuilds
+    moved[7, :3] += np.float32(0.2)
+    pos.copy_(torch.from_numpy(moved))
+    torch.cuda.synchronize()
+    assert e.update_positions() is True
+    assert e.step(stats=True) is True
+    ref = oracle_system(slab, xyz=moved)
+    goff, gj = e.export_pairs()
+    assert np.array_equal(goff, ref["off"]) and np.array_equal(gj, ref["j"])
+    assert rel_force_err(e.forces(), ref["F"]) <= TOL_F_MIXED
+
+
+def test_graph_replay_equals_eager_and_skin_flag(torch_cuda, slab):
+    import torch
+    e = Engine(skin=slab.skin)
+    slab.setup(e, tail=True)
+    pos = torch.from_numpy(slab.xyz.copy()).cuda()
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    e.set_perturbation(0.01, 123, 0)
+    results = []
+    for rep in range(3):
+        e.graph_run(1, stats=True, tail=True, perturb=True)
+        f_graph = e.forces_fixed()
+        s_graph = e.stats()
+        e.update_positions(check_skin=False)       # eager, same positions
+        e.compute(stats=True, tail=True)
+        assert np.array_equal(e.forces_fixed(), f_graph)
+        np.testing.assert_array_equal(e.stats(), s_graph)
+        results.append(f_graph)
+    assert not np.array_equal(results[0], results[1])   # the positions moved
+    e.check_skin_flag()
+    # 200 more perturbation steps without a rebuild must trip the flag
+    e.graph_run(200, stats=False, tail=False, perturb=True)
+    with pytest.raises(SkinExceeded):
+        e.check_skin_flag()
+
+
+def test_perturbation_is_reproducible(torch_cuda, slab):
+    import torch
+    outs = []
+    for _ in range(2):
+        e = Engine(skin=slab.skin)
+        slab.setup(e, tail=False)
+        pos = torch.from_numpy(slab.xyz.copy()).cuda()
+        torch.cuda.synchronize()
+        e.bind_positions(pos)
+        e.rebuild()
+        e.set_perturbation(0.01, 99, 0)
+        e.graph_run(5, perturb=True)
+        e.synchronize()
+        outs.append(pos.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    d = outs[0][:, :3] - slab.xyz[:, :3]
+    assert abs(d.std() / np.sqrt(5) - 0.01) < 5e-4
+
+
+def test_add_forces_f32_soa_accumulates(torch_cuda, slab):
+    import torch
+    e = Engine(skin=slab.skin)
+    slab.setup(e, tail=False)
+    e.rebuild()
+    e.compute()
+    F = e.forces()
+    fx = torch.ones(slab.n, device="cuda")
+    fy = torch.zeros(slab.n, device="cuda")
+    fz = torch.zeros(slab.n, device="cuda")
+    e.add_forces_f32_soa(fx, fy, fz)
+    e.synchronize()
+    np.testing.assert_allclose(fx.cpu().numpy(), (F[:, 0] + 1).astype(np.float32), rtol=1e-6,
+                               atol=1e-6)
+    np.testing.assert_allclose(fz.cpu().numpy(), F[:, 2].astype(np.float32), rtol=1e-6)
+
+
+def test_autotune_keeps_forces_bitwise(torch_cuda, tmp_path, slab):
+    e = Engine(skin=slab.skin)
+    slab.setup(e, tail=True)
+    e.rebuild()
+    e.compute(stats=True, tail=True)
+    f0 = e.forces_fixed()
+    cache = str(tmp_path / "tune.txt")
+    t = e.autotune(stats=True, tail=True, cache_path=cache)
+    e.compute(stats=True, tail=True)
+    assert np.array_equal(e.forces_fixed(), f0)
+    # a second engine reads the cache and lands on the same tuning
+    e2 = Engine(skin=slab.skin)
+    slab.setup(e2, tail=True)
+    t2 = e2.autotune(stats=True, tail=True, cache_path=cache)
+    assert t2 == t
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_adjoint_matches_oracle_gradient(torch_cuda, binary, prec):
+    import torch
+    w = binary
+    e = Engine(n_types=2, skin=w.skin, precision=prec)
+    w.setup(e, tail=True)
+    e.rebuild()
+    rng = np.random.default_rng(17)
+    g = rng.normal(size=(w.n, 3))
+    dLdE = 0.37
+    ge, gs, grc = e.adjoint(dLdE=[dLdE], dLdF=torch.from_numpy(g).cuda(), tail=True,
+                            n_rc=w.rc_tables[0].size)
+    box = w.boxes[0]
+    rc, _ = O.assign_rc(w.xyz, box, w.rc_tables[0])
+    off, j = O.build_pairs(w.xyz, box, rc, w.skin)
+    pe, ps = O.grad_pairs(w.xyz, box, w.types, w.eps, w.sigma, rc, off, j, dLdE, g)
+    te, ts, trc = O.grad_tail(w.xyz, box, w.types, w.eps, w.sigma, w.rc_tables[0].ravel(),
+                              w.rho[0], dLdE, g, h=1e-4)
+    tol = 2e-5 if prec == "mixed" else 1e-7
+    np.testing.assert_allclose(ge, pe + te, rtol=tol, atol=tol * np.abs(pe).max())
+    np.testing.assert_allclose(gs, ps + ts, rtol=tol, atol=tol * np.abs(ps).max())
+    np.testing.assert_allclose(grc, trc, rtol=1e-6, atol=1e-6 * np.abs(trc).max())

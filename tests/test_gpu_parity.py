@@ -113,7 +113,7 @@ def test_fp64_form_matches_oracle(torch_cuda, wname, request):
 def test_forces_bitwise_independent_of_tuning(torch_cuda, slab16):
     base = None
     for fl, bl, cxy, cz, blk in [(4, 8, 0.5, 0.5, 256), (1, 4, 0.35, 0.25, 64),
-                                 (2, 16, 0.7, 1.0, 128), (8, 2, 1.0, 0.35, 512),
+                                 (2, 16, 0.7, 1.0, 128), (8, 2, 1.0, 0.35, 128),
                                  (16, 1, 0.5, 0.5, 256)]:
         e = engine_for(slab16, force_lanes=fl, build_lanes=bl, cell_xy=cxy, cell_z=cz,
                        block=blk)

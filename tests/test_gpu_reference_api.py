@@ -1,0 +1,95 @@
+"""The reference's driver sequence (H/app/lj.nhc-xtc.cu:72-113, 169-188) through
+the reference-named API, on the reference's own 16000-atom slab configuration
+(conf.16000.gro), and the torch.autograd bridge of the adjoint."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sre import reference_api as R
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reference_driver_sequence_on_conf16000(torch_cuda):
+    d = np.load(os.path.join(GOLD, "conf16000.npz"))
+    xyz, box = d["xyz"], d["box"]
+    rcmax, skin, nz = 5.0, 0.49, 150
+    sys = R.MDSystem()
+    sys.initConfig(xyz, box)
+    ljparam = R.LennardJones6_12Parameter(1.0, 1.0, 0.0, rcmax)
+    nb = R.SystemNonBondedInteraction()
+    nb.add(0, 0, ljparam)
+    sys.initTopology()
+    sys.initDeviceData()
+    # an adaptive profile along x (the slab normal of this configuration)
+    table = np.where(np.abs((np.arange(nz) + 0.5) - 75) < 20, 3.5, 5.0).astype(np.float32)
+    arc = R.AssignRCut()
+    arc.reinit(sys, (nz, 1, 1))
+    arc.getRCut(table)
+    arc.assign(sys)
+    clist = R.CellList(sys, rcmax + skin)
+    nlist = R.NeighborList(nb, sys, rcmax + skin, skin)
+    sys.normalizeDeviceData()
+    clist.rebuild(sys)
+    nlist.rebuild(sys, clist)
+    inter = R.InteractionEngine(sys)
+    inter.registNonBondedInteraction(nb)
+    st = R.MDStatistic(sys)
+    st.clearDevice()
+    inter.clearInteraction(sys)
+    inter.applyNonBondedInteraction(sys, nlist, st)
+    st.updateHost()
+
+    rc, _ = O.assign_rc(xyz, box, table.reshape(nz, 1, 1))
+    off, j = O.build_pairs(xyz, box, rc, skin)
+    F, E, W, _ = O.forces(xyz, box, None, np.ones((1, 1)), np.ones((1, 1)), rc, off, j)
+    assert nlist.calSumNeighbor() == off[-1]
+    Fg = np.stack([sys.forcx.cpu().numpy(), sys.forcy.cpu().numpy(),
+                   sys.forcz.cpu().numpy()], 1).astype(np.float64)
+    rms = np.sqrt((F ** 2).sum(1).mean())
+    # float32 accumulation targets (the reference's own force arrays are FP32)
+    assert np.sqrt(((Fg - F) ** 2).sum(1)).max() / rms < 1e-5
+    assert abs(st.nonBondedEnergy() - E.sum()) / np.abs(E).sum() < 1e-6
+    np.testing.assert_allclose(st.virial(), W.sum(0), rtol=1e-6)
+    # the skin test of the reference loop: nothing moved
+    assert R.Displacement_max(sys).calMaxDisplacemant(sys) is False
+
+
+def test_autograd_bridge_matches_finite_differences(torch_cuda):
+    import torch
+    from sre import Engine, configs
+    from sre.autograd import lj_energy_forces
+    w = configs.slab(seed=41, n=6000, box=(16.0, 16.0, 50.0), nz=48, binary=True,
+                     name="binary_6k")
+    e = Engine(n_types=2, skin=w.skin, precision="fp64")
+    w.setup(e, tail=True)
+    e.rebuild()
+    eps = torch.tensor(w.eps, dtype=torch.float64, requires_grad=True)
+    sig = torch.tensor(w.sigma, dtype=torch.float64, requires_grad=True)
+    E, F = lj_energy_forces(e, eps, sig, None, True)
+    Ft = torch.from_numpy(np.random.default_rng(2).normal(size=(w.n, 3))).cuda() + F.detach()
+    loss = 0.1 * E.sum() + ((F - Ft) ** 2).sum()
+    loss.backward()
+
+    def L(ep, sg):
+        with torch.no_grad():
+            E2, F2 = lj_energy_forces(e, ep, sg, None, True)
+            return float(0.1 * E2.sum() + ((F2 - Ft) ** 2).sum())
+
+    h = 2.0 ** -12
+    for a, b in [(0, 0), (0, 1), (1, 1)]:
+        for which, grad in (("eps", eps.grad), ("sig", sig.grad)):
+            vals = []
+            for sgn in (1, -1):
+                ep, sg = eps.detach().clone(), sig.detach().clone()
+                p = ep if which == "eps" else sg
+                p[a, b] += sgn * h
+                p[b, a] = p[a, b]
+                # parameters travel as FP32: step by what FP32 represents
+                vals.append((L(ep, sg), float(np.float32(p[a, b].item()))))
+            fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])
+            g = float(grad[a, b])
+            assert abs(fd - g) <= 2e-4 * max(1.0, abs(g)), (which, a, b, fd, g)
