@@ -51,15 +51,20 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
         done = 0
         pairs = 0
         rebuilds = 0
+        nl, _, _ = e.neighbor_count()         # known since the last rebuild
         while done < n_steps:
             n = min(every, n_steps - done)
-            nl, _, _ = e.neighbor_count()
             e.graph_run(n, stats=True, tail=True, perturb=True)
             pairs += nl * n
             done += n
-            e.check_skin_flag()       # syncs; raises if the skin was exceeded
-            e.rebuild()
+            e.rebuild()                       # one synchronisation per rebuild
             rebuilds += 1
+            # the skin test recorded during the segment: every atom must have
+            # stayed within skin/2 of its position at the previous rebuild
+            disp = e.last_rebuild_displacement()
+            if disp > 0.5 * w.skin:
+                raise RuntimeError(f"skin exceeded between rebuilds ({disp:.4f})")
+            nl, _, _ = e.neighbor_count()
         return pairs, rebuilds
 
     segment_loop(warmup)
@@ -147,6 +152,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--workload", default="slab_64k",
+                    help="slab_64k (the metric's workload), binary_slab_64k, large_slab_2M")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
@@ -160,7 +169,17 @@ def main():
     if not torch.cuda.is_available():
         log("bench.py needs a GPU")
         sys.exit(1)
-    w = configs.slab()
+    if args.calibrate_copy:
+        # known traffic: each copy reads and writes MIB MiB once
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 5 copies of {args.calibrate_copy} MiB")
+        return
+    w = configs.WORKLOADS[args.workload]()
     log(f"[bench] workload {w.name}: N={w.n}, {w.notes}")
     mixed = run_form(w, "mixed", args.steps, args.warmup, args.rebuild_every, torch, Engine)
     log(f"[bench] mixed: {mixed['pair_evals_per_s']:.4g} pair evals/s, "
@@ -200,8 +219,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 pair math, i64 fixed-point forces, f64 energy/virial",
         "data": "synthetic (seeded random-sequential-addition slab)",
-        "config": {"workload": w.name, "n_atoms": N, "box": [30, 30, 120],
-                   "rho_liquid_vapour": [0.73, 0.02], "rc_z": "2.5->5.0, 256 bins",
+        "config": {"workload": w.name, "n_atoms": N,
+                   "box": [float(v) for v in w.boxes[0]], "n_types": w.n_types,
+                   "rho_liquid_vapour": [0.73, 0.02], "geometry": w.notes,
                    "skin": w.skin, "rebuild_every": args.rebuild_every,
                    "perturbation_sigma": PERTURB_SIGMA, "tail": "per-atom, bin mode",
                    "list_entries": L, "pairs_in_rc": mixed["n_in_rc"],

@@ -228,6 +228,11 @@ sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
 sre_status sre_set_perturbation(sre_engine *e, float sigma, uint64_t seed,
                                 uint64_t first_step);
 sre_status sre_check_skin_flag(sre_engine *e);
+/* Largest displacement of any atom between the previous rebuild and the last
+ * one, as recorded by the skin test (read back by sre_rebuild itself, no
+ * extra synchronisation).  A value above skin/2 means forces computed before
+ * the last rebuild may have missed pairs. */
+sre_status sre_last_rebuild_displacement(sre_engine *e, float *max_disp);
 
 /* Gaussian displacement of the bound positions (device), std sigma, seeded,
  * counter-based: x += sigma * N(0,1).  The Langevin-style perturbation the

@@ -72,6 +72,7 @@ struct sre_engine {
   std::vector<int> rep_n;
   std::vector<float> box;
   std::vector<RepInfo> reps;
+  std::vector<RepInfo> reps_uploaded;  // device copy of reps, to skip unchanged uploads
   RepInfo *d_reps = nullptr;
   int *d_rep_of = nullptr;
   int *d_types = nullptr;
@@ -126,7 +127,9 @@ struct sre_engine {
   size_t list_size = 0;
   int *d_list = nullptr;
   int *d_nnb = nullptr;
-  int *d_maxcount = nullptr;
+  unsigned long long *d_counters = nullptr;  // [0] longest list, [1] total
+  uint64_t *h_info = nullptr;               // pinned readback (4 words)
+  float last_disp2 = 0.f;                   // max displacement^2 before the last rebuild
   int max_count = 0;
   int64_t n_list = 0;
 
@@ -165,7 +168,8 @@ sre_engine::~sre_engine() {
   dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
   dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
   dfree(d_rcut); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
-  dfree(d_maxcount); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
+  dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
+  if (h_info) (void)hipHostFree(h_info);
   dfree(d_fyd); dfree(d_fzd); dfree(d_wave_part); dfree(d_stats);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -220,7 +224,8 @@ extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
   e->shift.assign(e->n_types * e->n_types, 0.f);
   hipError_t r = dalloc(&e->d_flags, 4);
   if (r == hipSuccess) r = hipMemsetAsync(e->d_flags, 0, 4 * sizeof(unsigned), e->stream);
-  if (r == hipSuccess) r = dalloc(&e->d_maxcount, 1);
+  if (r == hipSuccess) r = dalloc(&e->d_counters, 2);
+  if (r == hipSuccess) r = hipHostMalloc((void **)&e->h_info, 4 * sizeof(uint64_t), hipHostMallocDefault);
   if (r == hipSuccess) r = dalloc(&e->d_params, kMaxPairTypes);
   if (r == hipSuccess) r = dalloc(&e->d_paramsd, kMaxPairTypes);
   if (r != hipSuccess) {
@@ -264,6 +269,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   e->rep_n.assign(n_atoms, n_atoms + n_replicas);
   e->box.assign(box, box + 3 * n_replicas);
   e->reps.assign(n_replicas, RepInfo{});
+  e->reps_uploaded.clear();
   int off = 0;
   for (int r = 0; r < n_replicas; ++r) {
     RepInfo &R = e->reps[r];
@@ -607,8 +613,13 @@ static sre_status prepare_tables(sre_engine *e) {
     }
     e->graph_epoch++;
   }
-  HIP_TRY(hipMemcpyAsync(e->d_reps, e->reps.data(), sizeof(RepInfo) * e->n_rep,
-                         hipMemcpyHostToDevice, e->stream));
+  if (e->reps_uploaded.size() != e->reps.size() ||
+      memcmp(e->reps_uploaded.data(), e->reps.data(),
+             sizeof(RepInfo) * e->reps.size()) != 0) {
+    HIP_TRY(hipMemcpy(e->d_reps, e->reps.data(), sizeof(RepInfo) * e->n_rep,
+                      hipMemcpyHostToDevice));
+    e->reps_uploaded = e->reps;
+  }
   if (e->tail_dirty || e->tail_values_dirty) {
     if (e->tail_bins > 0)
       launch_tail_table(e->tail_bins * e->n_types, e->d_reps,
@@ -691,25 +702,33 @@ static sre_status do_rebuild(sre_engine *e) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     s = ensure_list(e, cap);
     if (s != SRE_OK) return s;
-    HIP_TRY(hipMemsetAsync(e->d_maxcount, 0, sizeof(int), st));
+    launch_fill_i64((int64_t *)e->d_counters, 0, 2, st);
     launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
                       e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                      e->d_list, e->d_nnb, e->d_maxcount, st);
+                      e->d_list, e->d_nnb, e->d_counters, st);
     HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
-    int mc = 0;
-    HIP_TRY(hipMemcpyAsync(&mc, e->d_maxcount, sizeof(int),
+    // one pinned readback per rebuild: longest list, total length and the
+    // displacement recorded since the previous rebuild
+    HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 2 * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(e->h_info + 2, e->d_flags, sizeof(uint64_t),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const int mc = (int)e->h_info[0];
     e->max_count = mc;
+    e->n_list = (int64_t)e->h_info[1];
     if (mc <= e->cap) break;
     cap = round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes);
     if (attempt == 3)
       return fail(SRE_ERR_LIST_OVERFLOW, "neighbor list overflow (%d > %d)", mc,
                   e->cap);
   }
-  HIP_TRY(hipMemsetAsync(e->d_flags, 0, sizeof(unsigned), st));
+  {
+    const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
+    memcpy(&e->last_disp2, &bits, sizeof(float));
+  }
+  launch_fill_u32(e->d_flags, 0u, 1, st);
   e->built = true;
-  e->n_list = -1;
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   return SRE_OK;
 }
@@ -1338,6 +1357,13 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
   for (int i = 0; i < n_steps; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
   e->forces_valid = true;
   e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_last_rebuild_displacement(sre_engine *e,
+                                                    float *max_disp) {
+  if (!e || !max_disp) return fail(SRE_ERR_INVALID, "null argument");
+  *max_disp = sqrtf(e->last_disp2);
   return SRE_OK;
 }
 

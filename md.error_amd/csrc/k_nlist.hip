@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_build_list(
     int n, int TF, const uint4 *__restrict__ pos4,
     const float *__restrict__ rcut, const RepInfo *__restrict__ reps,
     const int *__restrict__ start, float skin, int cap, int *__restrict__ list,
-    int *__restrict__ nnb, int *maxcount) {
+    int *__restrict__ nnb, unsigned long long *counters) {
   constexpr int A = kWave / TB;
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / TB, l = lane % TB;
@@ -113,16 +113,25 @@ __global__ __launch_bounds__(256) void k_build_list(
     }
     if (l == 0) nnb[s] = count;
   }
+  // counters[0]: longest list, counters[1]: total list length (integer
+  // atomics, one per wave: order independent)
   int m = count;
+  int t = (l == 0) ? count : 0;
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
-  if (lane == 0 && m > 0) atomicMax(maxcount, m);
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    m = max(m, __shfl_xor(m, o, kWave));
+    t += __shfl_xor(t, o, kWave);
+  }
+  if (lane == 0 && m > 0) {
+    atomicMax(&counters[0], (unsigned long long)m);
+    atomicAdd(&counters[1], (unsigned long long)t);
+  }
 }
 
 void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
-                       float skin, int cap, int *list, int *nnb, int *maxcount,
-                       hipStream_t st) {
+                       float skin, int cap, int *list, int *nnb,
+                       unsigned long long *counters, hipStream_t st) {
   if (n <= 0) return;
   const int A = kWave / lanes;
   const int waves = (n + A - 1) / A;
@@ -131,7 +140,7 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
 #define SRE_BUILD(TB)                                                        \
   hipLaunchKernelGGL(k_build_list<TB>, dim3(grid), dim3(block), 0, st, n,   \
                      force_lanes, pos4, rcut, reps, start, skin, cap, list, \
-                     nnb, maxcount)
+                     nnb, counters)
   switch (lanes) {
     case 1: SRE_BUILD(1); break;
     case 2: SRE_BUILD(2); break;

@@ -153,8 +153,8 @@ void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
                           float *rcut, int *tail_idx, hipStream_t st);
 void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
-                       float skin, int cap, int *list, int *nnb, int *maxcount,
-                       hipStream_t st);
+                       float skin, int cap, int *list, int *nnb,
+                       unsigned long long *counters, hipStream_t st);
 // step_ctr (optional) is advanced by one thread after the positions are read
 void launch_update_positions(int n, const uint32_t *perm, const float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,

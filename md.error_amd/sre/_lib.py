@@ -80,6 +80,7 @@ SIGNATURES = {
     "sre_graph_run": (_st, [_vp, C.c_uint, C.c_int]),
     "sre_set_perturbation": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64]),
     "sre_check_skin_flag": (_st, [_vp]),
+    "sre_last_rebuild_displacement": (_st, [_vp, _f32p]),
     "sre_perturb_positions": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64]),
     "sre_time_compute": (_st, [_vp, C.c_uint, C.c_int, _f64p]),
     "sre_time_rebuild": (_st, [_vp, C.c_int, _f64p]),

@@ -289,6 +289,12 @@ class Engine:
     def check_skin_flag(self):
         check(lib().sre_check_skin_flag(self._h))
 
+    def last_rebuild_displacement(self):
+        """Max displacement between the previous rebuild and the last one."""
+        v = C.c_float()
+        check(lib().sre_last_rebuild_displacement(self._h, C.byref(v)))
+        return v.value
+
     def perturb(self, sigma, seed, step):
         check(lib().sre_perturb_positions(self._h, float(sigma), int(seed), int(step)))
 

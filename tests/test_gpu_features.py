@@ -18,11 +18,13 @@ def rel_force_err(F, ref):
     return np.sqrt(((F - ref) ** 2).sum(1)).max() / rms
 
 
-def oracle_system(w, r=0, xyz=None, tail=False, exact=False):
+def oracle_system(w, r=0, xyz=None, tail=False, exact=False, rc_xyz=None):
+    """rc_xyz: positions at which rc was assigned (the engine assigns rc at
+    rebuild time, like AssignRCut::assign); defaults to the evaluated ones."""
     sl = w.replica_slice(r)
     x = w.xyz[sl] if xyz is None else xyz
     box, types = w.boxes[r], w.types[sl]
-    rc, _ = O.assign_rc(x, box, w.rc_tables[r])
+    rc, _ = O.assign_rc(x if rc_xyz is None else rc_xyz, box, w.rc_tables[r])
     off, j = O.build_pairs(x, box, rc, w.skin)
     tl = (O.tail_atoms(x, box, types, w.eps, w.sigma, w.rc_tables[r].ravel(), w.rho[r],
                        exact=exact) if tail else None)
@@ -113,9 +115,9 @@ def test_update_positions_and_skin(torch_cuda, slab):
     torch.cuda.synchronize()
     assert e.update_positions() is False
     e.compute(stats=True)
-    ref = oracle_system(slab, xyz=moved)
-    # the list was built at the old positions; with the skin it still holds every
-    # pair inside rc at the new ones, so the force sets agree
+    ref = oracle_system(slab, xyz=moved, rc_xyz=slab.xyz)
+    # the list and rc were assigned at the old positions; with the skin the list
+    # still holds every pair inside rc at the new ones, so the force sets agree
     assert rel_force_err(e.forces(), ref["F"]) <= TOL_F_MIXED
     # a large move trips the skin test, and step() then rebuilds
     moved[7, :3] += np.float32(0.2)

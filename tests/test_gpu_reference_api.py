@@ -79,7 +79,9 @@ def test_autograd_bridge_matches_finite_differences(torch_cuda):
             E2, F2 = lj_energy_forces(e, ep, sg, None, True)
             return float(0.1 * E2.sum() + ((F2 - Ft) ** 2).sum())
 
-    h = 2.0 ** -12
+    # a small step: the loss is quadratic in F and F ~ sigma^12, so the
+    # third-order term of the central difference is large at h = 2^-12
+    h = 2.0 ** -17
     for a, b in [(0, 0), (0, 1), (1, 1)]:
         for which, grad in (("eps", eps.grad), ("sig", sig.grad)):
             vals = []
