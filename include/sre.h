@@ -209,6 +209,9 @@ typedef struct {
   int block;            /* threads per block of the force kernel: 64..1024 */
   float cell_xy;        /* cell edge in x,y as a multiple of (rc_min + skin) */
   float cell_z;         /* cell edge in z as a multiple of (rc_min + skin) */
+  int build_mode;       /* 0: per-atom column search (build_lanes lanes per atom);
+                           1: one wavefront per 64-atom tile of a cell column */
+  int xcd_remap;        /* 1: map contiguous atom ranges to each XCD (L2 locality) */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */
@@ -225,8 +228,14 @@ sre_status sre_autotune(sre_engine *e, unsigned flags, const char *cache_path);
  * sre_set_perturbation, advancing its step counter on the device. */
 #define SRE_PERTURB 4u
 sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
+/* Perturbation of the bound positions applied by SRE_PERTURB replays, one
+ * Gaussian draw (std sigma per component, counter-based on seed and step) per
+ * replay.  anchored = 0: a random walk, x += sigma N(0,1).  anchored = 1:
+ * vibration about the bound positions as they are at this call (copied to
+ * the device), x = x0 + sigma N(0,1), so displacements stay bounded however
+ * many steps run. */
 sre_status sre_set_perturbation(sre_engine *e, float sigma, uint64_t seed,
-                                uint64_t first_step);
+                                uint64_t first_step, int anchored);
 sre_status sre_check_skin_flag(sre_engine *e);
 /* Largest displacement of any atom between the previous rebuild and the last
  * one, as recorded by the skin test (read back by sre_rebuild itself, no

@@ -65,7 +65,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 8, 256, 0.5f, 0.5f};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1};
 
   // system
   int n_rep = 0, n_atoms = 0;
@@ -116,6 +116,9 @@ struct sre_engine {
   int n_cells = 0;
   int *d_start = nullptr;
   int start_cap = 0;
+  int n_cols = 0;                  // (x, y) cell columns over all replicas
+  uint32_t *d_tiles_per_col = nullptr, *d_tile_off = nullptr, *d_tile_tmp = nullptr;
+  int tile_cap = 0;
   uint4 *d_pos4 = nullptr, *d_backup = nullptr;
   float4 *d_res4 = nullptr;   // FP64 form only
   float *d_rcut = nullptr;
@@ -145,6 +148,10 @@ struct sre_engine {
   uint64_t *d_step_ctr = nullptr;  // perturbation step counter (graph mode)
   float pert_sigma = 0.f;
   uint64_t pert_seed = 0;
+  float *d_anchor = nullptr;       // anchored perturbation: positions at set-up
+  size_t anchor_cap = 0, anchor_count = 0;
+  int anchor_stride = 0;
+  bool pert_anchored = false;
   bool stats_valid = false;
   bool atom_stats_valid = false;
   bool forces_valid = false;
@@ -168,10 +175,12 @@ sre_engine::~sre_engine() {
   dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
   dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
   dfree(d_rcut); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
+  dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
   dfree(d_fyd); dfree(d_fzd); dfree(d_wave_part); dfree(d_stats);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
+  dfree(d_anchor);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -523,7 +532,7 @@ static sre_status prepare_tables(sre_engine *e) {
     e->rc_dirty = false;
   }
   // cell grids
-  long long cells = 0;
+  long long cells = 0, cols = 0;
   for (int r = 0; r < e->n_rep; ++r) {
     RepInfo &R = e->reps[r];
     const float rlmax = R.rc_max + e->skin;
@@ -550,11 +559,23 @@ static sre_status prepare_tables(sre_engine *e) {
       R.h[d] = R.L[d] / (float)nc[d];
     }
     R.cell_off = (int)cells;
+    R.col_off = (int)cols;
     cells += (long long)nc[0] * nc[1] * nc[2];
+    cols += (long long)nc[0] * nc[1];
   }
   if (cells >= (1ll << 31) - 2)
     return fail(SRE_ERR_CELL, "cell grid too large (%lld cells)", cells);
   e->n_cells = (int)cells;
+  e->n_cols = (int)cols;
+  if (e->tile_cap < e->n_cols) {
+    dfree(e->d_tiles_per_col);
+    dfree(e->d_tile_off);
+    dfree(e->d_tile_tmp);
+    HIP_TRY(dalloc(&e->d_tiles_per_col, e->n_cols));
+    HIP_TRY(dalloc(&e->d_tile_off, e->n_cols));
+    HIP_TRY(dalloc(&e->d_tile_tmp, e->n_cols / 4096 + 8));
+    e->tile_cap = e->n_cols;
+  }
   e->key_bits = std::max(1, bitlen((unsigned)(cells - 1)));
   if (e->key_bits > 32) return fail(SRE_ERR_CELL, "cell key exceeds 32 bits");
   if (e->start_cap < e->n_cells + 1) {
@@ -645,6 +666,20 @@ static void refresh_tail_values(sre_engine *e) {
   e->tail_values_dirty = false;
 }
 
+// Per-wave statistic partials of the force kernel for the current list
+// layout; allocated at rebuild so that no timed or captured region allocates.
+static sre_status ensure_wave_part(sre_engine *e) {
+  const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
+  if (need > e->wave_part_cap) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    dfree(e->d_wave_part);
+    HIP_TRY(dalloc(&e->d_wave_part, need));
+    e->wave_part_cap = need;
+    e->graph_epoch++;
+  }
+  return SRE_OK;
+}
+
 static sre_status ensure_list(sre_engine *e, int cap) {
   const int T = e->tune.force_lanes;
   const int A = kWave / T;
@@ -703,9 +738,15 @@ static sre_status do_rebuild(sre_engine *e) {
     s = ensure_list(e, cap);
     if (s != SRE_OK) return s;
     launch_fill_i64((int64_t *)e->d_counters, 0, 2, st);
-    launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
-                      e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                      e->d_list, e->d_nnb, e->d_counters, st);
+    if (e->tune.build_mode == 1)
+      launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
+                         e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
+                         e->d_list, e->d_nnb, e->d_counters, e->d_tiles_per_col,
+                         e->d_tile_off, e->d_tile_tmp, st);
+    else
+      launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
+                        e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
+                        e->d_list, e->d_nnb, e->d_counters, st);
     HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
     // one pinned readback per rebuild: longest list, total length and the
     // displacement recorded since the previous rebuild
@@ -728,6 +769,8 @@ static sre_status do_rebuild(sre_engine *e) {
     memcpy(&e->last_disp2, &bits, sizeof(float));
   }
   launch_fill_u32(e->d_flags, 0u, 1, st);
+  s = ensure_wave_part(e);
+  if (s != SRE_OK) return s;
   e->built = true;
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   return SRE_OK;
@@ -783,6 +826,7 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.wave_part = e->d_wave_part;
   a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
   a.err_flag = e->d_flags + 1;
+  a.xcd_remap = e->tune.xcd_remap;
   return a;
 }
 
@@ -791,14 +835,8 @@ static sre_status launch_compute(sre_engine *e, unsigned flags) {
   const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
   const int T = e->list_lanes;
   if (stats) {
-    const size_t need = (size_t)force_num_waves(e->n_atoms, T) * 12;
-    if (need > e->wave_part_cap) {
-      HIP_TRY(hipStreamSynchronize(e->stream));
-      dfree(e->d_wave_part);
-      HIP_TRY(dalloc(&e->d_wave_part, need));
-      e->wave_part_cap = need;
-      e->graph_epoch++;
-    }
+    sre_status sw = ensure_wave_part(e);
+    if (sw != SRE_OK) return sw;
   }
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
@@ -1119,8 +1157,13 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
   if (!(t->cell_xy >= 0.05f && t->cell_xy <= 4.f && t->cell_z >= 0.05f &&
         t->cell_z <= 4.f))
     return fail(SRE_ERR_INVALID, "cell sizes must be in [0.05, 4]");
+  if (t->build_mode != 0 && t->build_mode != 1)
+    return fail(SRE_ERR_INVALID, "build_mode must be 0 or 1");
+  if (t->xcd_remap != 0 && t->xcd_remap != 1)
+    return fail(SRE_ERR_INVALID, "xcd_remap must be 0 or 1");
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
+                      t->build_mode != e->tune.build_mode ||
                       t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
   e->tune = *t;
   if (layout) {
@@ -1154,14 +1197,8 @@ static sre_status time_region(sre_engine *e, int n_iter, double *ms,
 static sre_status body_compute(sre_engine *e, unsigned flags) {
   const bool stats = flags & SRE_STATS;
   if (stats) {
-    const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
-    if (need > e->wave_part_cap) {
-      HIP_TRY(hipStreamSynchronize(e->stream));
-      dfree(e->d_wave_part);
-      HIP_TRY(dalloc(&e->d_wave_part, need));
-      e->wave_part_cap = need;
-      e->graph_epoch++;
-    }
+    sre_status sw = ensure_wave_part(e);
+    if (sw != SRE_OK) return sw;
   }
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
@@ -1226,8 +1263,9 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
         *bar = 0;
         if (strcmp(line, sig) == 0) {
           sre_tuning t;
-          if (sscanf(bar + 1, "%d %d %d %f %f", &t.force_lanes, &t.build_lanes,
-                     &t.block, &t.cell_xy, &t.cell_z) == 5) {
+          if (sscanf(bar + 1, "%d %d %d %f %f %d %d", &t.force_lanes,
+                     &t.build_lanes, &t.block, &t.cell_xy, &t.cell_z,
+                     &t.build_mode, &t.xcd_remap) == 7) {
             fclose(f);
             sre_status s = sre_set_tuning(e, &t);
             if (s != SRE_OK) return s;
@@ -1241,16 +1279,18 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   // phase 1: binning + list build
   const float cells_xy[] = {0.35f, 0.5f, 0.7f, 1.0f};
   const float cells_z[] = {0.25f, 0.35f, 0.5f, 1.0f};
-  const int build_lanes[] = {4, 8, 16};
+  // (build_mode, build_lanes): per-atom search with 4/8/16 lanes, or tiles
+  const int builds[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 8}};
   sre_tuning best = e->tune;
   double best_ms = 1e30;
   for (float cxy : cells_xy)
     for (float cz : cells_z)
-      for (int bl : build_lanes) {
+      for (const auto &bd : builds) {
         sre_tuning t = e->tune;
         t.cell_xy = cxy;
         t.cell_z = cz;
-        t.build_lanes = bl;
+        t.build_mode = bd[0];
+        t.build_lanes = bd[1];
         sre_status s = sre_set_tuning(e, &t);
         if (s != SRE_OK) return s;
         s = do_rebuild(e);
@@ -1267,10 +1307,12 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   sre_tuning bestf = best;
   std::vector<int64_t> ref, cur;
   for (int fl : force_lanes)
-    for (int bk : blocks) {
+    for (int bk : blocks)
+      for (int xr = 0; xr < 2; ++xr) {
       sre_tuning t = best;
       t.force_lanes = fl;
       t.block = bk;
+      t.xcd_remap = xr;
       sre_status s = sre_set_tuning(e, &t);
       if (s != SRE_OK) return s;
       s = do_rebuild(e);
@@ -1298,9 +1340,9 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   if (cache_path) {
     FILE *f = fopen(cache_path, "a");
     if (f) {
-      fprintf(f, "%s|%d %d %d %.4f %.4f\n", sig, bestf.force_lanes,
+      fprintf(f, "%s|%d %d %d %.4f %.4f %d %d\n", sig, bestf.force_lanes,
               bestf.build_lanes, bestf.block, (double)bestf.cell_xy,
-              (double)bestf.cell_z);
+              (double)bestf.cell_z, bestf.build_mode, bestf.xcd_remap);
       fclose(f);
     }
   }
@@ -1314,16 +1356,16 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
   if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
   HIP_TRY(hipSetDevice(e->device));
   if (flags & SRE_STATS) {
-    // make sure the wave partial buffer exists before capture
-    const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
-    if (need > e->wave_part_cap) {
-      dfree(e->d_wave_part);
-      HIP_TRY(dalloc(&e->d_wave_part, need));
-      e->wave_part_cap = need;
-      e->graph_epoch++;
-    }
+    sre_status sw = ensure_wave_part(e);  // before capture
+    if (sw != SRE_OK) return sw;
   }
   refresh_tail_values(e);
+  if ((flags & SRE_PERTURB) && e->pert_anchored &&
+      (e->anchor_stride != e->pos_stride ||
+       e->anchor_count != (size_t)e->n_atoms * e->pos_stride))
+    return fail(SRE_ERR_INVALID,
+                "perturbation anchor does not match the bound positions "
+                "(call sre_set_perturbation again)");
   if (!e->graph_exec || e->graph_flags != flags ||
       e->graph_epoch_built != e->graph_epoch) {
     if (e->graph_exec) {
@@ -1334,9 +1376,11 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
     const bool pert = (flags & SRE_PERTURB) != 0;
     if (pert && !e->d_step_ctr)
       return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
+
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     if (pert)
-      launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), e->pos_stride,
+      launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos),
+                     e->pert_anchored ? e->d_anchor : nullptr, e->pos_stride,
                      e->pert_sigma, e->pert_seed, 0, e->d_step_ctr, e->stream);
     launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
                             e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
@@ -1387,15 +1431,33 @@ extern "C" sre_status sre_perturb_positions(sre_engine *e, float sigma,
                                             uint64_t seed, uint64_t step) {
   if (!e || !e->d_pos) return fail(SRE_ERR_INVALID, "no positions bound");
   HIP_TRY(hipSetDevice(e->device));
-  launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), e->pos_stride,
-                 sigma, seed, step, nullptr, e->stream);
+  launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), nullptr,
+                 e->pos_stride, sigma, seed, step, nullptr, e->stream);
   return SRE_OK;
 }
 
 extern "C" sre_status sre_set_perturbation(sre_engine *e, float sigma,
-                                           uint64_t seed, uint64_t first_step) {
-  if (!e || !(sigma >= 0.f)) return fail(SRE_ERR_INVALID, "bad arguments");
+                                           uint64_t seed, uint64_t first_step,
+                                           int anchored) {
+  if (!e || !(sigma >= 0.f) || (anchored != 0 && anchored != 1))
+    return fail(SRE_ERR_INVALID, "bad arguments");
+  if (anchored && !e->d_pos) return fail(SRE_ERR_INVALID, "no positions bound");
   HIP_TRY(hipSetDevice(e->device));
+  if (anchored) {
+    const size_t cnt = (size_t)e->n_atoms * e->pos_stride;
+    if (e->anchor_cap < cnt) {
+      dfree(e->d_anchor);
+      HIP_TRY(dalloc(&e->d_anchor, cnt));
+      e->anchor_cap = cnt;
+      e->graph_epoch++;
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_anchor, e->d_pos, sizeof(float) * cnt,
+                           hipMemcpyDeviceToDevice, e->stream));
+    e->anchor_count = cnt;
+    e->anchor_stride = e->pos_stride;
+  }
+  if (e->pert_anchored != (anchored != 0)) e->graph_epoch++;
+  e->pert_anchored = anchored != 0;
   if (!e->d_step_ctr) {
     HIP_TRY(dalloc(&e->d_step_ctr, 1));
     e->graph_epoch++;

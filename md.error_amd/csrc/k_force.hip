@@ -101,6 +101,16 @@ __device__ __forceinline__ void write_wave_stats(double *wp, int wave,
 
 // -------------------------------------------------------------- mixed ----
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Renumbering block b to (b % 8) * q + min(b % 8, r) + b / 8, with
+// q = G / 8 and r = G % 8, gives each XCD one contiguous range of blocks and
+// so one contiguous range of sorted atoms, whose neighbours then share that
+// XCD's L2.  The map is a bijection of [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int x = b & 7, q = G >> 3, r = G & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // Wave-wide maximum of an int (butterfly over the 64 lanes).
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
@@ -130,7 +140,8 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   constexpr int A = kWave / T;
   const int lane = threadIdx.x & (kWave - 1);
   const int l = lane % T;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + lane / T;
   const bool act = s < a.n;
   const int s0 = wave * A;
@@ -254,7 +265,8 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   constexpr int A = kWave / T;
   const int lane = threadIdx.x & (kWave - 1);
   const int l = lane % T;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + lane / T;
   const bool act = s < a.n;
   const int s0 = wave * A;

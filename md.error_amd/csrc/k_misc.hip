@@ -22,7 +22,10 @@ __device__ inline float u01(uint64_t x) {
   return ((float)(x >> 40) + 1.0f) * (1.0f / 16777216.0f);
 }
 
-__global__ __launch_bounds__(256) void k_perturb(int n, float *pos, int stride,
+// pos = (anchor ? anchor : pos) + sigma * N(0,1) per component; the normal
+// deviates come from Box-Muller on counter-based (seed, step, atom) hashes.
+__global__ __launch_bounds__(256) void k_perturb(int n, float *pos,
+                                                 const float *anchor, int stride,
                                                  float sigma, uint64_t seed,
                                                  uint64_t step,
                                                  const uint64_t *step_ctr) {
@@ -35,16 +38,19 @@ __global__ __launch_bounds__(256) void k_perturb(int n, float *pos, int stride,
   const float r1 = sqrtf(-2.f * logf(u1)), r2 = sqrtf(-2.f * logf(u3));
   const float t1 = 6.2831853f * u2, t2 = 6.2831853f * u4;
   float *p = pos + (size_t)i * stride;
-  p[0] += sigma * r1 * cosf(t1);
-  p[1] += sigma * r1 * sinf(t1);
-  p[2] += sigma * r2 * cosf(t2);
+  const float *o = anchor ? anchor + (size_t)i * stride : p;
+  const float x = o[0], y = o[1], z = o[2];
+  p[0] = x + sigma * r1 * cosf(t1);
+  p[1] = y + sigma * r1 * sinf(t1);
+  p[2] = z + sigma * r2 * cosf(t2);
 }
 
-void launch_perturb(int n, float *pos, int stride, float sigma, uint64_t seed,
-                    uint64_t step, const uint64_t *step_ctr, hipStream_t st) {
+void launch_perturb(int n, float *pos, const float *anchor, int stride,
+                    float sigma, uint64_t seed, uint64_t step,
+                    const uint64_t *step_ctr, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_perturb, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                     pos, stride, sigma, seed, step, step_ctr);
+                     pos, anchor, stride, sigma, seed, step, step_ctr);
 }
 
 }  // namespace sre

@@ -153,4 +153,168 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
 #undef SRE_BUILD
 }
 
+// ------------------------------------------------------------ tile build --
+//
+// One wave per tile: up to 64 consecutive cell-sorted atoms of one (x, y)
+// cell column.  The tile's candidate region is the union of its atoms'
+// spheres: the columns within (rl_max + margin) of the tile's column and, in
+// each, the z cells from the tile's lowest cell - kz to its highest + kz --
+// one or two contiguous runs of sorted atoms.  Runs are staged 64 atoms at a
+// time in LDS and every lane tests its own atom against each staged candidate
+// (a broadcast LDS read), so all loops are wave-uniform and the only
+// divergence is the list write of an accepted pair.  The predicate is the
+// same exact one as above; entries are written in candidate order.
+
+__device__ __forceinline__ int find_replica_by_col(const RepInfo *reps, int n_rep,
+                                                   int c) {
+  int lo = 0, hi = n_rep - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (reps[mid].col_off <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_col_tiles(int n_cols, int n_rep,
+                                                   const RepInfo *reps,
+                                                   const int *start,
+                                                   uint32_t *tiles_per_col) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cols) return;
+  const RepInfo &R = reps[find_replica_by_col(reps, n_rep, c)];
+  const int base = R.cell_off + (c - R.col_off) * R.nc[2];
+  const int cnt = start[base + R.nc[2]] - start[base];
+  tiles_per_col[c] = (uint32_t)((cnt + kWave - 1) / kWave);
+}
+
+__global__ __launch_bounds__(kWave) void k_build_tiles(
+    int n_cols, int n_rep, int TF, const uint32_t *__restrict__ tile_off,
+    const uint32_t *__restrict__ tiles_per_col, const uint4 *__restrict__ pos4,
+    const float *__restrict__ rcut, const RepInfo *__restrict__ reps,
+    const int *__restrict__ start, float skin, int cap, int *__restrict__ list,
+    int *__restrict__ nnb, unsigned long long *counters) {
+  __shared__ uint4 cand[kWave];
+  const int t = blockIdx.x;
+  const int n_tiles = (int)(tile_off[n_cols - 1] + tiles_per_col[n_cols - 1]);
+  if (t >= n_tiles) return;  // the grid is an upper bound of the tile count
+  // column of this tile: last c with tile_off[c] <= t
+  int lo = 0, hi = n_cols - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int)tile_off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int c = lo;
+  const RepInfo &R = reps[find_replica_by_col(reps, n_rep, c)];
+  const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
+  const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
+  const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  const int col = c - R.col_off;
+  const int cx = col / ncy, cy = col - cx * ncy;
+  const int cbase = R.cell_off + col * ncz;
+  const int a0 = start[cbase] + (t - (int)tile_off[c]) * kWave;
+  const int a1 = min(start[cbase + ncz], a0 + kWave);
+  const int lane = threadIdx.x;
+  const int s = a0 + lane;
+  const bool act = s < a1;
+  const uint4 pi = pos4[act ? s : a0];
+  const float rl = rcut[act ? s : a0] + skin;
+  const float rl2 = rl * rl;
+  float rlmax = act ? rl : 0.f;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    rlmax = fmaxf(rlmax, __shfl_xor(rlmax, o, kWave));
+  const float rlm = rlmax * 1.0001f + 1e-4f;
+  const float rlm2 = rlm * rlm;
+  const int cz_lo = cell_of(pos4[a0].z, ncz), cz_hi = cell_of(pos4[a1 - 1].z, ncz);
+  const int kx = (int)ceilf(rlm / hx), ky = (int)ceilf(rlm / hy),
+            kz = (int)ceilf(rlm / hz);
+  const bool fullx = 2 * kx + 1 >= ncx, fully = 2 * ky + 1 >= ncy;
+  const int ox0 = fullx ? -cx : -kx, ox1 = fullx ? ncx - 1 - cx : kx;
+  const int oy0 = fully ? -cy : -ky, oy1 = fully ? ncy - 1 - cy : ky;
+  // list slots of this lane's atom (layout of the force kernel, T = TF)
+  const int A = kWave / TF;
+  const int lt = __builtin_ctz((unsigned)TF);
+  const size_t sbase = act ? (size_t)(s / A) * (size_t)cap * A + (size_t)((s % A) * TF) : 0;
+  int count = 0;
+  for (int ox = ox0; ox <= ox1; ++ox) {
+    const float gx = fullx ? 0.f : fmaxf(0.f, (float)(abs(ox) - 1) * hx);
+    if (gx * gx >= rlm2) continue;
+    int cxw = cx + ox;
+    cxw += cxw < 0 ? ncx : 0;
+    cxw -= cxw >= ncx ? ncx : 0;
+    for (int oy = oy0; oy <= oy1; ++oy) {
+      const float gy = fully ? 0.f : fmaxf(0.f, (float)(abs(oy) - 1) * hy);
+      if (gx * gx + gy * gy >= rlm2) continue;
+      int cyw = cy + oy;
+      cyw += cyw < 0 ? ncy : 0;
+      cyw -= cyw >= ncy ? ncy : 0;
+      const int colbase = R.cell_off + (cxw * ncy + cyw) * ncz;
+      int ra0 = 0, rb0 = ncz - 1, ra1 = 0, rb1 = -1;
+      const int zl = cz_lo - kz, zh = cz_hi + kz;
+      if (zh - zl + 1 < ncz) {
+        if (zl < 0) {
+          ra0 = zl + ncz; rb0 = ncz - 1; ra1 = 0; rb1 = zh;
+        } else if (zh >= ncz) {
+          ra0 = zl; rb0 = ncz - 1; ra1 = 0; rb1 = zh - ncz;
+        } else {
+          ra0 = zl; rb0 = zh;
+        }
+      }
+      for (int run = 0; run < 2; ++run) {
+        const int ca = run ? ra1 : ra0, cb = run ? rb1 : rb0;
+        if (cb < ca) continue;
+        const int p0 = start[colbase + ca], p1 = start[colbase + cb + 1];
+        for (int p = p0; p < p1; p += kWave) {
+          const int q = p + lane;
+          cand[lane] = pos4[q < p1 ? q : p0];
+          __syncthreads();
+          const int m = min(kWave, p1 - p);
+          for (int k = 0; k < m; ++k) {
+            const uint4 cj = cand[k];
+            const float d2 = dist2(disp(cj.x, pi.x, qx), disp(cj.y, pi.y, qy),
+                                   disp(cj.z, pi.z, qz));
+            const int j = p + k;
+            if (act && j != s && d2 < rl2) {
+              if (count < cap)
+                list[sbase + ((size_t)(count >> lt) << 6) + (count & (TF - 1))] = j;
+              ++count;
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (act) nnb[s] = count;
+  int mx = count, tot = count;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mx = max(mx, __shfl_xor(mx, o, kWave));
+    tot += __shfl_xor(tot, o, kWave);
+  }
+  if (lane == 0 && mx > 0) {
+    atomicMax(&counters[0], (unsigned long long)mx);
+    atomicAdd(&counters[1], (unsigned long long)tot);
+  }
+}
+
+void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
+                        const uint4 *pos4, const float *rcut,
+                        const RepInfo *reps, const int *start, float skin,
+                        int cap, int *list, int *nnb,
+                        unsigned long long *counters, uint32_t *tiles_per_col,
+                        uint32_t *tile_off, uint32_t *scan_tmp, hipStream_t st) {
+  if (n <= 0 || n_cols <= 0) return;
+  hipLaunchKernelGGL(k_col_tiles, dim3((n_cols + 255) / 256), dim3(256), 0, st,
+                     n_cols, n_rep, reps, start, tiles_per_col);
+  device_exscan_u32((size_t)n_cols, tiles_per_col, tile_off, scan_tmp, st);
+  // every column adds at most one partial tile
+  const int max_tiles = (n + kWave - 1) / kWave + n_cols;
+  hipLaunchKernelGGL(k_build_tiles, dim3(max_tiles), dim3(kWave), 0, st, n_cols,
+                     n_rep, force_lanes, tile_off, tiles_per_col, pos4, rcut,
+                     reps, start, skin, cap, list, nnb, counters);
+}
+
 }  // namespace sre

@@ -213,8 +213,8 @@ __global__ __launch_bounds__(kSortT) void k_scan_down(size_t m,
   }
 }
 
-static void device_exscan(size_t m, const uint32_t *in, uint32_t *out,
-                          uint32_t *tmp, hipStream_t st) {
+void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
+                       uint32_t *tmp, hipStream_t st) {
   if (m <= 65536) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSortT), 0, st, m, in,
                        out);
@@ -262,7 +262,7 @@ int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
     hipLaunchKernelGGL(k_sort_block, dim3(nblocks), dim3(kSortT), 0, st, n,
                        shift, bits[p], nblocks, kin, vin, b.tkey, b.tval,
                        b.counts);
-    device_exscan(m, b.counts, b.offsets, b.scan_tmp, st);
+    device_exscan_u32(m, b.counts, b.offsets, b.scan_tmp, st);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortT), 0, st, n,
                        shift, bits[p], nblocks, b.tkey, b.tval, b.counts,
                        b.offsets, kout, vout);

@@ -45,6 +45,7 @@ struct RepInfo {
   int nc[3];        // cell grid
   float h[3];       // cell edges L / nc
   int cell_off;     // first global cell id of this replica
+  int col_off;      // first global cell column (x, y) of this replica
   int atom_off;     // first atom (caller order and sorted order)
   int n_atoms;
   int rc_off;       // offset of the rc lattice in the concatenated table
@@ -155,6 +156,16 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
                        float skin, int cap, int *list, int *nnb,
                        unsigned long long *counters, hipStream_t st);
+// Tile build: tiles of <= 64 atoms inside one cell column.  tiles_per_col and
+// tile_off (n_cols each) are scratch arrays filled on the device.
+void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
+                        const uint4 *pos4, const float *rcut,
+                        const RepInfo *reps, const int *start, float skin,
+                        int cap, int *list, int *nnb,
+                        unsigned long long *counters, uint32_t *tiles_per_col,
+                        uint32_t *tile_off, uint32_t *scan_tmp, hipStream_t st);
+void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
+                       uint32_t *tmp, hipStream_t st);
 // step_ctr (optional) is advanced by one thread after the positions are read
 void launch_update_positions(int n, const uint32_t *perm, const float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
@@ -182,6 +193,7 @@ struct ForceArgs {
   double *wave_part;           // per wave: 2 slots x 5 (E, Wxx, Wyy, Wzz, n_in)
   double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
   unsigned *err_flag;
+  int xcd_remap;               // 1: contiguous block ranges per XCD
 };
 void launch_force(int precision, int lanes, int block, bool stats, bool tail,
                   const ForceArgs &a, hipStream_t st);
@@ -241,7 +253,8 @@ void launch_adjoint_tail(int n_bins, const int *entry_rep, const RepInfo *reps,
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_i64(int64_t *p, int64_t v, size_t n, hipStream_t st);
 // step_ctr != NULL: the step index is read from device memory
-void launch_perturb(int n, float *pos, int stride, float sigma, uint64_t seed,
-                    uint64_t step, const uint64_t *step_ctr, hipStream_t st);
+void launch_perturb(int n, float *pos, const float *anchor, int stride,
+                    float sigma, uint64_t seed, uint64_t step,
+                    const uint64_t *step_ctr, hipStream_t st);
 
 }  // namespace sre

@@ -37,7 +37,8 @@ class Options(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("force_lanes", C.c_int), ("build_lanes", C.c_int), ("block", C.c_int),
-                ("cell_xy", C.c_float), ("cell_z", C.c_float)]
+                ("cell_xy", C.c_float), ("cell_z", C.c_float), ("build_mode", C.c_int),
+                ("xcd_remap", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,7 +79,7 @@ SIGNATURES = {
     "sre_set_tuning": (_st, [_vp, C.POINTER(Tuning)]),
     "sre_autotune": (_st, [_vp, C.c_uint, C.c_char_p]),
     "sre_graph_run": (_st, [_vp, C.c_uint, C.c_int]),
-    "sre_set_perturbation": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64]),
+    "sre_set_perturbation": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64, C.c_int]),
     "sre_check_skin_flag": (_st, [_vp]),
     "sre_last_rebuild_displacement": (_st, [_vp, _f32p]),
     "sre_perturb_positions": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64]),

@@ -283,8 +283,12 @@ class Engine:
         f = _flags(stats, tail) | (_lib.PERTURB if perturb else 0)
         check(lib().sre_graph_run(self._h, f, int(n_steps)))
 
-    def set_perturbation(self, sigma, seed, first_step=0):
-        check(lib().sre_set_perturbation(self._h, float(sigma), int(seed), int(first_step)))
+    def set_perturbation(self, sigma, seed, first_step=0, anchored=False):
+        """Perturbation applied by graph_run(perturb=True): a random walk
+        (x += sigma N(0,1) per replay) or, anchored, vibration about the
+        positions bound now (x = x0 + sigma N(0,1))."""
+        check(lib().sre_set_perturbation(self._h, float(sigma), int(seed), int(first_step),
+                                         1 if anchored else 0))
 
     def check_skin_flag(self):
         check(lib().sre_check_skin_flag(self._h))

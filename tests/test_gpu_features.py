@@ -139,7 +139,7 @@ def test_graph_replay_equals_eager_and_skin_flag(torch_cuda, slab):
     torch.cuda.synchronize()
     e.bind_positions(pos)
     e.rebuild()
-    e.set_perturbation(0.01, 123, 0)
+    e.set_perturbation(0.01, 123, 0, anchored=False)
     results = []
     for rep in range(3):
         e.graph_run(1, stats=True, tail=True, perturb=True)
@@ -175,6 +175,40 @@ def test_perturbation_is_reproducible(torch_cuda, slab):
     assert np.array_equal(outs[0], outs[1])
     d = outs[0][:, :3] - slab.xyz[:, :3]
     assert abs(d.std() / np.sqrt(5) - 0.01) < 5e-4
+
+
+def test_anchored_perturbation_stays_bounded(torch_cuda, slab):
+    """Anchored vibration: x = x0 + sigma N(0,1) at every replay, so 300 steps
+    neither trip the skin test nor bring atoms closer than the set-up allows."""
+    import torch
+    e = Engine(skin=slab.skin)
+    slab.setup(e, tail=True)
+    pos = torch.from_numpy(slab.xyz.copy()).cuda()
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    e.set_perturbation(0.01, 5, 0, anchored=True)
+    e.graph_run(1, perturb=True)
+    e.synchronize()
+    first = pos.cpu().numpy()
+    for _ in range(50):
+        e.graph_run(6, stats=True, tail=True, perturb=True)
+        e.rebuild()
+        assert e.last_rebuild_displacement() <= 0.5 * slab.skin
+    e.check_skin_flag()
+    e.compute(stats=True, tail=True)
+    e.stats()                      # raises on a force-range overflow
+    last = pos.cpu().numpy()
+    d = last[:, :3] - slab.xyz[:, :3]
+    assert abs(d.std() - 0.01) < 5e-4         # one draw from x0, not a walk
+    assert not np.array_equal(first, last)
+    # the same step counter gives the same positions
+    pos.copy_(torch.from_numpy(slab.xyz.copy()))
+    torch.cuda.synchronize()
+    e.set_perturbation(0.01, 5, 0, anchored=True)
+    e.graph_run(1, perturb=True)
+    e.synchronize()
+    np.testing.assert_array_equal(pos.cpu().numpy(), first)
 
 
 def test_add_forces_f32_soa_accumulates(torch_cuda, slab):

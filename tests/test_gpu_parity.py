@@ -64,10 +64,12 @@ def slab64():
     return configs.slab()
 
 
+@pytest.mark.parametrize("build_mode", [0, 1])
 @pytest.mark.parametrize("wname", ["fcc", "slab16", "slab64"])
-def test_pairs_and_rc_bit_exact(torch_cuda, wname, request):
+def test_pairs_and_rc_bit_exact(torch_cuda, wname, build_mode, request):
+    """Both list builders (per-atom search, 64-atom tiles) give the oracle's pairs."""
     w = request.getfixturevalue(wname)
-    e = engine_for(w)
+    e = engine_for(w, build_mode=build_mode)
     ref = oracle_for(w)
     assert np.array_equal(e.rcut(), ref["rc"])
     off, j = e.export_pairs()
@@ -112,13 +114,14 @@ def test_fp64_form_matches_oracle(torch_cuda, wname, request):
 
 def test_forces_bitwise_independent_of_tuning(torch_cuda, slab16):
     base = None
-    for fl, bl, cxy, cz, blk in [(4, 8, 0.5, 0.5, 256), (1, 4, 0.35, 0.25, 64),
-                                 (2, 16, 0.7, 1.0, 128), (8, 2, 1.0, 0.35, 128),
-                                 (16, 1, 0.5, 0.5, 256)]:
+    cases = [(4, 8, 0.5, 0.5, 256, 0, 0), (1, 4, 0.35, 0.25, 64, 0, 1),
+             (2, 16, 0.7, 1.0, 128, 1, 0), (8, 2, 1.0, 0.35, 128, 1, 1),
+             (16, 1, 0.5, 0.5, 256, 1, 1), (4, 8, 1.0, 0.35, 64, 1, 1)]
+    for fl, bl, cxy, cz, blk, bm, xr in cases:
         e = engine_for(slab16, force_lanes=fl, build_lanes=bl, cell_xy=cxy, cell_z=cz,
-                       block=blk)
+                       block=blk, build_mode=bm, xcd_remap=xr)
         f = e.forces_fixed()
         if base is None:
             base = f
         else:
-            assert np.array_equal(f, base), (fl, bl, cxy, cz, blk)
+            assert np.array_equal(f, base), (fl, bl, cxy, cz, blk, bm, xr)
