@@ -4,11 +4,15 @@
 One step = thermal vibration of every atom about its site in the initial
 configuration (x = x0 + 0.01 sigma N(0,1) per component, seeded, drawn afresh
 each step, so no two atoms ever come closer than the set-up allows) ->
-position refresh + per-wavefront skin test -> LJ force, energy and virial
-with the per-atom cutoff rc(z) and the per-atom tail correction.  The neighbor list is rebuilt every --rebuild-every steps; the
-skin flag recorded on the device is checked before every rebuild, so a run in
-which any atom moved more than skin/2 between rebuilds fails instead of
-reporting.  Steps between rebuilds are replayed as one hipGraph.
+position refresh (the perturbation is applied inside it) + per-wavefront
+skin test -> LJ force, energy and virial with the per-atom cutoff rc(z) and
+the per-atom tail correction.  The neighbor list is rebuilt every
+--rebuild-every steps (10, the rebuild period of the large-slab workload);
+the displacement recorded on the device is checked at every rebuild, so a run
+in which any atom moved more than skin/2 between rebuilds fails instead of
+reporting.  The energy and virial are reduced and read back every 100 steps,
+like the reference driver's thermo output.  Steps between rebuilds are
+replayed as one hipGraph.
 
 Prints one JSON line:  value = pair evaluations per second (list entries
 processed by the force kernel per second of whole-job wall time, rebuilds
@@ -30,6 +34,8 @@ METRIC = "pair evals/s + ms/step, 64k LJ slab: mixed-prec vs FP64, max force err
 HBM_PEAK_GBS = 8000.0
 PERTURB_SIGMA = 0.01
 SEED = 7
+THERMO_EVERY = 100   # energy/virial read back like the reference's thermo output
+                     # (H/app/lj.nhc-xtc.cu:198-224)
 
 
 def log(*a):
@@ -58,6 +64,8 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
             e.graph_run(n, stats=True, tail=True, perturb=True)
             pairs += nl * n
             done += n
+            if done // THERMO_EVERY > (done - n) // THERMO_EVERY:
+                e.stats()                     # reduce + read back; raises on a force overflow
             e.rebuild()                       # one synchronisation per rebuild
             rebuilds += 1
             # the skin test recorded during the segment: every atom must have
@@ -150,7 +158,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=60)
-    ap.add_argument("--rebuild-every", type=int, default=6)
+    ap.add_argument("--rebuild-every", type=int, default=10,
+                    help="steps per neighbor-list rebuild (the large-slab workload's period)")
     ap.add_argument("--cpu-seconds", type=float, default=5.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")

@@ -148,12 +148,19 @@ struct sre_engine {
   uint64_t *d_step_ctr = nullptr;  // perturbation step counter (graph mode)
   float pert_sigma = 0.f;
   uint64_t pert_seed = 0;
+  float *d_skin_part = nullptr;    // per update wavefront: max displacement^2
+  int2 *d_build_part = nullptr;    // per build wavefront: (longest, total)
+  int build_part_cap = 0;
   float *d_anchor = nullptr;       // anchored perturbation: positions at set-up
   size_t anchor_cap = 0, anchor_count = 0;
   int anchor_stride = 0;
   bool pert_anchored = false;
   bool stats_valid = false;
   bool atom_stats_valid = false;
+  // the force kernel leaves per-wave partial sums; their fixed-order
+  // reduction runs when the statistics are read
+  bool stats_pending = false;
+  int stats_lanes = 0;
   bool forces_valid = false;
 
   bool built = false;
@@ -180,7 +187,7 @@ sre_engine::~sre_engine() {
   if (h_info) (void)hipHostFree(h_info);
   dfree(d_fyd); dfree(d_fzd); dfree(d_wave_part); dfree(d_stats);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
-  dfree(d_anchor);
+  dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -305,6 +312,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
   dfree(e->d_fxd); dfree(e->d_fyd); dfree(e->d_fzd); dfree(e->d_atom_stats);
   dfree(e->d_stats); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
+  dfree(e->d_skin_part);
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
   HIP_TRY(dalloc(&e->d_rep_of, N));
@@ -334,6 +342,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->sb.val_b, N));
   HIP_TRY(dalloc(&e->sb.tkey, N));
   HIP_TRY(dalloc(&e->sb.tval, N));
+  HIP_TRY(dalloc(&e->d_skin_part, skin_part_count(N)));
+  launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_part_count(N), e->stream);
   HIP_TRY(hipMemcpy(e->d_rep_of, rep_of.data(), sizeof(int) * N,
                     hipMemcpyHostToDevice));
   // defaults: every atom type 0, uniform rc lattice of 1 bin (set later)
@@ -734,22 +744,33 @@ static sre_status do_rebuild(sre_engine *e) {
     }
     cap = round_up((int)guess, e->tune.force_lanes);
   }
+  const int n_parts = build_part_count(e->tune.build_mode, N, e->tune.build_lanes,
+                                       e->n_cols);
+  if (e->build_part_cap < n_parts) {
+    dfree(e->d_build_part);
+    HIP_TRY(dalloc(&e->d_build_part, n_parts));
+    e->build_part_cap = n_parts;
+  }
   for (int attempt = 0; attempt < 4; ++attempt) {
     s = ensure_list(e, cap);
     if (s != SRE_OK) return s;
-    launch_fill_i64((int64_t *)e->d_counters, 0, 2, st);
     if (e->tune.build_mode == 1)
       launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
                          e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                         e->d_list, e->d_nnb, e->d_counters, e->d_tiles_per_col,
-                         e->d_tile_off, e->d_tile_tmp, st);
+                         e->d_list, e->d_nnb, e->d_build_part,
+                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp, st);
     else
       launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
                         e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                        e->d_list, e->d_nnb, e->d_counters, st);
+                        e->d_list, e->d_nnb, e->d_build_part, st);
+    // longest list and total length; on the first attempt also the largest
+    // displacement recorded since the previous rebuild (its partials are
+    // cleared for the next segment)
+    launch_rebuild_info(n_parts, e->d_build_part, e->d_counters,
+                        attempt == 0 ? skin_part_count(N) : 0, e->d_skin_part,
+                        1, e->d_flags, st);
     HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
-    // one pinned readback per rebuild: longest list, total length and the
-    // displacement recorded since the previous rebuild
+    // one pinned readback per rebuild
     HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 2 * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(e->h_info + 2, e->d_flags, sizeof(uint64_t),
@@ -768,7 +789,6 @@ static sre_status do_rebuild(sre_engine *e) {
     const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
     memcpy(&e->last_disp2, &bits, sizeof(float));
   }
-  launch_fill_u32(e->d_flags, 0u, 1, st);
   s = ensure_wave_part(e);
   if (s != SRE_OK) return s;
   e->built = true;
@@ -785,12 +805,14 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   if (!e) return fail(SRE_ERR_INVALID, "null engine");
   if (!e->built) return fail(SRE_ERR_NOT_READY, "update before the first rebuild");
   HIP_TRY(hipSetDevice(e->device));
-  launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
-                          e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
-                          e->d_flags, nullptr, e->stream);
+  launch_update_positions(e->n_atoms, e->sb.val_a, const_cast<float *>(e->d_pos),
+                          e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
+                          e->d_res4, e->d_skin_part, nullptr, e->stream);
   HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
+    launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+                        e->d_skin_part, 0, e->d_flags, e->stream);
     unsigned bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
                            hipMemcpyDeviceToHost, e->stream));
@@ -826,11 +848,15 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.wave_part = e->d_wave_part;
   a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
   a.err_flag = e->d_flags + 1;
+  a.step_ctr = nullptr;
   a.xcd_remap = e->tune.xcd_remap;
   return a;
 }
 
-static sre_status launch_compute(sre_engine *e, unsigned flags) {
+// step_ctr: perturbation step counter the force kernel advances (graph
+// replays with a perturbation), else NULL
+static sre_status launch_compute(sre_engine *e, unsigned flags,
+                                 uint64_t *step_ctr = nullptr) {
   const bool stats = flags & SRE_STATS;
   const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
   const int T = e->list_lanes;
@@ -840,11 +866,25 @@ static sre_status launch_compute(sre_engine *e, unsigned flags) {
   }
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
+  a.step_ctr = step_ctr;
   launch_force(e->precision, T, e->tune.block, stats, tail, a, e->stream);
-  if (stats)
-    launch_stats_reduce(e->n_rep, e->d_reps, T, e->d_wave_part, e->d_stats,
-                        e->stream);
+  if (stats) {
+    e->stats_pending = true;
+    e->stats_lanes = T;
+  }
   HIP_TRY(hipGetLastError());  // launch-configuration errors
+  return SRE_OK;
+}
+
+// Reduces the per-wave partial sums of the last SRE_STATS compute into
+// d_stats (stream-ordered; fixed order, so the result is reproducible).
+static sre_status reduce_stats(sre_engine *e) {
+  if (e->stats_pending) {
+    launch_stats_reduce(e->n_rep, e->d_reps, e->stats_lanes, e->d_wave_part,
+                        e->d_stats, e->stream);
+    HIP_TRY(hipGetLastError());
+    e->stats_pending = false;
+  }
   return SRE_OK;
 }
 
@@ -939,6 +979,8 @@ extern "C" sre_status sre_get_stats(sre_engine *e, double *out) {
   if (!e || !out) return fail(SRE_ERR_INVALID, "sre_get_stats: null argument");
   if (!e->stats_valid) return fail(SRE_ERR_NOT_READY, "no SRE_STATS compute yet");
   HIP_TRY(hipSetDevice(e->device));
+  sre_status rs = reduce_stats(e);
+  if (rs != SRE_OK) return rs;
   std::vector<double> st(5 * (size_t)e->n_rep);
   HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
                          hipMemcpyDeviceToHost, e->stream));
@@ -981,6 +1023,8 @@ extern "C" sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list,
   if (n_in_rc) {
     *n_in_rc = -1;
     if (e->stats_valid) {
+      sre_status rs = reduce_stats(e);
+      if (rs != SRE_OK) return rs;
       std::vector<double> st(5 * (size_t)e->n_rep);
       HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
                              hipMemcpyDeviceToHost, e->stream));
@@ -1193,19 +1237,9 @@ static sre_status time_region(sre_engine *e, int n_iter, double *ms,
   return s;
 }
 
-// the force kernel alone (no statistic reduction), for per-launch timing
+// one force launch (the statistic reduction is deferred to the read)
 static sre_status body_compute(sre_engine *e, unsigned flags) {
-  const bool stats = flags & SRE_STATS;
-  if (stats) {
-    sre_status sw = ensure_wave_part(e);
-    if (sw != SRE_OK) return sw;
-  }
-  refresh_tail_values(e);
-  ForceArgs a = force_args(e, stats);
-  launch_force(e->precision, e->list_lanes, e->tune.block, stats,
-               (flags & SRE_TAIL) && e->tail_bins > 0, a, e->stream);
-  HIP_TRY(hipGetLastError());
-  return SRE_OK;
+  return launch_compute(e, flags & (SRE_STATS | SRE_TAIL));
 }
 
 extern "C" sre_status sre_time_compute(sre_engine *e, unsigned flags,
@@ -1216,8 +1250,7 @@ extern "C" sre_status sre_time_compute(sre_engine *e, unsigned flags,
   sre_status s = time_region(e, n_iter, ms_per_iter, body_compute, flags);
   if (s == SRE_OK) {
     e->forces_valid = true;
-    e->stats_valid = false;  // the statistic reduction was not run
-    e->atom_stats_valid = (flags & SRE_STATS) != 0;
+    e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
   }
   return s;
 }
@@ -1378,15 +1411,14 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
       return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
 
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    if (pert)
-      launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos),
-                     e->pert_anchored ? e->d_anchor : nullptr, e->pos_stride,
-                     e->pert_sigma, e->pert_seed, 0, e->d_step_ctr, e->stream);
-    launch_update_positions(e->n_atoms, e->sb.val_a, e->d_pos, e->pos_stride,
+    PerturbArgs pa{e->pert_sigma, e->pert_seed,
+                   e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
+    launch_update_positions(e->n_atoms, e->sb.val_a,
+                            const_cast<float *>(e->d_pos), e->pos_stride,
                             e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
-                            e->d_flags, pert ? e->d_step_ctr : nullptr,
-                            e->stream);
-    sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL));
+                            e->d_skin_part, pert ? &pa : nullptr, e->stream);
+    sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL),
+                                  pert ? e->d_step_ctr : nullptr);
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (s != SRE_OK) return s;
     if (r != hipSuccess)
@@ -1399,8 +1431,12 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
     e->graph_epoch_built = e->graph_epoch;
   }
   for (int i = 0; i < n_steps; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
-  e->forces_valid = true;
-  e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
+  if (n_steps > 0) {
+    e->forces_valid = true;
+    e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
+    e->stats_pending = e->stats_valid;
+    e->stats_lanes = e->list_lanes;
+  }
   return SRE_OK;
 }
 
@@ -1414,6 +1450,9 @@ extern "C" sre_status sre_last_rebuild_displacement(sre_engine *e,
 extern "C" sre_status sre_check_skin_flag(sre_engine *e) {
   if (!e) return fail(SRE_ERR_INVALID, "null engine");
   HIP_TRY(hipSetDevice(e->device));
+  if (!e->d_skin_part) return fail(SRE_ERR_INVALID, "system not set");
+  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+                      e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
                          hipMemcpyDeviceToHost, e->stream));
@@ -1431,8 +1470,8 @@ extern "C" sre_status sre_perturb_positions(sre_engine *e, float sigma,
                                             uint64_t seed, uint64_t step) {
   if (!e || !e->d_pos) return fail(SRE_ERR_INVALID, "no positions bound");
   HIP_TRY(hipSetDevice(e->device));
-  launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), nullptr,
-                 e->pos_stride, sigma, seed, step, nullptr, e->stream);
+  launch_perturb(e->n_atoms, const_cast<float *>(e->d_pos), e->pos_stride,
+                 sigma, seed, step, e->stream);
   return SRE_OK;
 }
 

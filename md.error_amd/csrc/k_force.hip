@@ -18,46 +18,125 @@ namespace sre {
 
 // ----------------------------------------------------- update + skin test --
 
+// Sorted slot s refreshes atom perm[s]: with a perturbation the new position
+// is written back to the bound array first (each atom is visited exactly
+// once).  The skin test keeps, per wavefront, the largest squared lattice
+// displacement from the rebuild-time position seen since the last rebuild;
+// each wave owns its slot of skin_part, so no atomics are involved.
 __global__ __launch_bounds__(256) void k_update_positions(
-    int n, const uint32_t *__restrict__ perm, const float *__restrict__ pos,
+    int n, const uint32_t *__restrict__ perm, float *__restrict__ pos,
     int stride, const RepInfo *__restrict__ reps, uint4 *__restrict__ pos4,
     const uint4 *__restrict__ backup, float4 *__restrict__ res4,
-    unsigned *skin_flag, uint64_t *step_ctr) {
+    float *__restrict__ skin_part, PerturbArgs pert) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (step_ctr && s == 0) *step_ctr += 1;  // the perturbation ran before us
   float d2 = 0.f;
   if (s < n) {
     const uint4 b = backup[s];
+    const uint32_t i = perm[s];
+    float *p = pos + (size_t)i * stride;
+    float x, y, z;
+    if (pert.step_ctr) {
+      const float3 g = gauss3(pert.seed, *pert.step_ctr, (uint64_t)i);
+      const float *o = pert.anchor ? pert.anchor + (size_t)i * stride : p;
+      x = o[0] + pert.sigma * g.x;
+      y = o[1] + pert.sigma * g.y;
+      z = o[2] + pert.sigma * g.z;
+      p[0] = x;
+      p[1] = y;
+      p[2] = z;
+    } else {
+      x = p[0];
+      y = p[1];
+      z = p[2];
+    }
     const RepInfo &R = reps[b.w >> 8];
-    const float *p = pos + (size_t)perm[s] * stride;
-    const uint32_t ux = quantize(p[0], R.invq[0]);
-    const uint32_t uy = quantize(p[1], R.invq[1]);
-    const uint32_t uz = quantize(p[2], R.invq[2]);
+    const uint32_t ux = quantize(x, R.invq[0]);
+    const uint32_t uy = quantize(y, R.invq[1]);
+    const uint32_t uz = quantize(z, R.invq[2]);
     pos4[s] = make_uint4(ux, uy, uz, b.w);
     if (res4)
-      res4[s] = make_float4(quant_residual(p[0], R.invq[0], R.qd[0]),
-                            quant_residual(p[1], R.invq[1], R.qd[1]),
-                            quant_residual(p[2], R.invq[2], R.qd[2]), 0.f);
+      res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
+                            quant_residual(y, R.invq[1], R.qd[1]),
+                            quant_residual(z, R.invq[2], R.qd[2]), 0.f);
     d2 = dist2(disp(ux, b.x, R.q[0]), disp(uy, b.y, R.q[1]),
                disp(uz, b.z, R.q[2]));
   }
-  // per-wavefront maximum, one atomic per wave (max is order independent)
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1)
     d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0 && d2 > 0.f)
-    atomicMax(skin_flag, __float_as_uint(d2));
+  const int w = s / kWave;  // slots exist for the waves holding atoms
+  if ((threadIdx.x & (kWave - 1)) == 0 && w * kWave < n && d2 > skin_part[w])
+    skin_part[w] = d2;
 }
 
-void launch_update_positions(int n, const uint32_t *perm, const float *pos,
+void launch_update_positions(int n, const uint32_t *perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
-                             unsigned *skin_flag, uint64_t *step_ctr,
+                             float *skin_part, const PerturbArgs *pert,
                              hipStream_t st) {
   if (n <= 0) return;
+  PerturbArgs p{};
+  if (pert) p = *pert;
   hipLaunchKernelGGL(k_update_positions, dim3((n + 255) / 256), dim3(256), 0,
                      st, n, perm, pos, stride, reps, pos4, backup, res4,
-                     skin_flag, step_ctr);
+                     skin_part, p);
+}
+
+// ------------------------------------------------------- rebuild info --
+
+__global__ __launch_bounds__(1024) void k_rebuild_info(
+    int n_build, const int2 *__restrict__ build_part,
+    unsigned long long *counters, int n_skin, float *__restrict__ skin_part,
+    int clear_skin, unsigned *skin_bits) {
+  __shared__ unsigned long long s_tot[1024 / kWave];
+  __shared__ int s_max[1024 / kWave];
+  __shared__ float s_d2[1024 / kWave];
+  int mx = 0;
+  unsigned long long tot = 0;
+  float d2 = 0.f;
+  for (int k = threadIdx.x; k < n_build; k += blockDim.x) {
+    const int2 v = build_part[k];
+    mx = max(mx, v.x);
+    tot += (unsigned long long)(unsigned)v.y;
+  }
+  for (int k = threadIdx.x; k < n_skin; k += blockDim.x) {
+    d2 = fmaxf(d2, skin_part[k]);
+    if (clear_skin) skin_part[k] = 0.f;
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mx = max(mx, __shfl_xor(mx, o, kWave));
+    tot += __shfl_xor(tot, o, kWave);
+    d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
+  }
+  const int wv = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_max[wv] = mx;
+    s_tot[wv] = tot;
+    s_d2[wv] = d2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / kWave); ++k) {
+      mx = max(mx, s_max[k]);
+      tot += s_tot[k];
+      d2 = fmaxf(d2, s_d2[k]);
+    }
+    if (n_build > 0) {
+      counters[0] = (unsigned long long)mx;
+      counters[1] = tot;
+    }
+    if (n_skin > 0) *skin_bits = __float_as_uint(d2);
+  }
+}
+
+void launch_rebuild_info(int n_build, const int2 *build_part,
+                         unsigned long long *counters, int n_skin,
+                         float *skin_part, int clear_skin, unsigned *skin_bits,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_rebuild_info, dim3(1), dim3(1024), 0, st, n_build,
+                     build_part, counters, n_skin, skin_part, clear_skin,
+                     skin_bits);
 }
 
 // ------------------------------------------------------ wave reductions --
@@ -81,13 +160,16 @@ __device__ __forceinline__ void write_wave_stats(double *wp, int wave,
   double out0[5], out1[5];
   const bool in0 = own && rep == rep0, in1 = own && rep != rep0;
   int r1 = in1 ? rep : -1;
+  const bool any1 = __ballot(in1) != 0;  // wave-uniform: a replica starts here
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
     out0[c] = wave_sum(in0 ? vals[c] : 0.0);
-    out1[c] = wave_sum(in1 ? vals[c] : 0.0);
+    out1[c] = any1 ? wave_sum(in1 ? vals[c] : 0.0) : 0.0;
   }
+  if (any1) {
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) r1 = max(r1, __shfl_xor(r1, o, kWave));
+    for (int o = kWave / 2; o > 0; o >>= 1) r1 = max(r1, __shfl_xor(r1, o, kWave));
+  }
   if ((threadIdx.x & (kWave - 1)) == 0) {
     double *d = wp + (size_t)wave * 12;
     d[0] = (double)rep0;
@@ -131,6 +213,8 @@ __device__ __forceinline__ int wave_max_int(int v) {
 template <int T, bool STATS, bool TAIL, bool MULTI>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   constexpr int U = kForceUnroll;
+  // the position refresh of this step has read the counter (stream order)
+  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
   __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -256,6 +340,7 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
 
 template <int T, bool STATS, bool TAIL, bool MULTI>
 __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
+  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
   __shared__ PairParamD sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -411,39 +496,46 @@ void launch_force(int precision, int lanes, int block, bool stats, bool tail,
 
 // ------------------------------------------------------ statistic reduce --
 
-// One block per replica sums the wave partials in a fixed order.
-__global__ __launch_bounds__(256) void k_stats_reduce(const RepInfo *reps,
-                                                      int A,
-                                                      const double *wave_part,
-                                                      double *stats) {
-  __shared__ double red[256][5];
+// One block per replica sums the wave partials in a fixed order (thread k
+// takes rows k, k + 1024, ...; then a fixed shuffle and LDS tree).  A row is
+// 12 doubles: (replica, E, Wxx, Wyy, Wzz, n_in) for each of two slots.
+__global__ __launch_bounds__(1024) void k_stats_reduce(const RepInfo *reps,
+                                                       int A,
+                                                       const double *wave_part,
+                                                       double *stats) {
+  __shared__ double red[1024 / kWave][5];
   const int r = blockIdx.x;
   const RepInfo &R = reps[r];
   const int w_lo = R.atom_off / A;
   const int w_hi = (R.atom_off + R.n_atoms - 1) / A;
   double acc[5] = {0, 0, 0, 0, 0};
+  const double rr = (double)r;
   for (int w = w_lo + (int)threadIdx.x; w <= w_hi; w += blockDim.x) {
-    const double *d = wave_part + (size_t)w * 12;
-    if ((int)d[0] == r)
-      for (int c = 0; c < 5; ++c) acc[c] += d[1 + c];
-    if ((int)d[6] == r)
-      for (int c = 0; c < 5; ++c) acc[c] += d[7 + c];
+    const double2 *d = reinterpret_cast<const double2 *>(wave_part + (size_t)w * 12);
+    const double2 v0 = d[0], v1 = d[1], v2 = d[2], v3 = d[3], v4 = d[4], v5 = d[5];
+    if (v0.x == rr) {
+      acc[0] += v0.y; acc[1] += v1.x; acc[2] += v1.y; acc[3] += v2.x; acc[4] += v2.y;
+    }
+    if (v3.x == rr) {
+      acc[0] += v3.y; acc[1] += v4.x; acc[2] += v4.y; acc[3] += v5.x; acc[4] += v5.y;
+    }
   }
-  for (int c = 0; c < 5; ++c) red[threadIdx.x][c] = acc[c];
+  for (int c = 0; c < 5; ++c) acc[c] = wave_sum(acc[c]);
+  const int wv = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    for (int c = 0; c < 5; ++c) red[wv][c] = acc[c];
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int c = 0; c < 5; ++c) red[threadIdx.x][c] += red[threadIdx.x + o][c];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
+      for (int c = 0; c < 5; ++c) acc[c] += red[k][c];
+    for (int c = 0; c < 5; ++c) stats[(size_t)r * 5 + c] = acc[c];
   }
-  if (threadIdx.x == 0)
-    for (int c = 0; c < 5; ++c) stats[(size_t)r * 5 + c] = red[0][c];
 }
 
 void launch_stats_reduce(int n_rep, const RepInfo *reps, int lanes,
                          const double *wave_part, double *stats,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_stats_reduce, dim3(n_rep), dim3(256), 0, st, reps,
+  hipLaunchKernelGGL(k_stats_reduce, dim3(n_rep), dim3(1024), 0, st, reps,
                      kWave / lanes, wave_part, stats);
 }
 

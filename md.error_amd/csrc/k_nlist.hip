@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_build_list(
     int n, int TF, const uint4 *__restrict__ pos4,
     const float *__restrict__ rcut, const RepInfo *__restrict__ reps,
     const int *__restrict__ start, float skin, int cap, int *__restrict__ list,
-    int *__restrict__ nnb, unsigned long long *counters) {
+    int *__restrict__ nnb, int2 *__restrict__ part) {
   constexpr int A = kWave / TB;
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / TB, l = lane % TB;
@@ -113,8 +113,7 @@ __global__ __launch_bounds__(256) void k_build_list(
     }
     if (l == 0) nnb[s] = count;
   }
-  // counters[0]: longest list, counters[1]: total list length (integer
-  // atomics, one per wave: order independent)
+  // this wave's longest list and total list length
   int m = count;
   int t = (l == 0) ? count : 0;
 #pragma unroll
@@ -122,16 +121,21 @@ __global__ __launch_bounds__(256) void k_build_list(
     m = max(m, __shfl_xor(m, o, kWave));
     t += __shfl_xor(t, o, kWave);
   }
-  if (lane == 0 && m > 0) {
-    atomicMax(&counters[0], (unsigned long long)m);
-    atomicAdd(&counters[1], (unsigned long long)t);
-  }
+  if (lane == 0) part[wave] = make_int2(m, t);
+}
+
+int build_part_count(int mode, int n, int lanes, int n_cols) {
+  if (n <= 0) return 0;
+  if (mode == 1) return (n + kWave - 1) / kWave + n_cols;  // tile grid
+  const int A = kWave / lanes;
+  const int waves = (n + A - 1) / A;
+  return (waves * kWave + 255) / 256 * (256 / kWave);     // whole 256 blocks
 }
 
 void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
-                       float skin, int cap, int *list, int *nnb,
-                       unsigned long long *counters, hipStream_t st) {
+                       float skin, int cap, int *list, int *nnb, int2 *part,
+                       hipStream_t st) {
   if (n <= 0) return;
   const int A = kWave / lanes;
   const int waves = (n + A - 1) / A;
@@ -140,7 +144,7 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
 #define SRE_BUILD(TB)                                                        \
   hipLaunchKernelGGL(k_build_list<TB>, dim3(grid), dim3(block), 0, st, n,   \
                      force_lanes, pos4, rcut, reps, start, skin, cap, list, \
-                     nnb, counters)
+                     nnb, part)
   switch (lanes) {
     case 1: SRE_BUILD(1); break;
     case 2: SRE_BUILD(2); break;
@@ -193,11 +197,14 @@ __global__ __launch_bounds__(kWave) void k_build_tiles(
     const uint32_t *__restrict__ tiles_per_col, const uint4 *__restrict__ pos4,
     const float *__restrict__ rcut, const RepInfo *__restrict__ reps,
     const int *__restrict__ start, float skin, int cap, int *__restrict__ list,
-    int *__restrict__ nnb, unsigned long long *counters) {
+    int *__restrict__ nnb, int2 *__restrict__ part) {
   __shared__ uint4 cand[kWave];
   const int t = blockIdx.x;
   const int n_tiles = (int)(tile_off[n_cols - 1] + tiles_per_col[n_cols - 1]);
-  if (t >= n_tiles) return;  // the grid is an upper bound of the tile count
+  if (t >= n_tiles) {  // the grid is an upper bound of the tile count
+    if (threadIdx.x == 0) part[t] = make_int2(0, 0);
+    return;
+  }
   // column of this tile: last c with tile_off[c] <= t
   int lo = 0, hi = n_cols - 1;
   while (lo < hi) {
@@ -294,18 +301,15 @@ __global__ __launch_bounds__(kWave) void k_build_tiles(
     mx = max(mx, __shfl_xor(mx, o, kWave));
     tot += __shfl_xor(tot, o, kWave);
   }
-  if (lane == 0 && mx > 0) {
-    atomicMax(&counters[0], (unsigned long long)mx);
-    atomicAdd(&counters[1], (unsigned long long)tot);
-  }
+  if (lane == 0) part[t] = make_int2(mx, tot);
 }
 
 void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
                         const uint4 *pos4, const float *rcut,
                         const RepInfo *reps, const int *start, float skin,
-                        int cap, int *list, int *nnb,
-                        unsigned long long *counters, uint32_t *tiles_per_col,
-                        uint32_t *tile_off, uint32_t *scan_tmp, hipStream_t st) {
+                        int cap, int *list, int *nnb, int2 *part,
+                        uint32_t *tiles_per_col, uint32_t *tile_off,
+                        uint32_t *scan_tmp, hipStream_t st) {
   if (n <= 0 || n_cols <= 0) return;
   hipLaunchKernelGGL(k_col_tiles, dim3((n_cols + 255) / 256), dim3(256), 0, st,
                      n_cols, n_rep, reps, start, tiles_per_col);
@@ -314,7 +318,7 @@ void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
   const int max_tiles = (n + kWave - 1) / kWave + n_cols;
   hipLaunchKernelGGL(k_build_tiles, dim3(max_tiles), dim3(kWave), 0, st, n_cols,
                      n_rep, force_lanes, tile_off, tiles_per_col, pos4, rcut,
-                     reps, start, skin, cap, list, nnb, counters);
+                     reps, start, skin, cap, list, nnb, part);
 }
 
 }  // namespace sre

@@ -99,6 +99,31 @@ __host__ __device__ inline int rc_bin(float x, float L, int n) {
   return ix;
 }
 
+// Counter-based Gaussian deviates of (seed, step, atom): SplitMix64 (Steele,
+// Lea and Flood, "Fast splittable pseudorandom number generators", OOPSLA
+// 2014) hashes followed by the Box-Muller transform, so a perturbation does
+// not depend on launch shape or on the order atoms are visited in.
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ inline float u01(uint64_t x) {
+  // (0, 1]: never zero, so log() is finite
+  return ((float)(x >> 40) + 1.0f) * (1.0f / 16777216.0f);
+}
+
+__device__ inline float3 gauss3(uint64_t seed, uint64_t step, uint64_t i) {
+  const uint64_t base = splitmix64(seed ^ splitmix64(step * 0x632BE59BD9B4E019ull ^ i));
+  const float u1 = u01(splitmix64(base + 1)), u2 = u01(splitmix64(base + 2));
+  const float u3 = u01(splitmix64(base + 3)), u4 = u01(splitmix64(base + 4));
+  const float r1 = sqrtf(-2.f * logf(u1)), r2 = sqrtf(-2.f * logf(u3));
+  const float t1 = 6.2831853f * u2, t2 = 6.2831853f * u4;
+  return make_float3(r1 * cosf(t1), r1 * sinf(t1), r2 * cosf(t2));
+}
+
 __device__ __forceinline__ float disp(uint32_t uj, uint32_t ui, float q) {
   return (float)(int32_t)(uj - ui) * q;
 }
@@ -152,25 +177,50 @@ void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, uint4 *pos4, uint4 *backup, float4 *res4,
                           float *rcut, int *tail_idx, hipStream_t st);
+// Both builders write one (longest list, total length) pair per wavefront to
+// part[] (build_part_count entries, every one written by every launch);
+// launch_rebuild_info reduces them.
+int build_part_count(int mode, int n, int lanes, int n_cols);
 void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
-                       float skin, int cap, int *list, int *nnb,
-                       unsigned long long *counters, hipStream_t st);
+                       float skin, int cap, int *list, int *nnb, int2 *part,
+                       hipStream_t st);
 // Tile build: tiles of <= 64 atoms inside one cell column.  tiles_per_col and
 // tile_off (n_cols each) are scratch arrays filled on the device.
 void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
                         const uint4 *pos4, const float *rcut,
                         const RepInfo *reps, const int *start, float skin,
-                        int cap, int *list, int *nnb,
-                        unsigned long long *counters, uint32_t *tiles_per_col,
-                        uint32_t *tile_off, uint32_t *scan_tmp, hipStream_t st);
+                        int cap, int *list, int *nnb, int2 *part,
+                        uint32_t *tiles_per_col, uint32_t *tile_off,
+                        uint32_t *scan_tmp, hipStream_t st);
+// counters[0] = longest list, counters[1] = total list length over the
+// n_build partials (skipped when n_build == 0); *skin_bits = the largest
+// squared displacement (float bits) over the n_skin partials, which are
+// zeroed after reading when clear_skin (skipped when n_skin == 0).  One block,
+// fixed order: no atomics.
+void launch_rebuild_info(int n_build, const int2 *build_part,
+                         unsigned long long *counters, int n_skin,
+                         float *skin_part, int clear_skin, unsigned *skin_bits,
+                         hipStream_t st);
 void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                        uint32_t *tmp, hipStream_t st);
 // step_ctr (optional) is advanced by one thread after the positions are read
-void launch_update_positions(int n, const uint32_t *perm, const float *pos,
+// Optional perturbation folded into the position refresh:
+// pos = (anchor ? anchor : pos) + sigma * gauss3(seed, *step_ctr, atom).
+struct PerturbArgs {
+  float sigma;
+  uint64_t seed;
+  const float *anchor;
+  const uint64_t *step_ctr;
+};
+// Position refresh + skin test: each wavefront keeps the largest squared
+// displacement it has seen since the last rebuild in skin_part[wave]
+// (skin_part_count(n) entries).
+inline int skin_part_count(int n) { return (n + kWave - 1) / kWave; }
+void launch_update_positions(int n, const uint32_t *perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
-                             unsigned *skin_flag, uint64_t *step_ctr,
+                             float *skin_part, const PerturbArgs *pert,
                              hipStream_t st);
 
 struct ForceArgs {
@@ -193,6 +243,7 @@ struct ForceArgs {
   double *wave_part;           // per wave: 2 slots x 5 (E, Wxx, Wyy, Wzz, n_in)
   double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
   unsigned *err_flag;
+  uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
 };
 void launch_force(int precision, int lanes, int block, bool stats, bool tail,
@@ -252,9 +303,7 @@ void launch_adjoint_tail(int n_bins, const int *entry_rep, const RepInfo *reps,
 
 void launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_i64(int64_t *p, int64_t v, size_t n, hipStream_t st);
-// step_ctr != NULL: the step index is read from device memory
-void launch_perturb(int n, float *pos, const float *anchor, int stride,
-                    float sigma, uint64_t seed, uint64_t step,
-                    const uint64_t *step_ctr, hipStream_t st);
+void launch_perturb(int n, float *pos, int stride, float sigma, uint64_t seed,
+                    uint64_t step, hipStream_t st);
 
 }  // namespace sre
