@@ -65,7 +65,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1};
+  sre_tuning tune{4, 8, 256, 0.5f, 0.25f, 0, 1};
 
   // system
   int n_rep = 0, n_atoms = 0;
@@ -1311,7 +1311,7 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   }
   // phase 1: binning + list build
   const float cells_xy[] = {0.35f, 0.5f, 0.7f, 1.0f};
-  const float cells_z[] = {0.25f, 0.35f, 0.5f, 1.0f};
+  const float cells_z[] = {0.2f, 0.25f, 0.35f, 0.5f};
   // (build_mode, build_lanes): per-atom search with 4/8/16 lanes, or tiles
   const int builds[][2] = {{0, 4}, {0, 8}, {0, 16}, {1, 8}};
   sre_tuning best = e->tune;

@@ -5,17 +5,33 @@
 //     j in L(i)  <=>  j != i  and  |d_ij|^2 < (rc_i + skin)^2
 // with the per-atom radius rc_i + skin of atom i (the list is asymmetric).
 //
-// Candidates: a group of TB lanes per atom walks the cell columns (x, y) that
-// intersect its sphere; in each column the cells covering the sphere's chord in
-// z are one or two contiguous runs of cell-sorted atoms, streamed TB at a time.
-// The exact predicate is applied to every candidate, so the candidate windows
-// only need to be conservative.  Accepted neighbours are compacted with a
-// group-wide ballot into a deterministic order and written to the wave-tiled
-// column-major layout the force kernel reads (list_slot()).  Writes beyond the
-// capacity are dropped but still counted; the host regrows and rebuilds.
+// Candidates: a group of TB lanes per atom.  The (x, y) cell columns that
+// intersect the atom's sphere are taken kBuildCols * TB at a time ("chunk");
+// lane l of the group sets up kBuildCols of them: the z cells covering the
+// sphere's chord in that column are one or two contiguous runs of
+// cell-sorted atoms, whose bounds come from the cell start array (all the
+// lane's loads are independent, so they are in flight together).  A prefix
+// sum over the group turns the chunk's runs into one flat candidate sequence
+// held in LDS; the group then streams it kBuildUnroll * TB candidates per
+// round (every lane issues its kBuildUnroll gathers before testing any), so
+// short runs cost no partially filled batches.  The exact predicate is
+// applied to every candidate, so the windows only need to be conservative.
+// Accepted neighbours are compacted with a group-wide ballot into a
+// deterministic order and written to the wave-tiled column-major layout the
+// force kernel reads (list_slot()).  Writes beyond the capacity are dropped
+// but still counted; the host regrows and rebuilds.
 #include "sre_internal.h"
 
 namespace sre {
+
+constexpr int kBuildCols = 8;     // columns per lane per chunk
+constexpr int kBuildUnroll = 4;   // candidates per lane per round
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
 
 template <int TB>
 __global__ __launch_bounds__(256) void k_build_list(
@@ -24,98 +40,161 @@ __global__ __launch_bounds__(256) void k_build_list(
     const int *__restrict__ start, float skin, int cap, int *__restrict__ list,
     int *__restrict__ nnb, int2 *__restrict__ part) {
   constexpr int A = kWave / TB;
+  constexpr int CC = kBuildCols * TB;   // columns per chunk
+  constexpr int NS = 2 * CC;            // runs per chunk
+  constexpr int U = kBuildUnroll;
+  __shared__ int s_p0[256 / TB][NS];    // first sorted atom of each run
+  __shared__ int s_end[256 / TB][NS];   // inclusive prefix of run lengths
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / TB, l = lane % TB;
+  const int gb = threadIdx.x / TB;
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + g;
+  const bool act = s < n;
+  const int si = act ? s : n - 1;
   const int shift = g * TB;
   const uint64_t gmask = (TB == 64) ? ~0ull : ((1ull << TB) - 1ull);
   const uint64_t below = (1ull << l) - 1ull;
+
+  const uint4 pi = pos4[si];
+  const RepInfo &R = reps[pi.w >> 8];
+  const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
+  const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
+  const float rl = rcut[si] + skin;
+  const float rl2 = rl * rl;
+  const float rlm = rl * 1.0001f + 1e-4f;  // conservative window radius
+  const float rlm2 = rlm * rlm;
+  const int cx = cell_of(pi.x, ncx), cy = cell_of(pi.y, ncy),
+            cz = cell_of(pi.z, ncz);
+  // position inside the own cell from the low 32 bits of u * nc
+  const float fx = (float)(uint32_t)((uint64_t)pi.x * (uint64_t)ncx) * (hx * 0x1p-32f);
+  const float fy = (float)(uint32_t)((uint64_t)pi.y * (uint64_t)ncy) * (hy * 0x1p-32f);
+  const float fz = (float)(uint32_t)((uint64_t)pi.z * (uint64_t)ncz) * (hz * 0x1p-32f);
+  const int kx = (int)ceilf(rlm / hx), ky = (int)ceilf(rlm / hy),
+            kz = (int)ceilf(rlm / hz);
+  const bool fullx = 2 * kx + 1 >= ncx, fully = 2 * ky + 1 >= ncy,
+             fullz = 2 * kz + 1 >= ncz;
+  const int ox0 = fullx ? -cx : -kx, ox1 = fullx ? ncx - 1 - cx : kx;
+  const int oy0 = fully ? -cy : -ky, oy1 = fully ? ncy - 1 - cy : ky;
+  const int noy = oy1 - oy0 + 1;
+  const int ncol = act ? (ox1 - ox0 + 1) * noy : 0;
+  const int ncol_w = __builtin_amdgcn_readfirstlane(wave_max_i(ncol));
+
   int count = 0;
-  if (s < n) {
-    const uint4 pi = pos4[s];
-    const RepInfo &R = reps[pi.w >> 8];
-    const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
-    const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
-    const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
-    const float rl = rcut[s] + skin;
-    const float rl2 = rl * rl;
-    const float rlm = rl * 1.0001f + 1e-4f;  // conservative window radius
-    const float rlm2 = rlm * rlm;
-    const int cx = cell_of(pi.x, ncx), cy = cell_of(pi.y, ncy),
-              cz = cell_of(pi.z, ncz);
-    // position inside the own cell from the low 32 bits of u * nc
-    const float fx = (float)(uint32_t)((uint64_t)pi.x * (uint64_t)ncx) * (hx * 0x1p-32f);
-    const float fy = (float)(uint32_t)((uint64_t)pi.y * (uint64_t)ncy) * (hy * 0x1p-32f);
-    const float fz = (float)(uint32_t)((uint64_t)pi.z * (uint64_t)ncz) * (hz * 0x1p-32f);
-    const int kx = (int)ceilf(rlm / hx), ky = (int)ceilf(rlm / hy),
-              kz = (int)ceilf(rlm / hz);
-    const bool fullx = 2 * kx + 1 >= ncx, fully = 2 * ky + 1 >= ncy,
-               fullz = 2 * kz + 1 >= ncz;
-    const int ox0 = fullx ? -cx : -kx, ox1 = fullx ? ncx - 1 - cx : kx;
-    const int oy0 = fully ? -cy : -ky, oy1 = fully ? ncy - 1 - cy : ky;
-    for (int ox = ox0; ox <= ox1; ++ox) {
-      float gx = 0.f;
-      if (!fullx) gx = ox > 0 ? ox * hx - fx : (ox < 0 ? fx + (-ox - 1) * hx : 0.f);
-      gx = fmaxf(gx, 0.f);
-      if (gx * gx >= rlm2) continue;
-      int cxw = cx + ox;
-      cxw += cxw < 0 ? ncx : 0;
-      cxw -= cxw >= ncx ? ncx : 0;
-      for (int oy = oy0; oy <= oy1; ++oy) {
-        float gy = 0.f;
+  for (int cbase = 0; cbase < ncol_w; cbase += CC) {
+    // runs of this lane's kBuildCols columns
+    int p0[2 * kBuildCols], len[2 * kBuildCols];
+#pragma unroll
+    for (int m = 0; m < kBuildCols; ++m) {
+      const int c = cbase + l * kBuildCols + m;
+      int ra0 = 0, rb0 = -1, ra1 = 0, rb1 = -1, colbase = 0;
+      if (c < ncol) {
+        const int ox = ox0 + c / noy, oy = oy0 + c % noy;
+        float gx = 0.f, gy = 0.f;
+        if (!fullx) gx = ox > 0 ? ox * hx - fx : (ox < 0 ? fx + (-ox - 1) * hx : 0.f);
         if (!fully) gy = oy > 0 ? oy * hy - fy : (oy < 0 ? fy + (-oy - 1) * hy : 0.f);
+        gx = fmaxf(gx, 0.f);
         gy = fmaxf(gy, 0.f);
         const float gxy2 = gx * gx + gy * gy;
-        if (gxy2 >= rlm2) continue;
-        int cyw = cy + oy;
-        cyw += cyw < 0 ? ncy : 0;
-        cyw -= cyw >= ncy ? ncy : 0;
-        const int colbase = R.cell_off + (cxw * ncy + cyw) * ncz;
-        int ra0 = 0, rb0 = ncz - 1, ra1 = 0, rb1 = -1;
-        if (!fullz) {
-          const float dz = sqrtf(fmaxf(rlm2 - gxy2, 0.f));
-          const int lo = cz + (int)floorf((fz - dz) / hz);
-          const int hi = cz + (int)floorf((fz + dz) / hz);
-          if (hi - lo + 1 < ncz) {
-            if (lo < 0) {
-              ra0 = lo + ncz; rb0 = ncz - 1; ra1 = 0; rb1 = hi;
-            } else if (hi >= ncz) {
-              ra0 = lo; rb0 = ncz - 1; ra1 = 0; rb1 = hi - ncz;
-            } else {
-              ra0 = lo; rb0 = hi;
+        if (gxy2 < rlm2) {
+          int cxw = cx + ox;
+          cxw += cxw < 0 ? ncx : 0;
+          cxw -= cxw >= ncx ? ncx : 0;
+          int cyw = cy + oy;
+          cyw += cyw < 0 ? ncy : 0;
+          cyw -= cyw >= ncy ? ncy : 0;
+          colbase = R.cell_off + (cxw * ncy + cyw) * ncz;
+          rb0 = ncz - 1;  // whole column unless the chord is shorter
+          if (!fullz) {
+            const float dz = sqrtf(fmaxf(rlm2 - gxy2, 0.f));
+            const int lo = cz + (int)floorf((fz - dz) / hz);
+            const int hi = cz + (int)floorf((fz + dz) / hz);
+            if (hi - lo + 1 < ncz) {
+              if (lo < 0) {
+                ra0 = lo + ncz; rb0 = ncz - 1; ra1 = 0; rb1 = hi;
+              } else if (hi >= ncz) {
+                ra0 = lo; rb0 = ncz - 1; ra1 = 0; rb1 = hi - ncz;
+              } else {
+                ra0 = lo; rb0 = hi;
+              }
             }
-          }
-        }
-        for (int run = 0; run < 2; ++run) {
-          const int ca = run ? ra1 : ra0, cb = run ? rb1 : rb0;
-          if (cb < ca) continue;
-          const int p0 = start[colbase + ca], p1 = start[colbase + cb + 1];
-          for (int p = p0; p < p1; p += TB) {
-            const int pj = p + l;
-            bool ok = false;
-            if (pj < p1 && pj != s) {
-              const uint4 pj4 = pos4[pj];
-              const float dx = disp(pj4.x, pi.x, qx);
-              const float dy = disp(pj4.y, pi.y, qy);
-              const float dz2 = disp(pj4.z, pi.z, qz);
-              ok = dist2(dx, dy, dz2) < rl2;
-            }
-            const uint64_t gb = (__ballot(ok) >> shift) & gmask;
-            if (ok) {
-              const int k = count + __popcll(gb & below);
-              if (k < cap) list[list_slot(s, k, TF, cap)] = pj;
-            }
-            count += __popcll(gb);
           }
         }
       }
+      const int b0 = rb0 >= ra0 ? start[colbase + ra0] : 0;
+      const int e0 = rb0 >= ra0 ? start[colbase + rb0 + 1] : 0;
+      const int b1 = rb1 >= ra1 ? start[colbase + ra1] : 0;
+      const int e1 = rb1 >= ra1 ? start[colbase + rb1 + 1] : 0;
+      p0[2 * m] = b0;
+      len[2 * m] = e0 - b0;
+      p0[2 * m + 1] = b1;
+      len[2 * m + 1] = e1 - b1;
     }
-    if (l == 0) nnb[s] = count;
+    // flat sequence: prefix over the group (lane-major slot order)
+    int tot = 0;
+#pragma unroll
+    for (int r = 0; r < 2 * kBuildCols; ++r) tot += len[r];
+    int incl = tot;
+#pragma unroll
+    for (int o = 1; o < TB; o <<= 1) {
+      const int t = __shfl_up(incl, o, TB);
+      if (l >= o) incl += t;
+    }
+    const int T = __shfl(incl, TB - 1, TB);
+    int run = incl - tot;
+#pragma unroll
+    for (int r = 0; r < 2 * kBuildCols; ++r) {
+      s_p0[gb][l * 2 * kBuildCols + r] = p0[r];
+      run += len[r];
+      s_end[gb][l * 2 * kBuildCols + r] = run;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // stream the flat sequence: lane l takes v = l, l + TB, ...
+    const int rounds = __builtin_amdgcn_readfirstlane(wave_max_i((T + TB * U - 1) / (TB * U)));
+    int cur = 0, beg = 0, end = s_end[gb][0];
+    for (int k0 = 0; k0 < rounds; ++k0) {
+      uint4 cj[U];
+      int jj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int v = l + TB * (k0 * U + u);
+        int j = -1;
+        if (v < T) {
+          while (v >= end) {
+            ++cur;
+            beg = end;
+            end = s_end[gb][cur];
+          }
+          j = s_p0[gb][cur] + (v - beg);
+        }
+        jj[u] = j;
+        cj[u] = pos4[j >= 0 ? j : si];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = jj[u];
+        bool ok = false;
+        if (j >= 0 && j != s) {
+          const float dx = disp(cj[u].x, pi.x, qx);
+          const float dy = disp(cj[u].y, pi.y, qy);
+          const float dz2 = disp(cj[u].z, pi.z, qz);
+          ok = dist2(dx, dy, dz2) < rl2;
+        }
+        const uint64_t gbits = (__ballot(ok) >> shift) & gmask;
+        if (ok) {
+          const int k = count + __popcll(gbits & below);
+          if (k < cap) list[list_slot(s, k, TF, cap)] = j;
+        }
+        count += __popcll(gbits);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // LDS reuse by the next chunk
   }
+  if (act && l == 0) nnb[s] = count;
   // this wave's longest list and total list length
-  int m = count;
-  int t = (l == 0) ? count : 0;
+  int m = act ? count : 0;
+  int t = (act && l == 0) ? count : 0;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) {
     m = max(m, __shfl_xor(m, o, kWave));
