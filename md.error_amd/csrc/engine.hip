@@ -130,7 +130,8 @@ struct sre_engine {
   size_t list_size = 0;
   int *d_list = nullptr;
   int *d_nnb = nullptr;
-  unsigned long long *d_counters = nullptr;  // [0] longest list, [1] total
+  // [0] longest list, [1] total length, [2] low word: max displacement^2
+  unsigned long long *d_counters = nullptr;
   uint64_t *h_info = nullptr;               // pinned readback (4 words)
   float last_disp2 = 0.f;                   // max displacement^2 before the last rebuild
   int max_count = 0;
@@ -240,7 +241,8 @@ extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
   e->shift.assign(e->n_types * e->n_types, 0.f);
   hipError_t r = dalloc(&e->d_flags, 4);
   if (r == hipSuccess) r = hipMemsetAsync(e->d_flags, 0, 4 * sizeof(unsigned), e->stream);
-  if (r == hipSuccess) r = dalloc(&e->d_counters, 2);
+  if (r == hipSuccess) r = dalloc(&e->d_counters, 3);
+  if (r == hipSuccess) r = hipMemsetAsync(e->d_counters, 0, 3 * sizeof(uint64_t), e->stream);
   if (r == hipSuccess) r = hipHostMalloc((void **)&e->h_info, 4 * sizeof(uint64_t), hipHostMallocDefault);
   if (r == hipSuccess) r = dalloc(&e->d_params, kMaxPairTypes);
   if (r == hipSuccess) r = dalloc(&e->d_paramsd, kMaxPairTypes);
@@ -768,12 +770,10 @@ static sre_status do_rebuild(sre_engine *e) {
     // cleared for the next segment)
     launch_rebuild_info(n_parts, e->d_build_part, e->d_counters,
                         attempt == 0 ? skin_part_count(N) : 0, e->d_skin_part,
-                        1, e->d_flags, st);
+                        1, reinterpret_cast<unsigned *>(e->d_counters + 2), st);
     HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
     // one pinned readback per rebuild
-    HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 2 * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(e->h_info + 2, e->d_flags, sizeof(uint64_t),
+    HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 3 * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     const int mc = (int)e->h_info[0];
