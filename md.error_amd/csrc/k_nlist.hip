@@ -126,10 +126,12 @@ __global__ __launch_bounds__(256) void k_build_list(
       const int e0 = rb0 >= ra0 ? start[colbase + rb0 + 1] : 0;
       const int b1 = rb1 >= ra1 ? start[colbase + ra1] : 0;
       const int e1 = rb1 >= ra1 ? start[colbase + rb1 + 1] : 0;
-      p0[2 * m] = b0;
-      len[2 * m] = e0 - b0;
-      p0[2 * m + 1] = b1;
-      len[2 * m + 1] = e1 - b1;
+      // runs lie inside [0, n) by construction; the clamps keep every gather
+      // in bounds whatever the cell offsets hold
+      p0[2 * m] = min(max(b0, 0), n);
+      len[2 * m] = max(min(e0, n) - p0[2 * m], 0);
+      p0[2 * m + 1] = min(max(b1, 0), n);
+      len[2 * m + 1] = max(min(e1, n) - p0[2 * m + 1], 0);
     }
     // flat sequence: prefix over the group (lane-major slot order)
     int tot = 0;

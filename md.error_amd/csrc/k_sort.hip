@@ -3,20 +3,25 @@
 //
 // The reference bins atoms with fixed-capacity cells and atomicInc slot
 // insertion (H/src/NeighborList.cu:84-167), whose in-cell order depends on
-// scheduling.  Here binning is a stable counting sort built from
-//   (1) a per-block bitonic sort of (digit, local index) in LDS,
-//   (2) a digit-major exclusive scan of per-block digit counts,
-//   (3) a stable scatter,
-// repeated over 1-3 digits of the cell key.  No atomics are used, so the cell
-// order (and everything derived from it) is a pure function of the input.
+// scheduling.  Here binning is a stable LSD radix sort of the cell key over
+// digits of at most 8 bits, each pass built from
+//   (1) a per-block stable ranking of 1024 keys: lanes holding the same digit
+//       find each other with one ballot per digit bit (their rank is the
+//       number of such lanes below them), per-wave digit counts go through
+//       LDS, and the keys are written in (digit, original position) order;
+//   (2) a digit-major exclusive scan of the per-block digit counts;
+//   (3) a stable scatter to the global positions.
+// No atomics are used, so the cell order (and everything derived from it) is
+// a pure function of the input.
 #include "sre_internal.h"
 
 namespace sre {
 
-constexpr int kSortE = 2048;      // elements per sort block
+constexpr int kSortE = 1024;      // elements per sort block (one per thread)
 constexpr int kSortT = 1024;      // threads per sort block
-constexpr int kMaxDigitBits = 12;
-constexpr int kScanTile = 4096;   // elements per scan block (1024 threads x 4)
+constexpr int kMaxDigitBits = 8;
+constexpr int kMaxPasses = 4;
+constexpr int kScanTile = 4096;   // elements per scan tile (1024 threads x 4)
 
 // ------------------------------------------------------------------ keys --
 
@@ -76,55 +81,54 @@ __device__ uint32_t block_exscan_1024(uint32_t v, uint32_t *tmp,
   return r;
 }
 
-// ------------------------------------------------------------ sort block --
+// ------------------------------------------------------------ rank block --
 
+// Stable local ranking of one block of keys by digit (key >> shift) & mask:
+// writes the block's keys/values to tkey/tval in (digit, position) order and
+// the block's digit counts digit-major to counts[d * nblocks + b].
 __global__ __launch_bounds__(kSortT) void k_sort_block(
     int n, int shift, int bits, int nblocks, const uint32_t *key,
     const uint32_t *val, uint32_t *tkey, uint32_t *tval, uint32_t *counts) {
-  __shared__ uint32_t sk[kSortE];
-  __shared__ int run_start[1 << kMaxDigitBits];
-  __shared__ uint32_t cnt[1 << kMaxDigitBits];
-  const int b = blockIdx.x;
+  constexpr int W = kSortT / 64;
+  __shared__ uint32_t wcnt[W][1 << kMaxDigitBits];  // per wave, then prefix
+  __shared__ uint32_t dstart[1 << kMaxDigitBits];
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
   const int base = b * kSortE;
-  const int nv = min(kSortE, n - base);
+  const bool valid = base + t < n;
   const uint32_t R = 1u << bits, mask = R - 1u;
-  for (int t = threadIdx.x; t < kSortE; t += kSortT)
-    sk[t] = (t < nv) ? ((((key[base + t] >> shift) & mask) << 11) | (uint32_t)t)
-                     : 0xFFFFFFFFu;
-  for (uint32_t d = threadIdx.x; d < R; d += kSortT) cnt[d] = 0;
+  const uint32_t k = valid ? key[base + t] : 0u;
+  const uint32_t v = valid ? val[base + t] : 0u;
+  const uint32_t d = (k >> shift) & mask;
+  for (int x = t; x < W * (1 << kMaxDigitBits); x += kSortT) (&wcnt[0][0])[x] = 0u;
+  // lanes of this wave with the same digit
+  uint64_t peers = __ballot(valid);
+  for (int bit = 0; bit < bits; ++bit) {
+    const bool on = (d >> bit) & 1u;
+    const uint64_t bal = __ballot(on);
+    peers &= on ? bal : ~bal;
+  }
+  const int rank = __popcll(peers & ((1ull << lane) - 1ull));
   __syncthreads();
-  // bitonic sort of kSortE unique 32-bit keys, one compare-swap per thread
-  for (int k = 2; k <= kSortE; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const int t = threadIdx.x;
-      const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-      const int p = i | j;
-      uint32_t a = sk[i], c = sk[p];
-      const bool asc = (i & k) == 0;
-      if ((a > c) == asc) {
-        sk[i] = c;
-        sk[p] = a;
-      }
-      __syncthreads();
+  if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+  __syncthreads();
+  uint32_t bc = 0;
+  if ((uint32_t)t < R) {
+    for (int ww = 0; ww < W; ++ww) {
+      const uint32_t c = wcnt[ww][t];
+      wcnt[ww][t] = bc;
+      bc += c;
     }
+    counts[(size_t)t * nblocks + b] = bc;
   }
-  for (int p = threadIdx.x; p < nv; p += kSortT) {
-    uint32_t d = sk[p] >> 11;
-    if (p == 0 || (sk[p - 1] >> 11) != d) run_start[d] = p;
-  }
+  const uint32_t ds = block_exscan_1024((uint32_t)t < R ? bc : 0u, tmp, nullptr);
+  if ((uint32_t)t < R) dstart[t] = ds;
   __syncthreads();
-  for (int p = threadIdx.x; p < nv; p += kSortT) {
-    uint32_t d = sk[p] >> 11;
-    if (p == nv - 1 || (sk[p + 1] >> 11) != d)
-      cnt[d] = (uint32_t)(p + 1 - run_start[d]);
-  }
-  __syncthreads();
-  for (uint32_t d = threadIdx.x; d < R; d += kSortT)
-    counts[(size_t)d * nblocks + b] = cnt[d];
-  for (int p = threadIdx.x; p < nv; p += kSortT) {
-    int li = (int)(sk[p] & 2047u);
-    tkey[base + p] = key[base + li];
-    tval[base + p] = val[base + li];
+  if (valid) {
+    const uint32_t pos = dstart[d] + wcnt[w][d] + (uint32_t)rank;
+    tkey[base + pos] = k;
+    tval[base + pos] = v;
   }
 }
 
@@ -177,20 +181,30 @@ __global__ __launch_bounds__(kSortT) void k_scan_reduce(size_t m,
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// single-block exclusive scan of m elements (in place allowed)
+// single-block exclusive scan of m elements, kScanTile at a time with a
+// running carry (in place allowed: a tile is read before it is written)
 __global__ __launch_bounds__(kSortT) void k_scan_single(size_t m,
                                                         const uint32_t *in,
                                                         uint32_t *out) {
   __shared__ uint32_t tmp[kSortT / 64 + 1];
-  const size_t per = (m + kSortT - 1) / kSortT;
-  const size_t lo = min(m, (size_t)threadIdx.x * per), hi = min(m, lo + per);
-  uint32_t local = 0;
-  for (size_t i = lo; i < hi; ++i) local += in[i];
-  uint32_t run = block_exscan_1024(local, tmp, nullptr);
-  for (size_t i = lo; i < hi; ++i) {
-    uint32_t v = in[i];
-    out[i] = run;
-    run += v;
+  uint32_t carry = 0;
+  for (size_t base = 0; base < m; base += kScanTile) {
+    const size_t i0 = base + (size_t)threadIdx.x * 4;
+    uint32_t v[4];
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = (i0 + k < m) ? in[i0 + k] : 0u;
+      local += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exscan_1024(local, tmp, &total) + carry;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < m) out[i0 + k] = run;
+      run += v[k];
+    }
+    carry += total;
   }
 }
 
@@ -215,7 +229,7 @@ __global__ __launch_bounds__(kSortT) void k_scan_down(size_t m,
 
 void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                        uint32_t *tmp, hipStream_t st) {
-  if (m <= 65536) {
+  if (m <= 4 * (size_t)kScanTile) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSortT), 0, st, m, in,
                        out);
     return;
@@ -229,7 +243,7 @@ void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                      in, tmp, out);
 }
 
-static void digit_plan(int key_bits, int *np, int bits[3]) {
+static void digit_plan(int key_bits, int *np, int bits[kMaxPasses]) {
   if (key_bits < 1) key_bits = 1;
   int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
   int rem = key_bits;
@@ -241,7 +255,7 @@ static void digit_plan(int key_bits, int *np, int bits[3]) {
 }
 
 size_t radix_counts_capacity(int n, int key_bits) {
-  int np, bits[3];
+  int np, bits[kMaxPasses];
   digit_plan(key_bits, &np, bits);
   int mb = 0;
   for (int p = 0; p < np; ++p) mb = bits[p] > mb ? bits[p] : mb;
@@ -251,8 +265,8 @@ size_t radix_counts_capacity(int n, int key_bits) {
 
 int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
   if (n <= 0) return 0;
-  if (key_bits > 3 * kMaxDigitBits) return -1;
-  int np, bits[3];
+  if (key_bits > kMaxPasses * kMaxDigitBits) return -1;
+  int np, bits[kMaxPasses];
   digit_plan(key_bits, &np, bits);
   const int nblocks = (n + kSortE - 1) / kSortE;
   uint32_t *kin = b.key_a, *vin = b.val_a, *kout = b.key_b, *vout = b.val_b;
@@ -282,24 +296,23 @@ int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
 
 // ------------------------------------------------------------- cell start --
 
+// start[c] = first sorted position with key >= c, for c = 0..n_cells: thread
+// p (< n) fills the cells from key[p-1] + 1 to key[p]; thread n fills the
+// cells after the last key.  Every entry is written exactly once.
 __global__ __launch_bounds__(256) void k_cell_start(int n_cells, int n,
                                                     const uint32_t *keys,
                                                     int *start) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > n_cells) return;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (keys[mid] < (uint32_t)c) lo = mid + 1;
-    else hi = mid;
-  }
-  start[c] = lo;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > n) return;
+  const int lo = p == 0 ? 0 : (int)keys[p - 1] + 1;
+  const int hi = p == n ? n_cells : (int)keys[p];
+  for (int c = lo; c <= hi; ++c) start[c] = p;
 }
 
 void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_start, dim3((n_cells + 1 + 255) / 256), dim3(256),
-                     0, st, n_cells, n, keys, start);
+  hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + 255) / 256), dim3(256), 0, st,
+                     n_cells, n, keys, start);
 }
 
 // ---------------------------------------------------------- gather sorted --
