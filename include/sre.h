@@ -244,8 +244,8 @@ sre_status sre_check_skin_flag(sre_engine *e);
 sre_status sre_last_rebuild_displacement(sre_engine *e, float *max_disp);
 
 /* Gaussian displacement of the bound positions (device), std sigma, seeded,
- * counter-based: x += sigma * N(0,1).  The Langevin-style perturbation the
- * benchmark applies between force evaluations. */
+ * counter-based: x += sigma * N(0,1) (the same deviates as a graph replay of
+ * that step).  Stand-in for the integrator step outside the force path. */
 sre_status sre_perturb_positions(sre_engine *e, float sigma, uint64_t seed,
                                  uint64_t step);
 
