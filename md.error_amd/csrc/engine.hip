@@ -722,8 +722,10 @@ static sre_status do_rebuild(sre_engine *e) {
   if (s != SRE_OK) return s;
   const int N = e->n_atoms;
   hipStream_t st = e->stream;
-  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps, e->sb.key_a,
-              e->sb.val_a, st);
+  const bool odd = (radix_passes(e->key_bits) & 1) != 0;
+  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps,
+              odd ? e->sb.key_b : e->sb.key_a, odd ? e->sb.val_b : e->sb.val_a,
+              st);
   if (launch_radix_sort(N, e->key_bits, e->sb, st) != 0)
     return fail(SRE_ERR_CELL, "radix sort: key of %d bits unsupported", e->key_bits);
   launch_cell_start(e->n_cells, N, e->sb.key_a, e->d_start, st);

@@ -263,13 +263,22 @@ size_t radix_counts_capacity(int n, int key_bits) {
   return ((size_t)1 << mb) * (nblocks > 0 ? nblocks : 1);
 }
 
-int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
+int radix_passes(int key_bits) {
+  int np, bits[kMaxPasses];
+  digit_plan(key_bits, &np, bits);
+  return np;
+}
+
+int launch_radix_sort(int n, int key_bits, const SortBufs &b, hipStream_t st) {
   if (n <= 0) return 0;
   if (key_bits > kMaxPasses * kMaxDigitBits) return -1;
   int np, bits[kMaxPasses];
   digit_plan(key_bits, &np, bits);
   const int nblocks = (n + kSortE - 1) / kSortE;
-  uint32_t *kin = b.key_a, *vin = b.val_a, *kout = b.key_b, *vout = b.val_b;
+  // ping-pong so that the last pass writes key_a/val_a
+  const bool odd = (np & 1) != 0;
+  uint32_t *kin = odd ? b.key_b : b.key_a, *vin = odd ? b.val_b : b.val_a;
+  uint32_t *kout = odd ? b.key_a : b.key_b, *vout = odd ? b.val_a : b.val_b;
   int shift = 0;
   for (int p = 0; p < np; ++p) {
     const size_t m = ((size_t)1 << bits[p]) * nblocks;
@@ -284,12 +293,6 @@ int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st) {
     uint32_t *t;
     t = kin; kin = kout; kout = t;
     t = vin; vin = vout; vout = t;
-  }
-  if (kin != b.key_a) {
-    // odd number of passes ended in key_b/val_b: swap roles for the caller
-    uint32_t *t;
-    t = b.key_a; b.key_a = b.key_b; b.key_b = t;
-    t = b.val_a; b.val_a = b.val_b; b.val_b = t;
   }
   return 0;
 }

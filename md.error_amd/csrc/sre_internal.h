@@ -166,9 +166,13 @@ struct SortBufs {
 void launch_keys(int n, const float *pos, int stride, const int *rep_of,
                  const RepInfo *reps, uint32_t *key, uint32_t *val,
                  hipStream_t st);
-// Sorts (key, val) ascending by key (stable), key < 2^key_bits.  Result in
-// key_a/val_a of bufs.  Returns 0 on success.
-int launch_radix_sort(int n, int key_bits, SortBufs &b, hipStream_t st);
+// Sorts (key, val) ascending by key (stable), key < 2^key_bits.  The input
+// must be in key_b/val_b when radix_passes(key_bits) is odd, else in
+// key_a/val_a; the result is always in key_a/val_a, so the sorted
+// permutation has a fixed address (captured graphs read it).  Returns 0 on
+// success.
+int radix_passes(int key_bits);
+int launch_radix_sort(int n, int key_bits, const SortBufs &b, hipStream_t st);
 size_t radix_counts_capacity(int n, int key_bits);
 void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
                        hipStream_t st);

@@ -158,6 +158,32 @@ def test_graph_replay_equals_eager_and_skin_flag(torch_cuda, slab):
         e.check_skin_flag()
 
 
+@pytest.mark.parametrize("cell_z", [0.25, 1.0])
+def test_graph_replay_across_rebuilds(torch_cuda, cell_z):
+    """A captured graph keeps reading the sorted permutation after rebuilds.
+    The 64k slab's cell key has 17 bits at cell_z 0.25 (3 radix passes) and
+    15 bits at 1.0 (2 passes): the permutation must end in the same buffer
+    either way, or the replays refresh the wrong atoms."""
+    import torch
+    w = configs.slab()
+    e = Engine(skin=w.skin)
+    e.set_tuning(cell_xy=0.5, cell_z=cell_z)
+    w.setup(e, tail=True)
+    pos = torch.from_numpy(w.xyz.copy()).cuda()
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    e.set_perturbation(0.01, 11, 0, anchored=True)
+    for seg in range(3):
+        e.graph_run(4, stats=True, tail=True, perturb=True)
+        f_graph = e.forces_fixed()
+        e.update_positions(check_skin=False)       # eager, same positions
+        e.compute(stats=True, tail=True)
+        assert np.array_equal(e.forces_fixed(), f_graph), seg
+        e.rebuild()
+        assert e.last_rebuild_displacement() <= 0.5 * w.skin, seg
+
+
 def test_perturbation_is_reproducible(torch_cuda, slab):
     import torch
     outs = []
