@@ -124,6 +124,20 @@ def accuracy(w, torch, Engine):
     return out
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the mixed force kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counter values scaled by a copy
+    kernel of known size), or None when there is none for this workload."""
+    path = os.path.join(REPO, "profiles", f"k_force_mixed_{workload}_hbm.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(w, budget_s):
     """The FP64 oracle (C++/OpenMP) on the host: list build + force evaluations."""
     import numpy as np
@@ -204,14 +218,16 @@ def main():
     log(f"[bench] accuracy: {acc}")
 
     L, N = mixed["n_list"], w.n
-    b_force = 4.0 * L + 52.0 * N
+    b_force = 4.0 * L + 84.0 * N
     achieved = b_force / (mixed["force_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(w.name),
                 "kernel": "k_force_mixed<T,stats,tail>",
                 "bytes_per_launch": b_force,
-                "bytes_def": "4*L + 52*N (int32 list entry per pair; pos 16 + rc 4 + count 4 "
-                             "+ tail index 4 + 3x int64 force 24 B per atom)",
+                "bytes_def": "4*L + 84*N (int32 list entry per pair; per atom: pos 16 + rc 4 "
+                             "+ count 4 + tail index 4 read, 3x int64 force 24 + FP64 "
+                             "energy/virial 32 written)",
                 "avg_launch_us": round(mixed["force_ms"] * 1e3, 3)}
     cpu = None
     if not args.no_cpu:

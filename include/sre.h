@@ -180,7 +180,8 @@ sre_status sre_get_atom_stats(sre_engine *e, double *out);
 
 /* Sum of neighbor-list lengths (the reference's "tc" counter,
  * NeighborList::calSumNeighbor, H/src/NeighborList_interface.cu:652-662),
- * pairs inside rc_i at the last SRE_STATS compute, and the largest list. */
+ * pairs of the list inside rc_i at the current positions (counted on demand
+ * with the force kernel's mask), and the largest list. */
 sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list, int64_t *n_in_rc,
                               int32_t *max_list);
 /* Neighbor lists as CSR in caller order with caller indices, rows sorted

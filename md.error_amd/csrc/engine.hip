@@ -140,9 +140,8 @@ struct sre_engine {
   // outputs
   int64_t *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;
   double *d_fxd = nullptr, *d_fyd = nullptr, *d_fzd = nullptr;
-  double *d_wave_part = nullptr;
-  size_t wave_part_cap = 0;
-  double *d_stats = nullptr;
+  double *d_stats = nullptr;       // per replica (E, Wxx, Wyy, Wzz)
+  unsigned long long *d_nin = nullptr;  // per replica pairs inside rc
   double *d_atom_stats = nullptr;
   double *d_tmp_out = nullptr;
   unsigned *d_flags = nullptr;  // [0] max displacement^2 bits, [1] error bits
@@ -161,7 +160,6 @@ struct sre_engine {
   // the force kernel leaves per-wave partial sums; their fixed-order
   // reduction runs when the statistics are read
   bool stats_pending = false;
-  int stats_lanes = 0;
   bool forces_valid = false;
 
   bool built = false;
@@ -186,7 +184,7 @@ sre_engine::~sre_engine() {
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
-  dfree(d_fyd); dfree(d_fzd); dfree(d_wave_part); dfree(d_stats);
+  dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -313,7 +311,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_res4);
   dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
   dfree(e->d_fxd); dfree(e->d_fyd); dfree(e->d_fzd); dfree(e->d_atom_stats);
-  dfree(e->d_stats); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
+  dfree(e->d_stats); dfree(e->d_nin); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
   dfree(e->d_skin_part);
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
@@ -336,7 +334,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
     HIP_TRY(dalloc(&e->d_fzd, N));
   }
   HIP_TRY(dalloc(&e->d_atom_stats, (size_t)4 * N));
-  HIP_TRY(dalloc(&e->d_stats, (size_t)5 * n_replicas));
+  HIP_TRY(dalloc(&e->d_stats, (size_t)4 * n_replicas));
+  HIP_TRY(dalloc(&e->d_nin, n_replicas));
   HIP_TRY(dalloc(&e->d_tmp_out, (size_t)4 * N));
   HIP_TRY(dalloc(&e->sb.key_a, N));
   HIP_TRY(dalloc(&e->sb.val_a, N));
@@ -680,18 +679,6 @@ static void refresh_tail_values(sre_engine *e) {
 
 // Per-wave statistic partials of the force kernel for the current list
 // layout; allocated at rebuild so that no timed or captured region allocates.
-static sre_status ensure_wave_part(sre_engine *e) {
-  const size_t need = (size_t)force_num_waves(e->n_atoms, e->list_lanes) * 12;
-  if (need > e->wave_part_cap) {
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    dfree(e->d_wave_part);
-    HIP_TRY(dalloc(&e->d_wave_part, need));
-    e->wave_part_cap = need;
-    e->graph_epoch++;
-  }
-  return SRE_OK;
-}
-
 static sre_status ensure_list(sre_engine *e, int cap) {
   const int T = e->tune.force_lanes;
   const int A = kWave / T;
@@ -791,8 +778,6 @@ static sre_status do_rebuild(sre_engine *e) {
     const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
     memcpy(&e->last_disp2, &bits, sizeof(float));
   }
-  s = ensure_wave_part(e);
-  if (s != SRE_OK) return s;
   e->built = true;
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   return SRE_OK;
@@ -847,7 +832,6 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.tail_atom = (e->tail_mode == SRE_TAIL_EXACT) ? e->d_tail_atom : nullptr;
   a.fx = e->d_fx; a.fy = e->d_fy; a.fz = e->d_fz;
   a.fxd = e->d_fxd; a.fyd = e->d_fyd; a.fzd = e->d_fzd;
-  a.wave_part = e->d_wave_part;
   a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
   a.err_flag = e->d_flags + 1;
   a.step_ctr = nullptr;
@@ -862,28 +846,21 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
   const bool stats = flags & SRE_STATS;
   const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
   const int T = e->list_lanes;
-  if (stats) {
-    sre_status sw = ensure_wave_part(e);
-    if (sw != SRE_OK) return sw;
-  }
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
   a.step_ctr = step_ctr;
   launch_force(e->precision, T, e->tune.block, stats, tail, a, e->stream);
-  if (stats) {
-    e->stats_pending = true;
-    e->stats_lanes = T;
-  }
+  if (stats) e->stats_pending = true;
   HIP_TRY(hipGetLastError());  // launch-configuration errors
   return SRE_OK;
 }
 
-// Reduces the per-wave partial sums of the last SRE_STATS compute into
+// Reduces the per-atom statistics of the last SRE_STATS compute into
 // d_stats (stream-ordered; fixed order, so the result is reproducible).
 static sre_status reduce_stats(sre_engine *e) {
   if (e->stats_pending) {
-    launch_stats_reduce(e->n_rep, e->d_reps, e->stats_lanes, e->d_wave_part,
-                        e->d_stats, e->stream);
+    launch_stats_reduce(e->n_rep, e->d_reps, e->d_atom_stats, e->d_stats,
+                        e->stream);
     HIP_TRY(hipGetLastError());
     e->stats_pending = false;
   }
@@ -983,12 +960,9 @@ extern "C" sre_status sre_get_stats(sre_engine *e, double *out) {
   HIP_TRY(hipSetDevice(e->device));
   sre_status rs = reduce_stats(e);
   if (rs != SRE_OK) return rs;
-  std::vector<double> st(5 * (size_t)e->n_rep);
-  HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
+  HIP_TRY(hipMemcpyAsync(out, e->d_stats, sizeof(double) * 4 * e->n_rep,
                          hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
-  for (int r = 0; r < e->n_rep; ++r)
-    for (int c = 0; c < 4; ++c) out[4 * r + c] = st[5 * r + c];
   return check_errors(e);
 }
 
@@ -1023,18 +997,18 @@ extern "C" sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list,
     *n_list = e->n_list;
   }
   if (n_in_rc) {
-    *n_in_rc = -1;
-    if (e->stats_valid) {
-      sre_status rs = reduce_stats(e);
-      if (rs != SRE_OK) return rs;
-      std::vector<double> st(5 * (size_t)e->n_rep);
-      HIP_TRY(hipMemcpyAsync(st.data(), e->d_stats, sizeof(double) * st.size(),
-                             hipMemcpyDeviceToHost, e->stream));
-      HIP_TRY(hipStreamSynchronize(e->stream));
-      double t = 0;
-      for (int r = 0; r < e->n_rep; ++r) t += st[5 * r + 4];
-      *n_in_rc = (int64_t)llround(t);
-    }
+    HIP_TRY(hipMemsetAsync(e->d_nin, 0, sizeof(unsigned long long) * e->n_rep,
+                           e->stream));
+    launch_count_in_rc(e->n_atoms, e->list_lanes, e->cap, e->d_pos4, e->d_rcut,
+                       e->d_nnb, e->d_list, e->d_reps, e->d_nin, e->stream);
+    HIP_TRY(hipGetLastError());
+    std::vector<unsigned long long> c(e->n_rep);
+    HIP_TRY(hipMemcpyAsync(c.data(), e->d_nin, sizeof(unsigned long long) * e->n_rep,
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    int64_t t = 0;
+    for (unsigned long long v : c) t += (int64_t)v;
+    *n_in_rc = t;
   }
   if (max_list) *max_list = e->max_count;
   return SRE_OK;
@@ -1390,10 +1364,6 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
   if (!e || n_steps < 0) return fail(SRE_ERR_INVALID, "bad arguments");
   if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
   HIP_TRY(hipSetDevice(e->device));
-  if (flags & SRE_STATS) {
-    sre_status sw = ensure_wave_part(e);  // before capture
-    if (sw != SRE_OK) return sw;
-  }
   refresh_tail_values(e);
   if ((flags & SRE_PERTURB) && e->pert_anchored &&
       (e->anchor_stride != e->pos_stride ||
@@ -1437,7 +1407,6 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
     e->forces_valid = true;
     e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
     e->stats_pending = e->stats_valid;
-    e->stats_lanes = e->list_lanes;
   }
   return SRE_OK;
 }

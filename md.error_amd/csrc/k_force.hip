@@ -152,35 +152,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Writes the two replica slots of one wave's statistic partials.
-// vals[5] are this lane's group totals (only lanes with l == 0 contribute).
-__device__ __forceinline__ void write_wave_stats(double *wp, int wave,
-                                                 int rep, int rep0, bool own,
-                                                 const double vals[5]) {
-  double out0[5], out1[5];
-  const bool in0 = own && rep == rep0, in1 = own && rep != rep0;
-  int r1 = in1 ? rep : -1;
-  const bool any1 = __ballot(in1) != 0;  // wave-uniform: a replica starts here
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    out0[c] = wave_sum(in0 ? vals[c] : 0.0);
-    out1[c] = any1 ? wave_sum(in1 ? vals[c] : 0.0) : 0.0;
-  }
-  if (any1) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) r1 = max(r1, __shfl_xor(r1, o, kWave));
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    double *d = wp + (size_t)wave * 12;
-    d[0] = (double)rep0;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) d[1 + c] = out0[c];
-    d[6] = (double)r1;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) d[7 + c] = out1[c];
-  }
-}
-
 // -------------------------------------------------------------- mixed ----
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
@@ -246,7 +217,6 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
 
   uint64_t ax = 0, ay = 0, az = 0;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
-  int nin = 0;
   const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
   int jn[U];
 #pragma unroll
@@ -287,7 +257,6 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
         wx = __builtin_fmaf(fpx, dx, wx);
         wy = __builtin_fmaf(fpy, dy, wy);
         wz = __builtin_fmaf(fpz, dz, wz);
-        nin += in ? 1 : 0;
       }
     }
   }
@@ -300,13 +269,12 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // |pair component| * 2^32 must stay below 2^51 (exact rounding by the magic add)
   if (__ballot(act && fsmax * rc >= 2251799813685248.0f) && lane == 0)
     atomicOr(a.err_flag, 1u);
-  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0, Nin = 0.0;
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {
     E = group_sum((double)e, T) * 0.5;
     Wx = group_sum((double)wx, T) * (0.5 * kFixInv);
     Wy = group_sum((double)wy, T) * (0.5 * kFixInv);
     Wz = group_sum((double)wz, T) * (0.5 * kFixInv);
-    Nin = (double)group_sum(nin, T);
   }
   const bool own = act && l == 0;
   if (own) {
@@ -324,15 +292,11 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
     a.fx[s] = Fx;
     a.fy[s] = Fy;
     a.fz[s] = Fz;
-    if (STATS && a.atom_stats) {
-      double *d = a.atom_stats + (size_t)s * 4;
-      d[0] = E; d[1] = Wx; d[2] = Wy; d[3] = Wz;
+    if (STATS) {
+      double2 *d = reinterpret_cast<double2 *>(a.atom_stats + (size_t)s * 4);
+      d[0] = make_double2(E, Wx);
+      d[1] = make_double2(Wy, Wz);
     }
-  }
-  if (STATS) {
-    const int rep0 = (int)(a.pos4[s0].w >> 8);
-    const double vals[5] = {E, Wx, Wy, Wz, Nin};
-    write_wave_stats(a.wave_part, wave, rep, rep0, own, vals);
   }
 }
 
@@ -358,11 +322,9 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   if (s0 >= a.n) return;
 
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
-  int nin = 0, rep = 0;
   if (act) {
     const uint4 pi = a.pos4[s];
-    rep = (int)(pi.w >> 8);
-    const RepInfo &R = a.reps[rep];
+    const RepInfo &R = a.reps[pi.w >> 8];
     const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
     const double qdx = R.qd[0], qdy = R.qd[1], qdz = R.qd[2];
     const float4 ri = a.res4[s];
@@ -399,20 +361,18 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
         wx += fs * dx * dx;
         wy += fs * dy * dy;
         wz += fs * dz * dz;
-        nin += in ? 1 : 0;
       }
     }
   }
   fx = group_sum(fx, T);
   fy = group_sum(fy, T);
   fz = group_sum(fz, T);
-  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0, Nin = 0.0;
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {
     E = group_sum(e, T) * 0.5;
     Wx = group_sum(wx, T) * 0.5;
     Wy = group_sum(wy, T) * 0.5;
     Wz = group_sum(wz, T) * 0.5;
-    Nin = (double)group_sum(nin, T);
   }
   const bool own = act && l == 0;
   if (own) {
@@ -430,15 +390,11 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
     a.fxd[s] = fx;
     a.fyd[s] = fy;
     a.fzd[s] = fz;
-    if (STATS && a.atom_stats) {
-      double *d = a.atom_stats + (size_t)s * 4;
-      d[0] = E; d[1] = Wx; d[2] = Wy; d[3] = Wz;
+    if (STATS) {
+      double2 *d = reinterpret_cast<double2 *>(a.atom_stats + (size_t)s * 4);
+      d[0] = make_double2(E, Wx);
+      d[1] = make_double2(Wy, Wz);
     }
-  }
-  if (STATS) {
-    const int rep0 = (int)(a.pos4[s0].w >> 8);
-    const double vals[5] = {E, Wx, Wy, Wz, Nin};
-    write_wave_stats(a.wave_part, wave, rep, rep0, own, vals);
   }
 }
 
@@ -496,47 +452,88 @@ void launch_force(int precision, int lanes, int block, bool stats, bool tail,
 
 // ------------------------------------------------------ statistic reduce --
 
-// One block per replica sums the wave partials in a fixed order (thread k
-// takes rows k, k + 1024, ...; then a fixed shuffle and LDS tree).  A row is
-// 12 doubles: (replica, E, Wxx, Wyy, Wzz, n_in) for each of two slots.
+// One block per replica sums the per-atom (E, Wxx, Wyy, Wzz) of its atoms
+// (contiguous in sorted order) in a fixed order: thread k takes atoms
+// k, k + 1024, ..., then a fixed shuffle tree and LDS pass.
 __global__ __launch_bounds__(1024) void k_stats_reduce(const RepInfo *reps,
-                                                       int A,
-                                                       const double *wave_part,
+                                                       const double *atom_stats,
                                                        double *stats) {
-  __shared__ double red[1024 / kWave][5];
+  __shared__ double red[1024 / kWave][4];
   const int r = blockIdx.x;
   const RepInfo &R = reps[r];
-  const int w_lo = R.atom_off / A;
-  const int w_hi = (R.atom_off + R.n_atoms - 1) / A;
-  double acc[5] = {0, 0, 0, 0, 0};
-  const double rr = (double)r;
-  for (int w = w_lo + (int)threadIdx.x; w <= w_hi; w += blockDim.x) {
-    const double2 *d = reinterpret_cast<const double2 *>(wave_part + (size_t)w * 12);
-    const double2 v0 = d[0], v1 = d[1], v2 = d[2], v3 = d[3], v4 = d[4], v5 = d[5];
-    if (v0.x == rr) {
-      acc[0] += v0.y; acc[1] += v1.x; acc[2] += v1.y; acc[3] += v2.x; acc[4] += v2.y;
-    }
-    if (v3.x == rr) {
-      acc[0] += v3.y; acc[1] += v4.x; acc[2] += v4.y; acc[3] += v5.x; acc[4] += v5.y;
-    }
+  double acc[4] = {0, 0, 0, 0};
+  for (int i = (int)threadIdx.x; i < R.n_atoms; i += blockDim.x) {
+    const double2 *d = reinterpret_cast<const double2 *>(
+        atom_stats + (size_t)(R.atom_off + i) * 4);
+    const double2 v0 = d[0], v1 = d[1];
+    acc[0] += v0.x; acc[1] += v0.y; acc[2] += v1.x; acc[3] += v1.y;
   }
-  for (int c = 0; c < 5; ++c) acc[c] = wave_sum(acc[c]);
+  for (int c = 0; c < 4; ++c) acc[c] = wave_sum(acc[c]);
   const int wv = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0)
-    for (int c = 0; c < 5; ++c) red[wv][c] = acc[c];
+    for (int c = 0; c < 4; ++c) red[wv][c] = acc[c];
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
-      for (int c = 0; c < 5; ++c) acc[c] += red[k][c];
-    for (int c = 0; c < 5; ++c) stats[(size_t)r * 5 + c] = acc[c];
+      for (int c = 0; c < 4; ++c) acc[c] += red[k][c];
+    for (int c = 0; c < 4; ++c) stats[(size_t)r * 4 + c] = acc[c];
   }
 }
 
-void launch_stats_reduce(int n_rep, const RepInfo *reps, int lanes,
-                         const double *wave_part, double *stats,
+void launch_stats_reduce(int n_rep, const RepInfo *reps,
+                         const double *atom_stats, double *stats,
                          hipStream_t st) {
   hipLaunchKernelGGL(k_stats_reduce, dim3(n_rep), dim3(1024), 0, st, reps,
-                     kWave / lanes, wave_part, stats);
+                     atom_stats, stats);
+}
+
+// Pairs of the list inside the cutoff (d^2 < rc_i^2, the force kernel's
+// mask), counted per atom and summed per replica with integer atomics (one
+// per wave and replica run, order independent).  On demand only.
+__global__ __launch_bounds__(256) void k_count_in_rc(
+    int n, int T, int cap, const uint4 *__restrict__ pos4,
+    const float *__restrict__ rcut, const int *__restrict__ nnb,
+    const int *__restrict__ list, const RepInfo *__restrict__ reps,
+    unsigned long long *counts) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = 0, rep = -1;
+  if (s < n) {
+    const uint4 pi = pos4[s];
+    rep = (int)(pi.w >> 8);
+    const RepInfo &R = reps[rep];
+    const float rc = rcut[s];
+    const float rc2 = rc * rc;
+    const int A = kWave / T, lt = __builtin_ctz((unsigned)T);
+    const size_t base = (size_t)(s / A) * (size_t)cap * A + (size_t)((s % A) * T);
+    const int nn = nnb[s];
+    for (int k = 0; k < nn; ++k) {
+      const uint4 pj = pos4[list[base + ((size_t)(k >> lt) << 6) + (k & (T - 1))]];
+      const float d2 = dist2(disp(pj.x, pi.x, R.q[0]), disp(pj.y, pi.y, R.q[1]),
+                             disp(pj.z, pi.z, R.q[2]));
+      c += d2 < rc2 ? 1 : 0;
+    }
+  }
+  // a wave holds at most a few replicas: one atomic per distinct replica
+  for (;;) {
+    int r0 = rep < 0 ? 0x7fffffff : rep;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) r0 = min(r0, __shfl_xor(r0, o, kWave));
+    if (r0 == 0x7fffffff) break;  // wave-uniform
+    unsigned long long v = (rep == r0) ? (unsigned long long)c : 0ull;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&counts[r0], v);
+    if (rep == r0) rep = -1;
+  }
+}
+
+void launch_count_in_rc(int n, int lanes, int cap, const uint4 *pos4,
+                        const float *rcut, const int *nnb, const int *list,
+                        const RepInfo *reps, unsigned long long *counts,
+                        hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_count_in_rc, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                     lanes, cap, pos4, rcut, nnb, list, reps, counts);
 }
 
 // ----------------------------------------------------------------- export --

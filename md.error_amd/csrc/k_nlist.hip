@@ -81,11 +81,20 @@ __global__ __launch_bounds__(256) void k_build_list(
   const int ncol = act ? (ox1 - ox0 + 1) * noy : 0;
   const int ncol_w = __builtin_amdgcn_readfirstlane(wave_max_i(ncol));
 
+  // list slots of this atom in the force kernel's layout (T = TF lanes):
+  // slot(k) = sbase + (k / TF) * 64 + k % TF, TF a power of two
+  const int AF = kWave / TF;
+  const int lt = __builtin_ctz((unsigned)TF);
+  const size_t sbase =
+      (size_t)(si / AF) * (size_t)cap * (size_t)AF + (size_t)((si % AF) * TF);
+  int *const sp0 = s_p0[gb];
+  int *const send = s_end[gb];
   int count = 0;
   for (int cbase = 0; cbase < ncol_w; cbase += CC) {
-    // runs of this lane's kBuildCols columns
-    int p0[2 * kBuildCols], len[2 * kBuildCols];
-#pragma unroll
+    // runs of this lane's kBuildCols columns -> (first atom, length) in LDS
+    const int slot0 = l * 2 * kBuildCols;
+    int tot = 0;
+#pragma unroll 2
     for (int m = 0; m < kBuildCols; ++m) {
       const int c = cbase + l * kBuildCols + m;
       int ra0 = 0, rb0 = -1, ra1 = 0, rb1 = -1, colbase = 0;
@@ -128,15 +137,15 @@ __global__ __launch_bounds__(256) void k_build_list(
       const int e1 = rb1 >= ra1 ? start[colbase + rb1 + 1] : 0;
       // runs lie inside [0, n) by construction; the clamps keep every gather
       // in bounds whatever the cell offsets hold
-      p0[2 * m] = min(max(b0, 0), n);
-      len[2 * m] = max(min(e0, n) - p0[2 * m], 0);
-      p0[2 * m + 1] = min(max(b1, 0), n);
-      len[2 * m + 1] = max(min(e1, n) - p0[2 * m + 1], 0);
+      const int q0 = min(max(b0, 0), n), q1 = min(max(b1, 0), n);
+      const int n0 = max(min(e0, n) - q0, 0), n1 = max(min(e1, n) - q1, 0);
+      sp0[slot0 + 2 * m] = q0;
+      send[slot0 + 2 * m] = n0;       // length for now, prefix below
+      sp0[slot0 + 2 * m + 1] = q1;
+      send[slot0 + 2 * m + 1] = n1;
+      tot += n0 + n1;
     }
     // flat sequence: prefix over the group (lane-major slot order)
-    int tot = 0;
-#pragma unroll
-    for (int r = 0; r < 2 * kBuildCols; ++r) tot += len[r];
     int incl = tot;
 #pragma unroll
     for (int o = 1; o < TB; o <<= 1) {
@@ -145,16 +154,14 @@ __global__ __launch_bounds__(256) void k_build_list(
     }
     const int T = __shfl(incl, TB - 1, TB);
     int run = incl - tot;
-#pragma unroll
     for (int r = 0; r < 2 * kBuildCols; ++r) {
-      s_p0[gb][l * 2 * kBuildCols + r] = p0[r];
-      run += len[r];
-      s_end[gb][l * 2 * kBuildCols + r] = run;
+      run += send[slot0 + r];
+      send[slot0 + r] = run;
     }
     __builtin_amdgcn_wave_barrier();
     // stream the flat sequence: lane l takes v = l, l + TB, ...
     const int rounds = __builtin_amdgcn_readfirstlane(wave_max_i((T + TB * U - 1) / (TB * U)));
-    int cur = 0, beg = 0, end = s_end[gb][0];
+    int cur = 0, beg = 0, end = send[0];
     for (int k0 = 0; k0 < rounds; ++k0) {
       uint4 cj[U];
       int jj[U];
@@ -166,9 +173,9 @@ __global__ __launch_bounds__(256) void k_build_list(
           while (v >= end) {
             ++cur;
             beg = end;
-            end = s_end[gb][cur];
+            end = send[cur];
           }
-          j = s_p0[gb][cur] + (v - beg);
+          j = sp0[cur] + (v - beg);
         }
         jj[u] = j;
         cj[u] = pos4[j >= 0 ? j : si];
@@ -176,18 +183,13 @@ __global__ __launch_bounds__(256) void k_build_list(
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = jj[u];
-        bool ok = false;
-        if (j >= 0 && j != s) {
-          const float dx = disp(cj[u].x, pi.x, qx);
-          const float dy = disp(cj[u].y, pi.y, qy);
-          const float dz2 = disp(cj[u].z, pi.z, qz);
-          ok = dist2(dx, dy, dz2) < rl2;
-        }
+        const float dx = disp(cj[u].x, pi.x, qx);
+        const float dy = disp(cj[u].y, pi.y, qy);
+        const float dz2 = disp(cj[u].z, pi.z, qz);
+        const bool ok = (j >= 0) & (j != s) & (dist2(dx, dy, dz2) < rl2);
         const uint64_t gbits = (__ballot(ok) >> shift) & gmask;
-        if (ok) {
-          const int k = count + __popcll(gbits & below);
-          if (k < cap) list[list_slot(s, k, TF, cap)] = j;
-        }
+        const int k = count + __popcll(gbits & below);
+        if (ok && k < cap) list[sbase + ((size_t)(k >> lt) << 6) + (k & (TF - 1))] = j;
         count += __popcll(gbits);
       }
     }

@@ -244,7 +244,6 @@ struct ForceArgs {
   const double2 *tail_atom;    // exact mode: per sorted atom (F_z, E) or NULL
   int64_t *fx, *fy, *fz;       // fixed-point output (sorted order)
   double *fxd, *fyd, *fzd;     // FP64 output (sorted order)
-  double *wave_part;           // per wave: 2 slots x 5 (E, Wxx, Wyy, Wzz, n_in)
   double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
   unsigned *err_flag;
   uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
@@ -253,9 +252,16 @@ struct ForceArgs {
 void launch_force(int precision, int lanes, int block, bool stats, bool tail,
                   const ForceArgs &a, hipStream_t st);
 int force_num_waves(int n, int lanes);
-void launch_stats_reduce(int n_rep, const RepInfo *reps, int lanes,
-                         const double *wave_part, double *stats,
+// stats[4 r + c] = sum over replica r's atoms of atom_stats[4 s + c]
+void launch_stats_reduce(int n_rep, const RepInfo *reps,
+                         const double *atom_stats, double *stats,
                          hipStream_t st);
+// counts[r] += pairs of replica r's lists with d^2 < rc_i^2 (counts zeroed
+// by the caller)
+void launch_count_in_rc(int n, int lanes, int cap, const uint4 *pos4,
+                        const float *rcut, const int *nnb, const int *list,
+                        const RepInfo *reps, unsigned long long *counts,
+                        hipStream_t st);
 void launch_export_forces(int n, const uint32_t *perm, const int64_t *fx,
                           const int64_t *fy, const int64_t *fz,
                           const double *fxd, const double *fyd,
