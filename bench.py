@@ -58,7 +58,7 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
         done = 0
         pairs = 0
         rebuilds = 0
-        nl, _, _ = e.neighbor_count()         # known since the last rebuild
+        nl, _, _ = e.neighbor_count(in_rc=False)   # known since the last rebuild
         while done < n_steps:
             n = min(every, n_steps - done)
             e.graph_run(n, stats=True, tail=True, perturb=True)
@@ -73,7 +73,7 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
             disp = e.last_rebuild_displacement()
             if disp > 0.5 * w.skin:
                 raise RuntimeError(f"skin exceeded between rebuilds ({disp:.4f})")
-            nl, _, _ = e.neighbor_count()
+            nl, _, _ = e.neighbor_count(in_rc=False)
         return pairs, rebuilds
 
     segment_loop(warmup)

@@ -65,7 +65,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 8, 256, 0.5f, 0.25f, 0, 1};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1};
 
   // system
   int n_rep = 0, n_atoms = 0;
@@ -141,6 +141,7 @@ struct sre_engine {
   int64_t *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;
   double *d_fxd = nullptr, *d_fyd = nullptr, *d_fzd = nullptr;
   double *d_stats = nullptr;       // per replica (E, Wxx, Wyy, Wzz)
+  double *d_stat_part = nullptr;   // per replica and chunk, 4 partial sums
   unsigned long long *d_nin = nullptr;  // per replica pairs inside rc
   double *d_atom_stats = nullptr;
   double *d_tmp_out = nullptr;
@@ -184,7 +185,7 @@ sre_engine::~sre_engine() {
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
-  dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats);
+  dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -212,6 +213,9 @@ extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
   e->n_types = opt->n_types;
   e->skin = opt->skin;
   e->precision = opt->precision;
+  // measured best lanes per atom of the force kernel on the 64k slab: the
+  // FP64 form's longer pair arithmetic prefers more lanes
+  if (e->precision == SRE_PREC_FP64) e->tune.force_lanes = 8;
   if (opt->device >= 0) {
     hipError_t r = hipSetDevice(opt->device);
     if (r != hipSuccess) {
@@ -312,6 +316,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
   dfree(e->d_fxd); dfree(e->d_fyd); dfree(e->d_fzd); dfree(e->d_atom_stats);
   dfree(e->d_stats); dfree(e->d_nin); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
+  dfree(e->d_stat_part);
   dfree(e->d_skin_part);
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
@@ -336,6 +341,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->d_atom_stats, (size_t)4 * N));
   HIP_TRY(dalloc(&e->d_stats, (size_t)4 * n_replicas));
   HIP_TRY(dalloc(&e->d_nin, n_replicas));
+  HIP_TRY(dalloc(&e->d_stat_part, (size_t)4 * kStatBlocks * n_replicas));
   HIP_TRY(dalloc(&e->d_tmp_out, (size_t)4 * N));
   HIP_TRY(dalloc(&e->sb.key_a, N));
   HIP_TRY(dalloc(&e->sb.val_a, N));
@@ -859,8 +865,8 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
 // d_stats (stream-ordered; fixed order, so the result is reproducible).
 static sre_status reduce_stats(sre_engine *e) {
   if (e->stats_pending) {
-    launch_stats_reduce(e->n_rep, e->d_reps, e->d_atom_stats, e->d_stats,
-                        e->stream);
+    launch_stats_reduce(e->n_rep, e->d_reps, e->d_atom_stats, e->d_stat_part,
+                        e->d_stats, e->stream);
     HIP_TRY(hipGetLastError());
     e->stats_pending = false;
   }

@@ -452,17 +452,21 @@ void launch_force(int precision, int lanes, int block, bool stats, bool tail,
 
 // ------------------------------------------------------ statistic reduce --
 
-// One block per replica sums the per-atom (E, Wxx, Wyy, Wzz) of its atoms
-// (contiguous in sorted order) in a fixed order: thread k takes atoms
-// k, k + 1024, ..., then a fixed shuffle tree and LDS pass.
-__global__ __launch_bounds__(1024) void k_stats_reduce(const RepInfo *reps,
+// Replica sums of the per-atom (E, Wxx, Wyy, Wzz) in a fixed order: block
+// (b, r) sums the b-th of kStatBlocks contiguous chunks of replica r's atoms
+// (contiguous in sorted order; thread k takes atoms k, k + 256, ... of the
+// chunk, then a fixed shuffle tree and LDS pass); k_stats_final adds the
+// chunks in order.
+__global__ __launch_bounds__(256) void k_stats_partial(const RepInfo *reps,
                                                        const double *atom_stats,
-                                                       double *stats) {
-  __shared__ double red[1024 / kWave][4];
-  const int r = blockIdx.x;
+                                                       double *part) {
+  __shared__ double red[256 / kWave][4];
+  const int b = blockIdx.x, r = blockIdx.y;
   const RepInfo &R = reps[r];
+  const int chunk = (R.n_atoms + kStatBlocks - 1) / kStatBlocks;
+  const int i0 = b * chunk, i1 = min(R.n_atoms, i0 + chunk);
   double acc[4] = {0, 0, 0, 0};
-  for (int i = (int)threadIdx.x; i < R.n_atoms; i += blockDim.x) {
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
     const double2 *d = reinterpret_cast<const double2 *>(
         atom_stats + (size_t)(R.atom_off + i) * 4);
     const double2 v0 = d[0], v1 = d[1];
@@ -476,15 +480,27 @@ __global__ __launch_bounds__(1024) void k_stats_reduce(const RepInfo *reps,
   if (threadIdx.x == 0) {
     for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
       for (int c = 0; c < 4; ++c) acc[c] += red[k][c];
-    for (int c = 0; c < 4; ++c) stats[(size_t)r * 4 + c] = acc[c];
+    for (int c = 0; c < 4; ++c) part[((size_t)r * kStatBlocks + b) * 4 + c] = acc[c];
   }
 }
 
+__global__ __launch_bounds__(256) void k_stats_final(int n_rep, const double *part,
+                                                     double *stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // (replica, component)
+  if (t >= 4 * n_rep) return;
+  const int r = t / 4, c = t % 4;
+  double acc = 0.0;
+  for (int b = 0; b < kStatBlocks; ++b) acc += part[((size_t)r * kStatBlocks + b) * 4 + c];
+  stats[t] = acc;
+}
+
 void launch_stats_reduce(int n_rep, const RepInfo *reps,
-                         const double *atom_stats, double *stats,
+                         const double *atom_stats, double *part, double *stats,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_stats_reduce, dim3(n_rep), dim3(1024), 0, st, reps,
-                     atom_stats, stats);
+  hipLaunchKernelGGL(k_stats_partial, dim3(kStatBlocks, n_rep), dim3(256), 0, st,
+                     reps, atom_stats, part);
+  hipLaunchKernelGGL(k_stats_final, dim3((4 * n_rep + 255) / 256), dim3(256), 0,
+                     st, n_rep, part, stats);
 }
 
 // Pairs of the list inside the cutoff (d^2 < rc_i^2, the force kernel's

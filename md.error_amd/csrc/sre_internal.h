@@ -252,9 +252,11 @@ struct ForceArgs {
 void launch_force(int precision, int lanes, int block, bool stats, bool tail,
                   const ForceArgs &a, hipStream_t st);
 int force_num_waves(int n, int lanes);
-// stats[4 r + c] = sum over replica r's atoms of atom_stats[4 s + c]
+// stats[4 r + c] = sum over replica r's atoms of atom_stats[4 s + c];
+// part holds n_rep * kStatBlocks * 4 doubles of chunk partials
+constexpr int kStatBlocks = 32;
 void launch_stats_reduce(int n_rep, const RepInfo *reps,
-                         const double *atom_stats, double *stats,
+                         const double *atom_stats, double *part, double *stats,
                          hipStream_t st);
 // counts[r] += pairs of replica r's lists with d^2 < rc_i^2 (counts zeroed
 // by the caller)

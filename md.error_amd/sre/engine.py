@@ -221,9 +221,13 @@ class Engine:
         check(lib().sre_get_atom_stats(self._h, _ptr(out, C.c_double)))
         return out
 
-    def neighbor_count(self):
-        nl, ni, mx = C.c_int64(), C.c_int64(), C.c_int32()
-        check(lib().sre_neighbor_count(self._h, C.byref(nl), C.byref(ni), C.byref(mx)))
+    def neighbor_count(self, in_rc=True):
+        """(list entries, pairs inside rc at the current positions, longest
+        list).  The in-rc count runs a counting kernel; in_rc=False skips it
+        (returns -1 there)."""
+        nl, ni, mx = C.c_int64(), C.c_int64(-1), C.c_int32()
+        check(lib().sre_neighbor_count(self._h, C.byref(nl),
+                                       C.byref(ni) if in_rc else None, C.byref(mx)))
         return nl.value, ni.value, mx.value
 
     def export_pairs(self):

@@ -187,7 +187,7 @@ class NeighborList:
         e.rebuild()
 
     def calSumNeighbor(self):
-        return self.sys.engine.neighbor_count()[0]
+        return self.sys.engine.neighbor_count(in_rc=False)[0]
 
 
 class MDStatistic:
