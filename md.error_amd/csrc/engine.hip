@@ -65,7 +65,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 4};
 
   // system
   int n_rep = 0, n_atoms = 0;
@@ -855,7 +855,8 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
   a.step_ctr = step_ctr;
-  launch_force(e->precision, T, e->tune.block, stats, tail, a, e->stream);
+  launch_force(e->precision, T, e->tune.force_unroll, e->tune.block, stats, tail,
+               a, e->stream);
   if (stats) e->stats_pending = true;
   HIP_TRY(hipGetLastError());  // launch-configuration errors
   return SRE_OK;
@@ -1187,6 +1188,8 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "build_mode must be 0 or 1");
   if (t->xcd_remap != 0 && t->xcd_remap != 1)
     return fail(SRE_ERR_INVALID, "xcd_remap must be 0 or 1");
+  if (t->force_unroll != 1 && t->force_unroll != 2 && t->force_unroll != 4)
+    return fail(SRE_ERR_INVALID, "force_unroll must be 1, 2 or 4");
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
@@ -1278,9 +1281,9 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
         *bar = 0;
         if (strcmp(line, sig) == 0) {
           sre_tuning t;
-          if (sscanf(bar + 1, "%d %d %d %f %f %d %d", &t.force_lanes,
+          if (sscanf(bar + 1, "%d %d %d %f %f %d %d %d", &t.force_lanes,
                      &t.build_lanes, &t.block, &t.cell_xy, &t.cell_z,
-                     &t.build_mode, &t.xcd_remap) == 7) {
+                     &t.build_mode, &t.xcd_remap, &t.force_unroll) == 8) {
             fclose(f);
             sre_status s = sre_set_tuning(e, &t);
             if (s != SRE_OK) return s;
@@ -1321,12 +1324,16 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   double best_f = 1e30;
   sre_tuning bestf = best;
   std::vector<int64_t> ref, cur;
+  const int unrolls[] = {1, 2, 4};
   for (int fl : force_lanes)
     for (int bk : blocks)
+      for (int fu : unrolls)
       for (int xr = 0; xr < 2; ++xr) {
+      if (e->precision == SRE_PREC_FP64 && fu != 4) continue;  // unused there
       sre_tuning t = best;
       t.force_lanes = fl;
       t.block = bk;
+      t.force_unroll = fu;
       t.xcd_remap = xr;
       sre_status s = sre_set_tuning(e, &t);
       if (s != SRE_OK) return s;
@@ -1355,9 +1362,10 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   if (cache_path) {
     FILE *f = fopen(cache_path, "a");
     if (f) {
-      fprintf(f, "%s|%d %d %d %.4f %.4f %d %d\n", sig, bestf.force_lanes,
+      fprintf(f, "%s|%d %d %d %.4f %.4f %d %d %d\n", sig, bestf.force_lanes,
               bestf.build_lanes, bestf.block, (double)bestf.cell_xy,
-              (double)bestf.cell_z, bestf.build_mode, bestf.xcd_remap);
+              (double)bestf.cell_z, bestf.build_mode, bestf.xcd_remap,
+              bestf.force_unroll);
       fclose(f);
     }
   }

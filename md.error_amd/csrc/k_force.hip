@@ -181,9 +181,8 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
 // pair component c is already c*2^32 in FP32 and its fixed-point value is the
 // bit pattern of double(c*2^32) + 1.5*2^52.
-template <int T, bool STATS, bool TAIL, bool MULTI>
+template <int T, int U, bool STATS, bool TAIL, bool MULTI>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
-  constexpr int U = kForceUnroll;
   // the position refresh of this step has read the counter (stream order)
   if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
   __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
@@ -403,19 +402,27 @@ int force_num_waves(int n, int lanes) {
   return (n + A - 1) / A;
 }
 
-template <int T, bool STATS, bool TAIL>
-static void launch_force_t(int precision, int block, const ForceArgs &a,
+template <int T, int U, bool STATS, bool TAIL>
+static void launch_mixed_u(int grid, int block, bool multi, const ForceArgs &a,
                            hipStream_t st) {
+  if (multi)
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, true>), dim3(grid),
+                       dim3(block), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, false>), dim3(grid),
+                       dim3(block), 0, st, a);
+}
+
+template <int T, bool STATS, bool TAIL>
+static void launch_force_t(int precision, int unroll, int block,
+                           const ForceArgs &a, hipStream_t st) {
   const int waves = force_num_waves(a.n, T);
   const int grid = (waves * kWave + block - 1) / block;
   const bool multi = a.n_types > 1;
   if (precision == 0) {
-    if (multi)
-      hipLaunchKernelGGL((k_force_mixed<T, STATS, TAIL, true>), dim3(grid),
-                         dim3(block), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_force_mixed<T, STATS, TAIL, false>), dim3(grid),
-                         dim3(block), 0, st, a);
+    if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
+    else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);
+    else launch_mixed_u<T, 4, STATS, TAIL>(grid, block, multi, a, st);
   } else {
     if (multi)
       hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, true>), dim3(grid),
@@ -427,26 +434,26 @@ static void launch_force_t(int precision, int block, const ForceArgs &a,
 }
 
 template <int T>
-static void launch_force_lanes(int precision, int block, bool stats, bool tail,
-                               const ForceArgs &a, hipStream_t st) {
+static void launch_force_lanes(int precision, int unroll, int block, bool stats,
+                               bool tail, const ForceArgs &a, hipStream_t st) {
   if (stats) {
-    if (tail) launch_force_t<T, true, true>(precision, block, a, st);
-    else launch_force_t<T, true, false>(precision, block, a, st);
+    if (tail) launch_force_t<T, true, true>(precision, unroll, block, a, st);
+    else launch_force_t<T, true, false>(precision, unroll, block, a, st);
   } else {
-    if (tail) launch_force_t<T, false, true>(precision, block, a, st);
-    else launch_force_t<T, false, false>(precision, block, a, st);
+    if (tail) launch_force_t<T, false, true>(precision, unroll, block, a, st);
+    else launch_force_t<T, false, false>(precision, unroll, block, a, st);
   }
 }
 
-void launch_force(int precision, int lanes, int block, bool stats, bool tail,
-                  const ForceArgs &a, hipStream_t st) {
+void launch_force(int precision, int lanes, int unroll, int block, bool stats,
+                  bool tail, const ForceArgs &a, hipStream_t st) {
   if (a.n <= 0) return;
   switch (lanes) {
-    case 1: launch_force_lanes<1>(precision, block, stats, tail, a, st); break;
-    case 2: launch_force_lanes<2>(precision, block, stats, tail, a, st); break;
-    case 4: launch_force_lanes<4>(precision, block, stats, tail, a, st); break;
-    case 8: launch_force_lanes<8>(precision, block, stats, tail, a, st); break;
-    default: launch_force_lanes<16>(precision, block, stats, tail, a, st); break;
+    case 1: launch_force_lanes<1>(precision, unroll, block, stats, tail, a, st); break;
+    case 2: launch_force_lanes<2>(precision, unroll, block, stats, tail, a, st); break;
+    case 4: launch_force_lanes<4>(precision, unroll, block, stats, tail, a, st); break;
+    case 8: launch_force_lanes<8>(precision, unroll, block, stats, tail, a, st); break;
+    default: launch_force_lanes<16>(precision, unroll, block, stats, tail, a, st); break;
   }
 }
 

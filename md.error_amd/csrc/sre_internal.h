@@ -31,7 +31,7 @@ constexpr double kFixScale = 4294967296.0;          // 2^32
 constexpr double kFixInv = 1.0 / 4294967296.0;
 constexpr double kMagic = 6755399441055744.0;       // 1.5 * 2^52
 constexpr float kPairForceLimit = 524288.0f;        // 2^19: |pair component| limit of the magic rounding
-constexpr int kForceUnroll = 4;   // list rows per group in the mixed force kernel
+constexpr int kForceUnroll = 4;   // most list rows per group in the mixed force kernel
 // list rows allocated past the last tile: the force kernel prefetches up to
 // 2 * kForceUnroll rows beyond a tile's end
 constexpr int kListPadRows = 2 * kForceUnroll + 2;
@@ -249,7 +249,9 @@ struct ForceArgs {
   uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
 };
-void launch_force(int precision, int lanes, int block, bool stats, bool tail,
+// unroll: list rows per group of the mixed kernel (1, 2 or 4)
+void launch_force(int precision, int lanes, int unroll, int block, bool stats,
+                  bool tail,
                   const ForceArgs &a, hipStream_t st);
 int force_num_waves(int n, int lanes);
 // stats[4 r + c] = sum over replica r's atoms of atom_stats[4 s + c];
