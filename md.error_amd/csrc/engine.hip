@@ -65,7 +65,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 4};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2};
 
   // system
   int n_rep = 0, n_atoms = 0;
