@@ -173,11 +173,11 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 // The loop runs a wave-uniform number of groups of U list rows (the wave's
 // longest list / (T U)); a lane whose entry k is past its atom's list
-// evaluates its own atom (d = 0) and masks the result.  Each group first
-// prefetches the next group's U list rows, then issues its U position gathers
-// together, then evaluates the U pairs, so one memory latency is exposed per
-// U pairs per wave.  Rows past the end of a wave's tile read the next tile or
-// the list's tail padding; those values are never used.
+// evaluates its own atom (d = 0) and masks the result.  The loop is software
+// pipelined: while group g is evaluated, the U position gathers of group g+1
+// and the U list rows of group g+2 are in flight, so the gather latency hides
+// behind a group's arithmetic.  Rows past the end of a wave's tile (up to 2U)
+// read the next tile or the list's tail padding; those values are never used.
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
 // pair component c is already c*2^32 in FP32 and its fixed-point value is the
 // bit pattern of double(c*2^32) + 1.5*2^52.
@@ -217,21 +217,33 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   uint64_t ax = 0, ay = 0, az = 0;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
   const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
-  int jn[U];
+  // software pipeline: while group g is evaluated, the gathers of group g+1
+  // and the list rows of group g+2 are in flight
+  int jn[U];    // list rows of the next group
+  uint4 pn[U];  // positions of the current group, gathered one group ahead
+  {
+    int j0[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)u * kWave];
+    for (int u = 0; u < U; ++u) j0[u] = lp[(size_t)u * kWave];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)(U + u) * kWave];
+#pragma unroll
+    for (int u = 0; u < U; ++u) pn[u] = a.pos4[(u * T + l < nn) ? j0[u] : si];
+  }
   for (int g = 0; g < groups; ++g) {
-    int jc[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = (g * U + u) * T + l;
-      jc[u] = (k < nn) ? jn[u] : si;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)((g + 1) * U + u) * kWave];
     uint4 pj[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) pj[u] = a.pos4[jc[u]];
+    for (int u = 0; u < U; ++u) pj[u] = pn[u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = ((g + 1) * U + u) * T + l;
+      pn[u] = a.pos4[(k < nn) ? jn[u] : si];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)((g + 2) * U + u) * kWave];
+    // keep those loads ahead of this group's arithmetic (the compiler would
+    // otherwise sink them to their use in the next iteration)
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool v = (g * U + u) * T + l < nn;
