@@ -58,6 +58,9 @@ static void dfree(T *&p) {
 
 // ----------------------------------------------------------------- engine --
 
+constexpr int kStepGraphSlots = 4;   // cached step graphs (flags, step count)
+constexpr int kMaxGraphSteps = 64;   // steps per captured graph
+
 struct sre_engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -166,15 +169,27 @@ struct sre_engine {
   bool built = false;
   bool system_set = false;
   int graph_epoch = 0;
-  hipGraphExec_t graph_exec = nullptr;
-  unsigned graph_flags = 0;
-  int graph_epoch_built = -1;
+  // step graphs: K steps of [refresh + force] each, cached by (flags, K)
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    unsigned flags = 0;
+    int steps = 0;
+    int epoch = -1;
+    uint64_t last_use = 0;
+  };
+  StepGraph step_graphs[kStepGraphSlots];
+  uint64_t graph_uses = 0;
+  // rebuild graph: the device part of a rebuild, cached by its arguments
+  hipGraphExec_t rebuild_exec = nullptr;
+  std::vector<unsigned char> rebuild_sig;
 
   ~sre_engine();
 };
 
 sre_engine::~sre_engine() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  for (StepGraph &g : step_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (rebuild_exec) (void)hipGraphExecDestroy(rebuild_exec);
   dfree(d_reps); dfree(d_rep_of); dfree(d_types); dfree(d_params);
   dfree(d_paramsd); dfree(d_rc); dfree(d_rho); dfree(d_tail);
   dfree(d_tail_entry_rep); dfree(d_tail_atom); dfree(d_pos_own);
@@ -698,12 +713,77 @@ static sre_status ensure_list(sre_engine *e, int cap) {
     e->list_size = need;
     e->graph_epoch++;
   }
+  if (e->cap != cap || e->list_lanes != T) e->graph_epoch++;  // layout of captured kernels
   e->cap = cap;
   e->list_lanes = T;
   return SRE_OK;
 }
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Everything a captured rebuild reads by value or address: a change of any
+// of it means a new capture.
+static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
+                                                    int n_parts) {
+  std::vector<unsigned char> v;
+  auto put = [&v](const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    v.insert(v.end(), c, c + n);
+  };
+  const int ints[] = {e->n_atoms, e->key_bits, e->n_cells, e->n_cols, e->n_rep,
+                      e->cap, e->tune.build_mode, e->tune.build_lanes,
+                      e->tune.force_lanes, e->n_types, e->pos_stride, n_parts,
+                      (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) ? 1 : 0};
+  put(ints, sizeof ints);
+  put(&e->skin, sizeof e->skin);
+  const void *ptrs[] = {e->d_pos, e->d_rep_of, e->d_reps, e->sb.key_a, e->sb.key_b,
+                        e->sb.val_a, e->sb.val_b, e->sb.tkey, e->sb.tval,
+                        e->sb.counts, e->sb.offsets, e->sb.scan_tmp, e->d_start,
+                        e->d_types, e->d_rc, e->d_pos4, e->d_backup, e->d_res4,
+                        e->d_rcut, e->d_tail_idx, e->d_tail_atom, e->d_paramsd,
+                        e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
+                        e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
+                        e->d_counters, e->d_skin_part};
+  put(ptrs, sizeof ptrs);
+  return v;
+}
+
+static void enqueue_build(sre_engine *e, hipStream_t st) {
+  const int N = e->n_atoms;
+  if (e->tune.build_mode == 1)
+    launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
+                       e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
+                       e->d_list, e->d_nnb, e->d_build_part, e->d_tiles_per_col,
+                       e->d_tile_off, e->d_tile_tmp, st);
+  else
+    launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
+                      e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
+                      e->d_list, e->d_nnb, e->d_build_part, st);
+}
+
+// The device part of a rebuild (capturable, no host decisions): cell keys,
+// radix sort, cell offsets, gather into cell order with rc assignment, exact
+// tail, list build, and the reduction of the longest/total list length and
+// of the segment's largest displacement (whose partials it clears).
+static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
+  const int N = e->n_atoms;
+  const bool odd = (radix_passes(e->key_bits) & 1) != 0;
+  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps,
+              odd ? e->sb.key_b : e->sb.key_a, odd ? e->sb.val_b : e->sb.val_a,
+              st);
+  launch_radix_sort(N, e->key_bits, e->sb, st);
+  launch_cell_start(e->n_cells, N, e->sb.key_a, e->d_start, st);
+  launch_gather_sorted(N, e->sb.val_a, e->d_pos, e->pos_stride, e->d_types,
+                       e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->d_pos4,
+                       e->d_backup, e->d_res4, e->d_rcut, e->d_tail_idx, st);
+  if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
+    launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
+                      e->d_paramsd, e->d_rho, e->d_tail_atom, st);
+  enqueue_build(e, st);
+  launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_part_count(N),
+                      e->d_skin_part, 1, reinterpret_cast<unsigned *>(e->d_counters + 2),
+                      st);
+}
 
 static sre_status do_rebuild(sre_engine *e) {
   if (!e->system_set) return fail(SRE_ERR_INVALID, "rebuild: system not set");
@@ -715,21 +795,10 @@ static sre_status do_rebuild(sre_engine *e) {
   if (s != SRE_OK) return s;
   const int N = e->n_atoms;
   hipStream_t st = e->stream;
-  const bool odd = (radix_passes(e->key_bits) & 1) != 0;
-  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps,
-              odd ? e->sb.key_b : e->sb.key_a, odd ? e->sb.val_b : e->sb.val_a,
-              st);
-  if (launch_radix_sort(N, e->key_bits, e->sb, st) != 0)
+  if (e->key_bits > 32 || radix_passes(e->key_bits) > 4)
     return fail(SRE_ERR_CELL, "radix sort: key of %d bits unsupported", e->key_bits);
-  launch_cell_start(e->n_cells, N, e->sb.key_a, e->d_start, st);
-  launch_gather_sorted(N, e->sb.val_a, e->d_pos, e->pos_stride, e->d_types,
-                       e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->d_pos4,
-                       e->d_backup, e->d_res4, e->d_rcut, e->d_tail_idx, st);
-  if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) {
-    if (!e->d_tail_atom) HIP_TRY(dalloc(&e->d_tail_atom, N));
-    launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
-                      e->d_paramsd, e->d_rho, e->d_tail_atom, st);
-  }
+  if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT && !e->d_tail_atom)
+    HIP_TRY(dalloc(&e->d_tail_atom, N));
   // list capacity: first guess from the densest possible shell, then exact
   int cap = e->cap;
   if (cap <= 0 || e->list_lanes != e->tune.force_lanes) {
@@ -748,41 +817,57 @@ static sre_status do_rebuild(sre_engine *e) {
     HIP_TRY(dalloc(&e->d_build_part, n_parts));
     e->build_part_cap = n_parts;
   }
-  for (int attempt = 0; attempt < 4; ++attempt) {
-    s = ensure_list(e, cap);
-    if (s != SRE_OK) return s;
-    if (e->tune.build_mode == 1)
-      launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
-                         e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                         e->d_list, e->d_nnb, e->d_build_part,
-                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp, st);
-    else
-      launch_build_list(e->tune.build_lanes, e->tune.force_lanes, N, e->d_pos4,
-                        e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
-                        e->d_list, e->d_nnb, e->d_build_part, st);
-    // longest list and total length; on the first attempt also the largest
-    // displacement recorded since the previous rebuild (its partials are
-    // cleared for the next segment)
-    launch_rebuild_info(n_parts, e->d_build_part, e->d_counters,
-                        attempt == 0 ? skin_part_count(N) : 0, e->d_skin_part,
-                        1, reinterpret_cast<unsigned *>(e->d_counters + 2), st);
-    HIP_TRY(hipGetLastError());  // launch errors of the whole rebuild sequence
-    // one pinned readback per rebuild
-    HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 3 * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+  s = ensure_list(e, cap);
+  if (s != SRE_OK) return s;
+  // the whole device sequence as one captured graph (re-captured when any
+  // argument changes), then one pinned readback
+  std::vector<unsigned char> sig = rebuild_signature(e, n_parts);
+  if (!e->rebuild_exec || sig != e->rebuild_sig) {
+    if (e->rebuild_exec) {
+      (void)hipGraphExecDestroy(e->rebuild_exec);
+      e->rebuild_exec = nullptr;
+    }
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_rebuild(e, n_parts, st);
+    hipError_t r = hipStreamEndCapture(st, &g);
+    if (r != hipSuccess)
+      return fail(SRE_ERR_HIP, "rebuild capture: %s", hipGetErrorString(r));
+    r = hipGraphInstantiate(&e->rebuild_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) {
+      e->rebuild_exec = nullptr;
+      return fail(SRE_ERR_HIP, "rebuild instantiate: %s", hipGetErrorString(r));
+    }
+    e->rebuild_sig = std::move(sig);
+  }
+  HIP_TRY(hipGraphLaunch(e->rebuild_exec, st));
+  HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 3 * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  {
+    const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
+    memcpy(&e->last_disp2, &bits, sizeof(float));
+  }
+  // overflow: grow the capacity and build again (eagerly; rare)
+  for (int attempt = 0;; ++attempt) {
     const int mc = (int)e->h_info[0];
     e->max_count = mc;
     e->n_list = (int64_t)e->h_info[1];
     if (mc <= e->cap) break;
-    cap = round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes);
     if (attempt == 3)
       return fail(SRE_ERR_LIST_OVERFLOW, "neighbor list overflow (%d > %d)", mc,
                   e->cap);
-  }
-  {
-    const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
-    memcpy(&e->last_disp2, &bits, sizeof(float));
+    cap = round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes);
+    s = ensure_list(e, cap);
+    if (s != SRE_OK) return s;
+    enqueue_build(e, st);
+    launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, 0, e->d_skin_part,
+                        0, nullptr, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 2 * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
   }
   e->built = true;
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
@@ -1374,6 +1459,58 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
 
 // ----------------------------------------------------------------- graphs --
 
+// Captures K steps of [refresh (+ perturbation) + force] into one graph, or
+// returns the cached one; the graph reads the engine's buffers by address, so
+// every reallocation or layout change bumps graph_epoch and invalidates it.
+static sre_status step_graph(sre_engine *e, unsigned flags, int K,
+                             hipGraphExec_t *out) {
+  sre_engine::StepGraph *slot = nullptr, *lru = &e->step_graphs[0];
+  for (sre_engine::StepGraph &g : e->step_graphs) {
+    if (g.exec && g.flags == flags && g.steps == K && g.epoch == e->graph_epoch)
+      slot = &g;
+    if (g.last_use < lru->last_use) lru = &g;
+  }
+  if (!slot) {
+    slot = lru;
+    if (slot->exec) {
+      (void)hipGraphExecDestroy(slot->exec);
+      slot->exec = nullptr;
+    }
+    const bool pert = (flags & SRE_PERTURB) != 0;
+    if (pert && !e->d_step_ctr)
+      return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    PerturbArgs pa{e->pert_sigma, e->pert_seed,
+                   e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
+    sre_status s = SRE_OK;
+    for (int k = 0; k < K && s == SRE_OK; ++k) {
+      launch_update_positions(e->n_atoms, e->sb.val_a,
+                              const_cast<float *>(e->d_pos), e->pos_stride,
+                              e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                              e->d_skin_part, pert ? &pa : nullptr, e->stream);
+      s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL),
+                         pert ? e->d_step_ctr : nullptr);
+    }
+    hipError_t r = hipStreamEndCapture(e->stream, &g);
+    if (s != SRE_OK) return s;
+    if (r != hipSuccess)
+      return fail(SRE_ERR_HIP, "graph capture: %s", hipGetErrorString(r));
+    r = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) {
+      slot->exec = nullptr;
+      return fail(SRE_ERR_HIP, "graph instantiate: %s", hipGetErrorString(r));
+    }
+    slot->flags = flags;
+    slot->steps = K;
+    slot->epoch = e->graph_epoch;
+  }
+  slot->last_use = ++e->graph_uses;
+  *out = slot->exec;
+  return SRE_OK;
+}
+
 extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) {
   if (!e || n_steps < 0) return fail(SRE_ERR_INVALID, "bad arguments");
   if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
@@ -1385,38 +1522,17 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
     return fail(SRE_ERR_INVALID,
                 "perturbation anchor does not match the bound positions "
                 "(call sre_set_perturbation again)");
-  if (!e->graph_exec || e->graph_flags != flags ||
-      e->graph_epoch_built != e->graph_epoch) {
-    if (e->graph_exec) {
-      (void)hipGraphExecDestroy(e->graph_exec);
-      e->graph_exec = nullptr;
-    }
-    hipGraph_t g;
-    const bool pert = (flags & SRE_PERTURB) != 0;
-    if (pert && !e->d_step_ctr)
-      return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
-
-    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    PerturbArgs pa{e->pert_sigma, e->pert_seed,
-                   e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
-    launch_update_positions(e->n_atoms, e->sb.val_a,
-                            const_cast<float *>(e->d_pos), e->pos_stride,
-                            e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
-                            e->d_skin_part, pert ? &pa : nullptr, e->stream);
-    sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL),
-                                  pert ? e->d_step_ctr : nullptr);
-    hipError_t r = hipStreamEndCapture(e->stream, &g);
+  // whole graphs of kMaxGraphSteps steps, then one graph for the remainder
+  int left = n_steps;
+  while (left > 0) {
+    const int K = std::min(left, kMaxGraphSteps);
+    hipGraphExec_t ex;
+    sre_status s = step_graph(e, flags, K, &ex);
     if (s != SRE_OK) return s;
-    if (r != hipSuccess)
-      return fail(SRE_ERR_HIP, "graph capture: %s", hipGetErrorString(r));
-    r = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (r != hipSuccess)
-      return fail(SRE_ERR_HIP, "graph instantiate: %s", hipGetErrorString(r));
-    e->graph_flags = flags;
-    e->graph_epoch_built = e->graph_epoch;
+    const int reps = left / K;
+    for (int i = 0; i < reps; ++i) HIP_TRY(hipGraphLaunch(ex, e->stream));
+    left -= reps * K;
   }
-  for (int i = 0; i < n_steps; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
   if (n_steps > 0) {
     e->forces_valid = true;
     e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
