@@ -90,11 +90,14 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
     # force kernel alone, back-to-back launches on the same stream (HIP events)
     e.time_compute(20, stats=True, tail=True)      # warm the clocks and caches
     k_ms = e.time_compute(200, stats=True, tail=True)
+    dist = force_time_distribution(e, torch, stream)
+    rb_ms = e.time_rebuild(10)
     e.compute(stats=True, tail=True)
     e.stats()                   # raises if any pair force left the fixed-point range
     nl, nin, mx = e.neighbor_count()
     return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs,
                 rebuilds=rebuilds, force_ms=k_ms, n_list=nl, n_in_rc=nin, max_list=mx,
+                force_us_dist=dist, rebuild_ms=rb_ms,
                 pair_evals_per_s=pairs / wall, ms_per_step=1e3 * wall / steps)
 
 
@@ -138,13 +141,16 @@ def measured_traffic(workload):
     return None
 
 
-def cpu_baseline(w, budget_s):
-    """The FP64 oracle (C++/OpenMP) on the host: list build + force evaluations."""
-    import numpy as np
+def _cpu_rate(w, budget_s, max_evals=2000):
+    """FP64 oracle on the host: one list build, then force evaluations for
+    about budget_s.  Returns (pair evals/s, ms per force, ms per build, L, n)."""
     from oracle import oracle as O
     box = w.boxes[0]
     rc, _ = O.assign_rc(w.xyz, box, w.rc_tables[0])
-    tail = O.tail_atoms(w.xyz, box, w.types, w.eps, w.sigma, w.rc_tables[0].ravel(), w.rho[0])
+    tail = None
+    if w.rho and w.rho[0] is not None:
+        tail = O.tail_atoms(w.xyz, box, w.types, w.eps, w.sigma, w.rc_tables[0].ravel(),
+                            w.rho[0])
     t0 = time.perf_counter()
     off, j = O.build_pairs(w.xyz, box, rc, w.skin)
     t_build = time.perf_counter() - t0
@@ -154,17 +160,56 @@ def cpu_baseline(w, budget_s):
         O.forces(w.xyz, box, w.types, w.eps, w.sigma, rc, off, j, tail=tail)
         n_eval += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n_eval >= 2000:
+        if el >= budget_s or n_eval >= max_evals:
             break
     t_force = el / n_eval
     L = int(off[-1])
-    return {"value": L / t_force, "unit": "pair evals/s", "cores": O.num_threads(),
-            "kind": "port",
-            "sample": f"{n_eval} FP64 force evaluations of the 64k slab ({L} list entries, "
-                      f"{t_force * 1e3:.2f} ms each) + 1 list build ({t_build * 1e3:.1f} ms); "
-                      f"ms/step at the same rebuild period "
-                      f"{(t_force + t_build / 6) * 1e3:.2f}",
-            "ms_force": t_force * 1e3, "ms_build": t_build * 1e3}
+    return L / t_force, t_force * 1e3, t_build * 1e3, L, n_eval
+
+
+def cpu_baseline(w, budget_s, every, configs):
+    """The FP64 oracle (C++/OpenMP) on the host: all threads and one thread,
+    on the benchmark workload and on the 4k FCC plain-method case."""
+    from oracle import oracle as O
+    threads = O.num_threads()
+    rate, ms_f, ms_b, L, n_eval = _cpu_rate(w, budget_s)
+    out = {"value": rate, "unit": "pair evals/s", "cores": threads, "kind": "port",
+           "sample": f"{n_eval} FP64 force evaluations of the {w.name} workload ({L} list "
+                     f"entries, {ms_f:.2f} ms each) + 1 list build ({ms_b:.1f} ms); "
+                     f"ms/step at the same rebuild period {ms_f + ms_b / every:.2f}",
+           "ms_force": ms_f, "ms_build": ms_b,
+           "ms_per_step": ms_f + ms_b / every}
+    try:
+        O.set_threads(1)
+        r1, f1, b1, _, n1 = _cpu_rate(w, budget_s / 2, max_evals=50)
+        out["single_thread"] = {"value": r1, "ms_force": f1, "ms_build": b1,
+                                "ms_per_step": f1 + b1 / every, "evals": n1}
+        fcc = configs.plain_fcc()
+        for t, key in ((threads, "fcc_4k_all_threads"), (1, "fcc_4k_single_thread")):
+            O.set_threads(t)
+            rf, ff, bf, Lf, nf = _cpu_rate(fcc, 1.0, max_evals=500)
+            out[key] = {"value": rf, "ms_force": ff, "ms_build": bf, "list_entries": Lf,
+                        "ms_per_step": ff + bf / every, "threads": t}
+    finally:
+        O.set_threads(threads)
+    return out
+
+
+def force_time_distribution(e, torch, stream, n=200):
+    """Per-launch force kernel durations (HIP events around each launch on the
+    engine stream): median, p10, p90 in microseconds."""
+    import numpy as np
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(n)]
+    for a, b in evs:
+        a.record(stream)
+        e.compute(stats=True, tail=True)
+        b.record(stream)
+    torch.cuda.synchronize()
+    t = np.array([a.elapsed_time(b) * 1e3 for a, b in evs])
+    return {"p10": round(float(np.percentile(t, 10)), 3),
+            "median": round(float(np.median(t)), 3),
+            "p90": round(float(np.percentile(t, 90)), 3)}
 
 
 def main():
@@ -231,7 +276,7 @@ def main():
                 "avg_launch_us": round(mixed["force_ms"] * 1e3, 3)}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(w, args.cpu_seconds)
+        cpu = cpu_baseline(w, args.cpu_seconds, args.rebuild_every, configs)
         log(f"[bench] cpu baseline: {cpu['value']:.4g} pair evals/s on {cpu['cores']} threads")
     result = {
         "metric": METRIC,
@@ -255,6 +300,9 @@ def main():
                    "list_entries": L, "pairs_in_rc": mixed["n_in_rc"],
                    "tuning": mixed["engine"].tuning()},
         "force_kernel_us": round(mixed["force_ms"] * 1e3, 3),
+        "force_kernel_us_per_launch": mixed["force_us_dist"],
+        "rebuild_us": round(mixed["rebuild_ms"] * 1e3, 1),
+        "rebuild_us_per_step": round(mixed["rebuild_ms"] * 1e3 / args.rebuild_every, 2),
         "force_pair_evals_per_s": round(L / (mixed["force_ms"] * 1e-3), 1),
         "roofline": roofline,
         "accuracy_vs_fp64": acc,
@@ -263,7 +311,9 @@ def main():
     if fp64 is not None:
         result["fp64_form"] = {
             "value": round(fp64["pair_evals_per_s"], 1), "ms_per_step": round(fp64["ms_per_step"], 5),
-            "force_kernel_us": round(fp64["force_ms"] * 1e3, 3)}
+            "force_kernel_us": round(fp64["force_ms"] * 1e3, 3),
+            "force_kernel_us_per_launch": fp64["force_us_dist"],
+            "rebuild_us": round(fp64["rebuild_ms"] * 1e3, 1)}
         result["speedup_mixed_vs_fp64"] = {
             "whole_step": round(mixed["pair_evals_per_s"] / fp64["pair_evals_per_s"], 3),
             "force_kernel": round(fp64["force_ms"] / mixed["force_ms"], 3)}

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void k_build_list(
   const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
   const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
   const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
+  const float ihz = 1.f / hz;
   const float rl = rcut[si] + skin;
   const float rl2 = rl * rl;
   const float rlm = rl * 1.0001f + 1e-4f;  // conservative window radius
@@ -94,12 +95,13 @@ __global__ __launch_bounds__(256) void k_build_list(
     // runs of this lane's kBuildCols columns -> (first atom, length) in LDS
     const int slot0 = l * 2 * kBuildCols;
     int tot = 0;
+    // this lane's first column; the following ones step oy, then ox
+    int c = cbase + l * kBuildCols;
+    int ox = ox0 + c / noy, oy = oy0 + c % noy;
 #pragma unroll 2
-    for (int m = 0; m < kBuildCols; ++m) {
-      const int c = cbase + l * kBuildCols + m;
+    for (int m = 0; m < kBuildCols; ++m, ++c) {
       int ra0 = 0, rb0 = -1, ra1 = 0, rb1 = -1, colbase = 0;
       if (c < ncol) {
-        const int ox = ox0 + c / noy, oy = oy0 + c % noy;
         float gx = 0.f, gy = 0.f;
         if (!fullx) gx = ox > 0 ? ox * hx - fx : (ox < 0 ? fx + (-ox - 1) * hx : 0.f);
         if (!fully) gy = oy > 0 ? oy * hy - fy : (oy < 0 ? fy + (-oy - 1) * hy : 0.f);
@@ -116,9 +118,11 @@ __global__ __launch_bounds__(256) void k_build_list(
           colbase = R.cell_off + (cxw * ncy + cyw) * ncz;
           rb0 = ncz - 1;  // whole column unless the chord is shorter
           if (!fullz) {
+            // (the window radius carries a 1e-4 margin, far above the
+            // rounding of a multiplication by 1/hz)
             const float dz = sqrtf(fmaxf(rlm2 - gxy2, 0.f));
-            const int lo = cz + (int)floorf((fz - dz) / hz);
-            const int hi = cz + (int)floorf((fz + dz) / hz);
+            const int lo = cz + (int)floorf((fz - dz) * ihz);
+            const int hi = cz + (int)floorf((fz + dz) * ihz);
             if (hi - lo + 1 < ncz) {
               if (lo < 0) {
                 ra0 = lo + ncz; rb0 = ncz - 1; ra1 = 0; rb1 = hi;
@@ -143,6 +147,10 @@ __global__ __launch_bounds__(256) void k_build_list(
       send[slot0 + 2 * m] = n0;       // length for now, prefix below
       sp0[slot0 + 2 * m + 1] = q1;
       send[slot0 + 2 * m + 1] = n1;
+      if (++oy > oy1) {
+        oy = oy0;
+        ++ox;
+      }
       tot += n0 + n1;
     }
     // flat sequence: prefix over the group (lane-major slot order)
@@ -161,7 +169,10 @@ __global__ __launch_bounds__(256) void k_build_list(
     __builtin_amdgcn_wave_barrier();
     // stream the flat sequence: lane l takes v = l, l + TB, ...
     const int rounds = __builtin_amdgcn_readfirstlane(wave_max_i((T + TB * U - 1) / (TB * U)));
-    int cur = 0, beg = 0, end = send[0];
+    // cursor over the runs: run cur covers flat indices [beg, end) and
+    // starts at sorted atom p0c (kept in registers; LDS is read only when
+    // the cursor moves to the next run)
+    int cur = 0, beg = 0, end = send[0], p0c = sp0[0];
     for (int k0 = 0; k0 < rounds; ++k0) {
       uint4 cj[U];
       int jj[U];
@@ -174,8 +185,9 @@ __global__ __launch_bounds__(256) void k_build_list(
             ++cur;
             beg = end;
             end = send[cur];
+            p0c = sp0[cur];
           }
-          j = sp0[cur] + (v - beg);
+          j = p0c + (v - beg);
         }
         jj[u] = j;
         cj[u] = pos4[j >= 0 ? j : si];
