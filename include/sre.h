@@ -90,7 +90,8 @@ const char *sre_version(void);
 
 /* ---- system description -------------------------------------------------- */
 
-/* Declares R replicas with n_atoms[r] atoms and box edge lengths box[3r..3r+2].
+/* Declares R replicas with n_atoms[r] atoms and box edge lengths box[3r..3r+2]
+ * (fewer than 2^27 atoms in total; batched replicas need >= 64 atoms each).
  * Replaces MDSystem::initConfig/initDeviceData and RectangularBox setup
  * (H/src/MDSystem_interface.cu:39-171, H/include/BoxGeometry.h:10-14). */
 sre_status sre_set_system(sre_engine *e, int n_replicas, const int *n_atoms,

@@ -296,8 +296,9 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
         return fail(SRE_ERR_INVALID, "replica %d: box edge %d must be > 0", r, d);
     total += n_atoms[r];
   }
-  if (total > (1ll << 30))
-    return fail(SRE_ERR_INVALID, "too many atoms (%lld)", total);
+  // pos4 is addressed with 32-bit byte offsets (16 B per atom)
+  if (total >= (1ll << 27))
+    return fail(SRE_ERR_INVALID, "too many atoms (%lld, limit 2^27 - 1)", total);
   HIP_TRY(hipStreamSynchronize(e->stream));
   e->n_rep = n_replicas;
   e->n_atoms = (int)total;

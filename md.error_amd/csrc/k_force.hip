@@ -164,6 +164,25 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return x * q + min(x, r) + (b >> 3);
 }
 
+// Position gathers through a buffer resource over pos4: the 32-bit byte
+// offset j * 16 replaces a 64-bit address computation per pair (pos4 holds
+// fewer than 2^27 atoms, checked when the system is set).  gfx9 resource
+// word 3 = 0x00020000 (32-bit data format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc(const uint4 *p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(p), (short)0,
+                                           n * 16, 0x00020000);
+}
+template <bool W>
+__device__ __forceinline__ uint4 gather_pos(__amdgpu_buffer_rsrc_t r, int j) {
+  if (W) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, j * 16, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+  } else {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 16, 0, 0);
+    return make_uint4(v[0], v[1], v[2], 0u);
+  }
+}
+
 // Wave-wide maximum of an int (butterfly over the 64 lanes).
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
@@ -217,33 +236,11 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   uint64_t ax = 0, ay = 0, az = 0;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
   const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
-  // software pipeline: while group g is evaluated, the gathers of group g+1
-  // and the list rows of group g+2 are in flight
-  int jn[U];    // list rows of the next group
-  uint4 pn[U];  // positions of the current group, gathered one group ahead
-  {
-    int j0[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) j0[u] = lp[(size_t)u * kWave];
-#pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)(U + u) * kWave];
-#pragma unroll
-    for (int u = 0; u < U; ++u) pn[u] = a.pos4[(u * T + l < nn) ? j0[u] : si];
-  }
-  for (int g = 0; g < groups; ++g) {
-    uint4 pj[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) pj[u] = pn[u];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = ((g + 1) * U + u) * T + l;
-      pn[u] = a.pos4[(k < nn) ? jn[u] : si];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = lp[(size_t)((g + 2) * U + u) * kWave];
-    // keep those loads ahead of this group's arithmetic (the compiler would
-    // otherwise sink them to their use in the next iteration)
-    asm volatile("" ::: "memory");
+  const __amdgpu_buffer_rsrc_t prs = pos_rsrc(a.pos4, a.n);
+  // software pipeline over two position buffers: while group g is evaluated
+  // from one, the gathers of group g+1 fill the other and the list rows of
+  // group g+2 are in flight
+  auto eval_group = [&](const uint4 *pj, int g) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool v = (g * U + u) * T + l < nn;
@@ -269,6 +266,32 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
         wy = __builtin_fmaf(fpy, dy, wy);
         wz = __builtin_fmaf(fpz, dz, wz);
       }
+    }
+  };
+  // issue the gathers of group g into p from rows jr, then load rows of
+  // group g + 2 into jr; the compiler fence keeps these loads ahead of the
+  // arithmetic that follows (it would otherwise sink them to their use)
+  auto prefetch = [&](uint4 *p, int *jr, int g) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      p[u] = gather_pos<MULTI>(prs, ((g * U + u) * T + l < nn) ? jr[u] : si);
+#pragma unroll
+    for (int u = 0; u < U; ++u) jr[u] = lp[(size_t)((g + 2) * U + u) * kWave];
+    asm volatile("" ::: "memory");
+  };
+  int ja[U], jb[U];   // list rows of the groups due in buffers a and b
+  uint4 pa[U], pb[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) ja[u] = lp[(size_t)u * kWave];
+#pragma unroll
+  for (int u = 0; u < U; ++u) jb[u] = lp[(size_t)(U + u) * kWave];
+  prefetch(pa, ja, 0);
+  for (int g = 0; g < groups; g += 2) {
+    prefetch(pb, jb, g + 1);
+    eval_group(pa, g);
+    if (g + 1 < groups) {  // wave-uniform
+      prefetch(pa, ja, g + 2);
+      eval_group(pb, g + 1);
     }
   }
   const int iters = groups * U;

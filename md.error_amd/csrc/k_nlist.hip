@@ -16,10 +16,11 @@
 // round (every lane issues its kBuildUnroll gathers before testing any), so
 // short runs cost no partially filled batches.  The exact predicate is
 // applied to every candidate, so the windows only need to be conservative.
-// Accepted neighbours are compacted with a group-wide ballot into a
-// deterministic order and written to the wave-tiled column-major layout the
-// force kernel reads (list_slot()).  Writes beyond the capacity are dropped
-// but still counted; the host regrows and rebuilds.
+// Each round's accepted neighbours are placed by an exclusive prefix over the
+// group's lanes (deterministic (lane, unroll) order) and written to the
+// wave-tiled column-major layout the force kernel reads (list_slot()).
+// Writes beyond the capacity are dropped but still counted; the host regrows
+// and rebuilds.
 #include "sre_internal.h"
 
 namespace sre {
@@ -52,9 +53,6 @@ __global__ __launch_bounds__(256) void k_build_list(
   const int s = wave * A + g;
   const bool act = s < n;
   const int si = act ? s : n - 1;
-  const int shift = g * TB;
-  const uint64_t gmask = (TB == 64) ? ~0ull : ((1ull << TB) - 1ull);
-  const uint64_t below = (1ull << l) - 1ull;
 
   const uint4 pi = pos4[si];
   const RepInfo &R = reps[pi.w >> 8];
@@ -88,6 +86,7 @@ __global__ __launch_bounds__(256) void k_build_list(
   const int lt = __builtin_ctz((unsigned)TF);
   const size_t sbase =
       (size_t)(si / AF) * (size_t)cap * (size_t)AF + (size_t)((si % AF) * TF);
+  int *const lrow = list + sbase;
   int *const sp0 = s_p0[gb];
   int *const send = s_end[gb];
   int count = 0;
@@ -192,6 +191,9 @@ __global__ __launch_bounds__(256) void k_build_list(
         jj[u] = j;
         cj[u] = pos4[j >= 0 ? j : si];
       }
+      // accepted candidates of this round as a bit mask per lane; their list
+      // positions follow (lane, u) order: an exclusive prefix over the group
+      unsigned okm = 0u;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = jj[u];
@@ -199,11 +201,24 @@ __global__ __launch_bounds__(256) void k_build_list(
         const float dy = disp(cj[u].y, pi.y, qy);
         const float dz2 = disp(cj[u].z, pi.z, qz);
         const bool ok = (j >= 0) & (j != s) & (dist2(dx, dy, dz2) < rl2);
-        const uint64_t gbits = (__ballot(ok) >> shift) & gmask;
-        const int k = count + __popcll(gbits & below);
-        if (ok && k < cap) list[sbase + ((size_t)(k >> lt) << 6) + (k & (TF - 1))] = j;
-        count += __popcll(gbits);
+        okm |= ok ? (1u << u) : 0u;
       }
+      const int c = __popc(okm);
+      int incl = c;
+#pragma unroll
+      for (int o = 1; o < TB; o <<= 1) {
+        const int t = __shfl_up(incl, o, TB);
+        if (l >= o) incl += t;
+      }
+      int k = count + incl - c;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if ((okm >> u) & 1u) {
+          if (k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = jj[u];
+          ++k;
+        }
+      }
+      count += __shfl(incl, TB - 1, TB);
     }
     __builtin_amdgcn_wave_barrier();  // LDS reuse by the next chunk
   }
