@@ -768,15 +768,12 @@ static void enqueue_build(sre_engine *e, hipStream_t st) {
 // of the segment's largest displacement (whose partials it clears).
 static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   const int N = e->n_atoms;
-  const bool odd = (radix_passes(e->key_bits) & 1) != 0;
-  launch_keys(N, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps,
-              odd ? e->sb.key_b : e->sb.key_a, odd ? e->sb.val_b : e->sb.val_a,
-              st);
-  launch_radix_sort(N, e->key_bits, e->sb, st);
-  launch_cell_start(e->n_cells, N, e->sb.key_a, e->d_start, st);
-  launch_gather_sorted(N, e->sb.val_a, e->d_pos, e->pos_stride, e->d_types,
-                       e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->d_pos4,
-                       e->d_backup, e->d_res4, e->d_rcut, e->d_tail_idx, st);
+  launch_radix_sort(N, e->key_bits, e->d_pos, e->pos_stride, e->d_rep_of,
+                    e->d_reps, e->sb, st);
+  launch_gather_sorted(N, e->n_cells, e->sb.key_a, e->sb.val_a, e->d_pos,
+                       e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
+                       e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
+                       e->d_res4, e->d_rcut, e->d_tail_idx, st);
   if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);

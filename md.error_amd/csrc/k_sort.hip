@@ -25,30 +25,17 @@ constexpr int kScanTile = 4096;   // elements per scan tile (1024 threads x 4)
 
 // ------------------------------------------------------------------ keys --
 
-__global__ __launch_bounds__(256) void k_keys(int n, const float *pos,
-                                              int stride, const int *rep_of,
-                                              const RepInfo *reps,
-                                              uint32_t *key, uint32_t *val) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Cell key of atom i: replica cell offset + (cx, cy, cz) with z fastest, so a
+// column of cells is one contiguous range of sorted atoms.
+__device__ __forceinline__ uint32_t cell_key(const float *pos, int stride,
+                                             const int *rep_of,
+                                             const RepInfo *reps, int i) {
   const RepInfo &R = reps[rep_of[i]];
   const float *p = pos + (size_t)i * stride;
-  uint32_t ux = quantize(p[0], R.invq[0]);
-  uint32_t uy = quantize(p[1], R.invq[1]);
-  uint32_t uz = quantize(p[2], R.invq[2]);
-  int cx = cell_of(ux, R.nc[0]);
-  int cy = cell_of(uy, R.nc[1]);
-  int cz = cell_of(uz, R.nc[2]);
-  key[i] = (uint32_t)(R.cell_off + (cx * R.nc[1] + cy) * R.nc[2] + cz);
-  val[i] = (uint32_t)i;
-}
-
-void launch_keys(int n, const float *pos, int stride, const int *rep_of,
-                 const RepInfo *reps, uint32_t *key, uint32_t *val,
-                 hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, pos,
-                     stride, rep_of, reps, key, val);
+  const int cx = cell_of(quantize(p[0], R.invq[0]), R.nc[0]);
+  const int cy = cell_of(quantize(p[1], R.invq[1]), R.nc[1]);
+  const int cz = cell_of(quantize(p[2], R.invq[2]), R.nc[2]);
+  return (uint32_t)(R.cell_off + (cx * R.nc[1] + cy) * R.nc[2] + cz);
 }
 
 // ------------------------------------------------------- block scan 1024 --
@@ -85,10 +72,13 @@ __device__ uint32_t block_exscan_1024(uint32_t v, uint32_t *tmp,
 
 // Stable local ranking of one block of keys by digit (key >> shift) & mask:
 // writes the block's keys/values to tkey/tval in (digit, position) order and
-// the block's digit counts digit-major to counts[d * nblocks + b].
+// the block's digit counts digit-major to counts[d * nblocks + b].  The first
+// pass (pos != NULL) computes the cell keys from the positions (value = atom
+// index) instead of reading key/val.
 __global__ __launch_bounds__(kSortT) void k_sort_block(
     int n, int shift, int bits, int nblocks, const uint32_t *key,
-    const uint32_t *val, uint32_t *tkey, uint32_t *tval, uint32_t *counts) {
+    const uint32_t *val, const float *pos, int stride, const int *rep_of,
+    const RepInfo *reps, uint32_t *tkey, uint32_t *tval, uint32_t *counts) {
   constexpr int W = kSortT / 64;
   __shared__ uint32_t wcnt[W][1 << kMaxDigitBits];  // per wave, then prefix
   __shared__ uint32_t dstart[1 << kMaxDigitBits];
@@ -98,8 +88,16 @@ __global__ __launch_bounds__(kSortT) void k_sort_block(
   const int base = b * kSortE;
   const bool valid = base + t < n;
   const uint32_t R = 1u << bits, mask = R - 1u;
-  const uint32_t k = valid ? key[base + t] : 0u;
-  const uint32_t v = valid ? val[base + t] : 0u;
+  uint32_t k = 0u, v = 0u;
+  if (valid) {
+    if (pos) {
+      k = cell_key(pos, stride, rep_of, reps, base + t);
+      v = (uint32_t)(base + t);
+    } else {
+      k = key[base + t];
+      v = val[base + t];
+    }
+  }
   const uint32_t d = (k >> shift) & mask;
   for (int x = t; x < W * (1 << kMaxDigitBits); x += kSortT) (&wcnt[0][0])[x] = 0u;
   // lanes of this wave with the same digit
@@ -269,64 +267,60 @@ int radix_passes(int key_bits) {
   return np;
 }
 
-int launch_radix_sort(int n, int key_bits, const SortBufs &b, hipStream_t st) {
+int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
+                      const int *rep_of, const RepInfo *reps, const SortBufs &b,
+                      hipStream_t st) {
   if (n <= 0) return 0;
   if (key_bits > kMaxPasses * kMaxDigitBits) return -1;
   int np, bits[kMaxPasses];
   digit_plan(key_bits, &np, bits);
   const int nblocks = (n + kSortE - 1) / kSortE;
-  // ping-pong so that the last pass writes key_a/val_a
+  // pass p writes key_a/val_a when np - 1 - p is even, so the last pass
+  // always ends there; pass 0 reads the positions
   const bool odd = (np & 1) != 0;
-  uint32_t *kin = odd ? b.key_b : b.key_a, *vin = odd ? b.val_b : b.val_a;
+  uint32_t *kin = nullptr, *vin = nullptr;
   uint32_t *kout = odd ? b.key_a : b.key_b, *vout = odd ? b.val_a : b.val_b;
   int shift = 0;
   for (int p = 0; p < np; ++p) {
     const size_t m = ((size_t)1 << bits[p]) * nblocks;
     hipLaunchKernelGGL(k_sort_block, dim3(nblocks), dim3(kSortT), 0, st, n,
-                       shift, bits[p], nblocks, kin, vin, b.tkey, b.tval,
-                       b.counts);
+                       shift, bits[p], nblocks, kin, vin, p == 0 ? pos : nullptr,
+                       stride, rep_of, reps, b.tkey, b.tval, b.counts);
     device_exscan_u32(m, b.counts, b.offsets, b.scan_tmp, st);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortT), 0, st, n,
                        shift, bits[p], nblocks, b.tkey, b.tval, b.counts,
                        b.offsets, kout, vout);
     shift += bits[p];
-    uint32_t *t;
-    t = kin; kin = kout; kout = t;
-    t = vin; vin = vout; vout = t;
+    kin = kout;
+    vin = vout;
+    kout = (kout == b.key_a) ? b.key_b : b.key_a;
+    vout = (vout == b.val_a) ? b.val_b : b.val_a;
   }
   return 0;
 }
 
 // ------------------------------------------------------------- cell start --
 
-// start[c] = first sorted position with key >= c, for c = 0..n_cells: thread
-// p (< n) fills the cells from key[p-1] + 1 to key[p]; thread n fills the
-// cells after the last key.  Every entry is written exactly once.
-__global__ __launch_bounds__(256) void k_cell_start(int n_cells, int n,
-                                                    const uint32_t *keys,
-                                                    int *start) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > n) return;
-  const int lo = p == 0 ? 0 : (int)keys[p - 1] + 1;
-  const int hi = p == n ? n_cells : (int)keys[p];
-  for (int c = lo; c <= hi; ++c) start[c] = p;
-}
-
-void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_start, dim3((n + 1 + 255) / 256), dim3(256), 0, st,
-                     n_cells, n, keys, start);
-}
-
-// ---------------------------------------------------------- gather sorted --
-
+// One thread per sorted slot s (plus one): gathers atom perm[s] into cell
+// order -- lattice coordinates, type/replica word, rebuild-time backup, FP64
+// sub-lattice residual, rc from the table (the reference's AssignRCut rule)
+// and tail-table index -- and fills the cell offsets: start[c] = first sorted
+// position with key >= c, thread s covering the cells from key[s-1] + 1 to
+// key[s] (thread n: the cells after the last key), so every entry of
+// start[0..n_cells] is written exactly once.
 __global__ __launch_bounds__(256) void k_gather_sorted(
-    int n, const uint32_t *perm, const float *pos, int stride,
-    const int *types, const int *rep_of, const RepInfo *reps,
-    const float *rc_table, int n_types, uint4 *pos4, uint4 *backup,
-    float4 *res4, float *rcut, int *tail_idx) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+    int n, int n_cells, const uint32_t *keys, const uint32_t *perm,
+    const float *pos, int stride, const int *types, const int *rep_of,
+    const RepInfo *reps, const float *rc_table, int n_types, int *start,
+    uint4 *pos4, uint4 *backup, float4 *res4, float *rcut, int *tail_idx) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > n) return;
+  {
+    const int lo = s == 0 ? 0 : (int)keys[s - 1] + 1;
+    const int hi = s == n ? n_cells : (int)keys[s];
+    for (int c = lo; c <= hi; ++c) start[c] = s;
+  }
+  if (s == n) return;
   const int i = (int)perm[s];
   const int r = rep_of[i];
   const RepInfo &R = reps[r];
@@ -349,15 +343,17 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
   tail_idx[s] = (R.tail_off >= 0) ? (R.tail_off + iz) * n_types + ty : -1;
 }
 
-void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
-                          int stride, const int *types, const int *rep_of,
+void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
+                          const uint32_t *perm, const float *pos, int stride,
+                          const int *types, const int *rep_of,
                           const RepInfo *reps, const float *rc_table,
-                          int n_types, uint4 *pos4, uint4 *backup, float4 *res4,
-                          float *rcut, int *tail_idx, hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_gather_sorted, dim3((n + 255) / 256), dim3(256), 0, st,
-                     n, perm, pos, stride, types, rep_of, reps, rc_table,
-                     n_types, pos4, backup, res4, rcut, tail_idx);
+                          int n_types, int *start, uint4 *pos4, uint4 *backup,
+                          float4 *res4, float *rcut, int *tail_idx,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_sorted, dim3((n + 1 + 255) / 256), dim3(256), 0,
+                     st, n, n_cells, keys, perm, pos, stride, types, rep_of,
+                     reps, rc_table, n_types, start, pos4, backup, res4, rcut,
+                     tail_idx);
 }
 
 // ------------------------------------------------------------------ fills --

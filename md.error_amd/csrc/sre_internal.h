@@ -163,24 +163,24 @@ struct SortBufs {
   uint32_t *counts, *offsets, *scan_tmp;
 };
 
-void launch_keys(int n, const float *pos, int stride, const int *rep_of,
-                 const RepInfo *reps, uint32_t *key, uint32_t *val,
-                 hipStream_t st);
-// Sorts (key, val) ascending by key (stable), key < 2^key_bits.  The input
-// must be in key_b/val_b when radix_passes(key_bits) is odd, else in
-// key_a/val_a; the result is always in key_a/val_a, so the sorted
-// permutation has a fixed address (captured graphs read it).  Returns 0 on
-// success.
+// Sorts the atoms by cell key (stable; key < 2^key_bits, computed from the
+// positions in the first pass): sorted keys in key_a, the permutation
+// (sorted slot -> atom) in val_a, always there, so it has a fixed address
+// that captured graphs read.  Returns 0 on success.
 int radix_passes(int key_bits);
-int launch_radix_sort(int n, int key_bits, const SortBufs &b, hipStream_t st);
+int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
+                      const int *rep_of, const RepInfo *reps, const SortBufs &b,
+                      hipStream_t st);
 size_t radix_counts_capacity(int n, int key_bits);
-void launch_cell_start(int n_cells, int n, const uint32_t *keys, int *start,
-                       hipStream_t st);
-void launch_gather_sorted(int n, const uint32_t *perm, const float *pos,
-                          int stride, const int *types, const int *rep_of,
+// Gather into cell order (+ rc assignment) fused with the cell offsets
+// start[0..n_cells] computed from the sorted keys.
+void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
+                          const uint32_t *perm, const float *pos, int stride,
+                          const int *types, const int *rep_of,
                           const RepInfo *reps, const float *rc_table,
-                          int n_types, uint4 *pos4, uint4 *backup, float4 *res4,
-                          float *rcut, int *tail_idx, hipStream_t st);
+                          int n_types, int *start, uint4 *pos4, uint4 *backup,
+                          float4 *res4, float *rcut, int *tail_idx,
+                          hipStream_t st);
 // Both builders write one (longest list, total length) pair per wavefront to
 // part[] (build_part_count entries, every one written by every launch);
 // launch_rebuild_info reduces them.
