@@ -22,6 +22,9 @@ constexpr int kSortT = 1024;      // threads per sort block
 constexpr int kMaxDigitBits = 8;
 constexpr int kMaxPasses = 4;
 constexpr int kScanTile = 4096;   // elements per scan tile (1024 threads x 4)
+// up to this many sort blocks the scatter derives its offsets itself (each
+// block reads all nblocks x 2^bits counts) instead of a separate scan
+constexpr int kFusedScanBlocks = 128;
 
 // ------------------------------------------------------------------ keys --
 
@@ -130,18 +133,48 @@ __global__ __launch_bounds__(kSortT) void k_sort_block(
   }
 }
 
+// Stable scatter of block b's locally ranked keys.  Global offsets come from
+// the digit-major exclusive scan (offsets), or, when offsets is NULL (few
+// blocks), are computed here: every block sums each digit's row of counts
+// over all blocks and over the blocks before it (one wave per digit row,
+// coalesced, fixed-order wave reduction) and scans the digit totals.
 __global__ __launch_bounds__(kSortT) void k_sort_scatter(
     int n, int shift, int bits, int nblocks, const uint32_t *tkey,
     const uint32_t *tval, const uint32_t *counts, const uint32_t *offsets,
     uint32_t *okey, uint32_t *oval) {
   __shared__ uint32_t lstart[1 << kMaxDigitBits];
   __shared__ uint32_t goff[1 << kMaxDigitBits];
+  __shared__ uint32_t s_pre[1 << kMaxDigitBits];
   __shared__ uint32_t tmp[kSortT / 64 + 1];
   const int b = blockIdx.x;
   const int base = b * kSortE;
   const int nv = min(kSortE, n - base);
   const int R = 1 << bits;
   const uint32_t mask = (uint32_t)R - 1u;
+  if (!offsets) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = w; d < R; d += kSortT / 64) {
+      uint32_t tot = 0, pre = 0;
+      for (int bb = lane; bb < nblocks; bb += 64) {
+        const uint32_t c = counts[(size_t)d * nblocks + bb];
+        tot += c;
+        pre += bb < b ? c : 0u;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        tot += __shfl_xor(tot, o, 64);
+        pre += __shfl_xor(pre, o, 64);
+      }
+      if (lane == 0) {
+        goff[d] = tot;
+        s_pre[d] = pre;
+      }
+    }
+    __syncthreads();
+    const uint32_t tot = (int)threadIdx.x < R ? goff[threadIdx.x] : 0u;
+    const uint32_t ds = block_exscan_1024(tot, tmp, nullptr);
+    if ((int)threadIdx.x < R) goff[threadIdx.x] = ds + s_pre[threadIdx.x];
+  }
   const int per = (R + kSortT - 1) / kSortT;
   const int lo = min(R, (int)threadIdx.x * per), hi = min(R, lo + per);
   uint32_t local = 0;
@@ -150,7 +183,7 @@ __global__ __launch_bounds__(kSortT) void k_sort_scatter(
   for (int d = lo; d < hi; ++d) {
     lstart[d] = run;
     run += counts[(size_t)d * nblocks + b];
-    goff[d] = offsets[(size_t)d * nblocks + b];
+    if (offsets) goff[d] = offsets[(size_t)d * nblocks + b];
   }
   __syncthreads();
   for (int p = threadIdx.x; p < nv; p += kSortT) {
@@ -286,10 +319,12 @@ int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
     hipLaunchKernelGGL(k_sort_block, dim3(nblocks), dim3(kSortT), 0, st, n,
                        shift, bits[p], nblocks, kin, vin, p == 0 ? pos : nullptr,
                        stride, rep_of, reps, b.tkey, b.tval, b.counts);
-    device_exscan_u32(m, b.counts, b.offsets, b.scan_tmp, st);
+    // few blocks: each scatter block derives its offsets from the counts
+    const bool fused = nblocks <= kFusedScanBlocks;
+    if (!fused) device_exscan_u32(m, b.counts, b.offsets, b.scan_tmp, st);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortT), 0, st, n,
                        shift, bits[p], nblocks, b.tkey, b.tval, b.counts,
-                       b.offsets, kout, vout);
+                       fused ? nullptr : b.offsets, kout, vout);
     shift += bits[p];
     kin = kout;
     vin = vout;
@@ -316,8 +351,10 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > n) return;
   {
-    const int lo = s == 0 ? 0 : (int)keys[s - 1] + 1;
-    const int hi = s == n ? n_cells : (int)keys[s];
+    // keys are < n_cells by construction; the clamp keeps the writes inside
+    // start[] whatever they hold
+    const int lo = s == 0 ? 0 : (int)min(keys[s - 1], (uint32_t)n_cells) + 1;
+    const int hi = s == n ? n_cells : (int)min(keys[s], (uint32_t)n_cells);
     for (int c = lo; c <= hi; ++c) start[c] = s;
   }
   if (s == n) return;
