@@ -122,7 +122,8 @@ struct sre_engine {
   int n_cols = 0;                  // (x, y) cell columns over all replicas
   uint32_t *d_tiles_per_col = nullptr, *d_tile_off = nullptr, *d_tile_tmp = nullptr;
   int tile_cap = 0;
-  uint4 *d_pos4 = nullptr, *d_backup = nullptr;
+  uint4 *d_pos4 = nullptr, *d_backup = nullptr;  // backup: caller order
+  uint32_t *d_inv_perm = nullptr;  // caller order -> sorted slot
   float4 *d_res4 = nullptr;   // FP64 form only
   float *d_rcut = nullptr;
   int *d_tail_idx = nullptr;
@@ -196,6 +197,7 @@ sre_engine::~sre_engine() {
   dfree(sb.key_a); dfree(sb.val_a); dfree(sb.key_b); dfree(sb.val_b);
   dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
   dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
+  dfree(d_inv_perm);
   dfree(d_rcut); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
@@ -328,6 +330,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   const int N = e->n_atoms;
   dfree(e->d_rep_of); dfree(e->d_types); dfree(e->d_reps);
   dfree(e->d_pos4); dfree(e->d_backup); dfree(e->d_rcut); dfree(e->d_tail_idx);
+  dfree(e->d_inv_perm);
   dfree(e->d_res4);
   dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
   dfree(e->d_fxd); dfree(e->d_fyd); dfree(e->d_fzd); dfree(e->d_atom_stats);
@@ -341,6 +344,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->d_reps, n_replicas));
   HIP_TRY(dalloc(&e->d_pos4, N));
   HIP_TRY(dalloc(&e->d_backup, N));
+  HIP_TRY(dalloc(&e->d_inv_perm, N));
   HIP_TRY(dalloc(&e->d_rcut, N));
   HIP_TRY(dalloc(&e->d_tail_idx, N));
   HIP_TRY(dalloc(&e->d_nnb, N));
@@ -740,7 +744,8 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
   const void *ptrs[] = {e->d_pos, e->d_rep_of, e->d_reps, e->sb.key_a, e->sb.key_b,
                         e->sb.val_a, e->sb.val_b, e->sb.tkey, e->sb.tval,
                         e->sb.counts, e->sb.offsets, e->sb.scan_tmp, e->d_start,
-                        e->d_types, e->d_rc, e->d_pos4, e->d_backup, e->d_res4,
+                        e->d_types, e->d_rc, e->d_pos4, e->d_backup,
+                        e->d_inv_perm, e->d_res4,
                         e->d_rcut, e->d_tail_idx, e->d_tail_atom, e->d_paramsd,
                         e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
@@ -773,7 +778,7 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   launch_gather_sorted(N, e->n_cells, e->sb.key_a, e->sb.val_a, e->d_pos,
                        e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
                        e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
-                       e->d_res4, e->d_rcut, e->d_tail_idx, st);
+                       e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx, st);
   if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);
@@ -881,7 +886,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   if (!e) return fail(SRE_ERR_INVALID, "null engine");
   if (!e->built) return fail(SRE_ERR_NOT_READY, "update before the first rebuild");
   HIP_TRY(hipSetDevice(e->device));
-  launch_update_positions(e->n_atoms, e->sb.val_a, const_cast<float *>(e->d_pos),
+  launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
                           e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
                           e->d_res4, e->d_skin_part, nullptr, e->stream);
   HIP_TRY(hipGetLastError());
@@ -1483,7 +1488,7 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
                    e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
     sre_status s = SRE_OK;
     for (int k = 0; k < K && s == SRE_OK; ++k) {
-      launch_update_positions(e->n_atoms, e->sb.val_a,
+      launch_update_positions(e->n_atoms, e->d_inv_perm,
                               const_cast<float *>(e->d_pos), e->pos_stride,
                               e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
                               e->d_skin_part, pert ? &pa : nullptr, e->stream);

@@ -18,21 +18,22 @@ namespace sre {
 
 // ----------------------------------------------------- update + skin test --
 
-// Sorted slot s refreshes atom perm[s]: with a perturbation the new position
-// is written back to the bound array first (each atom is visited exactly
-// once).  The skin test keeps, per wavefront, the largest squared lattice
-// displacement from the rebuild-time position seen since the last rebuild;
-// each wave owns its slot of skin_part, so no atomics are involved.
+// Thread i refreshes atom i (caller order, so every load is coalesced and
+// independent): with a perturbation the new position is written back to the
+// bound array first; the lattice coordinates go to the atom's sorted slot
+// inv_perm[i].  The skin test compares with the rebuild-time lattice position
+// backup[i]; each wavefront keeps the largest squared displacement it has
+// seen since the last rebuild in its own slot of skin_part (no atomics).
 __global__ __launch_bounds__(256) void k_update_positions(
-    int n, const uint32_t *__restrict__ perm, float *__restrict__ pos,
+    int n, const uint32_t *__restrict__ inv_perm, float *__restrict__ pos,
     int stride, const RepInfo *__restrict__ reps, uint4 *__restrict__ pos4,
     const uint4 *__restrict__ backup, float4 *__restrict__ res4,
     float *__restrict__ skin_part, PerturbArgs pert) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
-  if (s < n) {
-    const uint4 b = backup[s];
-    const uint32_t i = perm[s];
+  if (i < n) {
+    const uint4 b = backup[i];
+    const uint32_t s = inv_perm[i];
     float *p = pos + (size_t)i * stride;
     float x, y, z;
     if (pert.step_ctr) {
@@ -64,12 +65,12 @@ __global__ __launch_bounds__(256) void k_update_positions(
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1)
     d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
-  const int w = s / kWave;  // slots exist for the waves holding atoms
+  const int w = i / kWave;  // slots exist for the waves holding atoms
   if ((threadIdx.x & (kWave - 1)) == 0 && w * kWave < n && d2 > skin_part[w])
     skin_part[w] = d2;
 }
 
-void launch_update_positions(int n, const uint32_t *perm, float *pos,
+void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
@@ -78,7 +79,7 @@ void launch_update_positions(int n, const uint32_t *perm, float *pos,
   PerturbArgs p{};
   if (pert) p = *pert;
   hipLaunchKernelGGL(k_update_positions, dim3((n + 255) / 256), dim3(256), 0,
-                     st, n, perm, pos, stride, reps, pos4, backup, res4,
+                     st, n, inv_perm, pos, stride, reps, pos4, backup, res4,
                      skin_part, p);
 }
 

@@ -336,10 +336,12 @@ int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
 
 // ------------------------------------------------------------- cell start --
 
-// One thread per sorted slot s (plus one): gathers atom perm[s] into cell
-// order -- lattice coordinates, type/replica word, rebuild-time backup, FP64
-// sub-lattice residual, rc from the table (the reference's AssignRCut rule)
-// and tail-table index -- and fills the cell offsets: start[c] = first sorted
+// One thread per sorted slot s (plus one): gathers atom i = perm[s] into
+// cell order -- lattice coordinates, type/replica word, FP64 sub-lattice
+// residual, rc from the table (the reference's AssignRCut rule) and
+// tail-table index -- records inv_perm[i] = s and the rebuild-time lattice
+// position backup[i] (caller order, read by the position refresh), and fills
+// the cell offsets: start[c] = first sorted
 // position with key >= c, thread s covering the cells from key[s-1] + 1 to
 // key[s] (thread n: the cells after the last key), so every entry of
 // start[0..n_cells] is written exactly once.
@@ -347,7 +349,8 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
     int n, int n_cells, const uint32_t *keys, const uint32_t *perm,
     const float *pos, int stride, const int *types, const int *rep_of,
     const RepInfo *reps, const float *rc_table, int n_types, int *start,
-    uint4 *pos4, uint4 *backup, float4 *res4, float *rcut, int *tail_idx) {
+    uint4 *pos4, uint4 *backup, uint32_t *inv_perm, float4 *res4, float *rcut,
+    int *tail_idx) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > n) return;
   {
@@ -368,7 +371,8 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
                        quantize(z, R.invq[2]),
                        (uint32_t)ty | ((uint32_t)r << 8));
   pos4[s] = v;
-  backup[s] = v;
+  backup[i] = v;
+  inv_perm[i] = (uint32_t)s;
   if (res4)
     res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
                           quant_residual(y, R.invq[1], R.qd[1]),
@@ -385,12 +389,12 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const int *types, const int *rep_of,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
-                          float4 *res4, float *rcut, int *tail_idx,
-                          hipStream_t st) {
+                          uint32_t *inv_perm, float4 *res4, float *rcut,
+                          int *tail_idx, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_sorted, dim3((n + 1 + 255) / 256), dim3(256), 0,
                      st, n, n_cells, keys, perm, pos, stride, types, rep_of,
-                     reps, rc_table, n_types, start, pos4, backup, res4, rcut,
-                     tail_idx);
+                     reps, rc_table, n_types, start, pos4, backup, inv_perm,
+                     res4, rcut, tail_idx);
 }
 
 // ------------------------------------------------------------------ fills --

@@ -179,8 +179,8 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const int *types, const int *rep_of,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
-                          float4 *res4, float *rcut, int *tail_idx,
-                          hipStream_t st);
+                          uint32_t *inv_perm, float4 *res4, float *rcut,
+                          int *tail_idx, hipStream_t st);
 // Both builders write one (longest list, total length) pair per wavefront to
 // part[] (build_part_count entries, every one written by every launch);
 // launch_rebuild_info reduces them.
@@ -221,7 +221,8 @@ struct PerturbArgs {
 // displacement it has seen since the last rebuild in skin_part[wave]
 // (skin_part_count(n) entries).
 inline int skin_part_count(int n) { return (n + kWave - 1) / kWave; }
-void launch_update_positions(int n, const uint32_t *perm, float *pos,
+// backup and inv_perm are in caller order (written by the sorted gather)
+void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
