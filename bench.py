@@ -42,7 +42,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(w, precision, steps, warmup, every, torch, Engine):
+def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
     """Times `steps` perturb+update+force steps; returns a dict of figures."""
     stream = torch.cuda.Stream()          # default priority
     e = Engine(n_types=w.n_types, skin=w.skin, precision=precision,
@@ -52,6 +52,11 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
     torch.cuda.synchronize()
     e.bind_positions(pos)
     e.rebuild()
+    tune_s = None
+    if autotune:
+        t0 = time.perf_counter()
+        e.autotune(stats=True, tail=True)
+        tune_s = time.perf_counter() - t0
     e.set_perturbation(PERTURB_SIGMA, SEED, 0, anchored=True)
 
     def segment_loop(n_steps):
@@ -97,7 +102,7 @@ def run_form(w, precision, steps, warmup, every, torch, Engine):
     nl, nin, mx = e.neighbor_count()
     return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs,
                 rebuilds=rebuilds, force_ms=k_ms, n_list=nl, n_in_rc=nin, max_list=mx,
-                force_us_dist=dist, rebuild_ms=rb_ms,
+                force_us_dist=dist, rebuild_ms=rb_ms, autotune_s=tune_s,
                 pair_evals_per_s=pairs / wall, ms_per_step=1e3 * wall / steps)
 
 
@@ -167,6 +172,27 @@ def _cpu_rate(w, budget_s, max_evals=2000):
     return L / t_force, t_force * 1e3, t_build * 1e3, L, n_eval
 
 
+def time_adjoint(e, w, torch, reps=5):
+    """The adjoint of E and of a force-matching loss (dLdE = 1 per replica,
+    dLdF = seeded Gaussian device tensor) w.r.t. eps, sigma and the rc
+    lattice: median wall ms per call (includes the host readback of the
+    gradients, which the call returns)."""
+    import numpy as np
+    g = torch.Generator(device="cpu").manual_seed(SEED)
+    dLdF = torch.randn(w.n, 3, generator=g, dtype=torch.float64).to("cuda")
+    n_rc = int(sum(t.size for t in w.rc_tables))
+    torch.cuda.synchronize()
+    e.adjoint(dLdE=1.0, dLdF=dLdF, tail=True, n_rc=n_rc)      # warm-up
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        e.adjoint(dLdE=1.0, dLdF=dLdF, tail=True, n_rc=n_rc)
+        t.append(time.perf_counter() - t0)
+    return {"ms": round(1e3 * float(np.median(t)), 3), "n_rc_bins": n_rc,
+            "what": "gradients of E and of sum_i dLdF_i . F_i w.r.t. eps, sigma "
+                    "(n_types^2 each) and every rc bin, pair terms recomputed"}
+
+
 def cpu_baseline(w, budget_s, every, configs):
     """The FP64 oracle (C++/OpenMP) on the host: all threads and one thread,
     on the benchmark workload and on the 4k FCC plain-method case."""
@@ -223,7 +249,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--workload", default="slab_64k",
-                    help="slab_64k (the metric's workload), binary_slab_64k, large_slab_2M")
+                    help="slab_64k (the metric's workload), binary_slab_64k, large_slab_2M, "
+                         "replicas_256, plain_fcc_4k")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick tile/block/cell sizes with sre_autotune before timing")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint (parameter gradients) of the mixed form")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -251,12 +282,14 @@ def main():
         return
     w = configs.WORKLOADS[args.workload]()
     log(f"[bench] workload {w.name}: N={w.n}, {w.notes}")
-    mixed = run_form(w, "mixed", args.steps, args.warmup, args.rebuild_every, torch, Engine)
+    mixed = run_form(w, "mixed", args.steps, args.warmup, args.rebuild_every, torch, Engine,
+                     autotune=args.autotune)
     log(f"[bench] mixed: {mixed['pair_evals_per_s']:.4g} pair evals/s, "
         f"{mixed['ms_per_step']:.4f} ms/step, force kernel {mixed['force_ms'] * 1e3:.2f} us")
     fp64 = None
     if not args.mixed_only:
-        fp64 = run_form(w, "fp64", args.steps, args.warmup, args.rebuild_every, torch, Engine)
+        fp64 = run_form(w, "fp64", args.steps, args.warmup, args.rebuild_every, torch, Engine,
+                        autotune=args.autotune)
         log(f"[bench] fp64:  {fp64['pair_evals_per_s']:.4g} pair evals/s, "
             f"{fp64['ms_per_step']:.4f} ms/step, force kernel {fp64['force_ms'] * 1e3:.2f} us")
     acc = accuracy(w, torch, Engine)
@@ -275,7 +308,7 @@ def main():
                              "energy/virial 32 written)",
                 "avg_launch_us": round(mixed["force_ms"] * 1e3, 3)}
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and len(w.n_atoms) == 1:
         cpu = cpu_baseline(w, args.cpu_seconds, args.rebuild_every, configs)
         log(f"[bench] cpu baseline: {cpu['value']:.4g} pair evals/s on {cpu['cores']} threads")
     result = {
@@ -293,6 +326,7 @@ def main():
         "data": "synthetic (seeded random-sequential-addition slab)",
         "config": {"workload": w.name, "n_atoms": N,
                    "box": [float(v) for v in w.boxes[0]], "n_types": w.n_types,
+                   "n_replicas": len(w.n_atoms),
                    "rho_liquid_vapour": [0.73, 0.02], "geometry": w.notes,
                    "skin": w.skin, "rebuild_every": args.rebuild_every,
                    "motion": "vibration about the initial sites", "perturbation_sigma": PERTURB_SIGMA,
@@ -308,6 +342,10 @@ def main():
         "accuracy_vs_fp64": acc,
         "cpu_baseline": cpu,
     }
+    if args.autotune:
+        result["autotune_s"] = round(mixed["autotune_s"], 2)
+    if args.adjoint:
+        result["adjoint"] = time_adjoint(mixed["engine"], w, torch)
     if fp64 is not None:
         result["fp64_form"] = {
             "value": round(fp64["pair_evals_per_s"], 1), "ms_per_step": round(fp64["ms_per_step"], 5),
