@@ -113,20 +113,23 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_pairs(AdjointArgs a) {
     }
   }
   __syncthreads();
-  // every launched wave writes its slot (zeros beyond the atoms)
+  // every launched wave writes its slot (zeros beyond the atoms); value-major
+  // so that the reduction below reads each value's row contiguously
+  const size_t n_waves = (size_t)gridDim.x * (blockDim.x / kWave);
   for (int k = threadIdx.x % kWave; k < nv; k += kWave)
-    a.wave_part[(size_t)wave * nv + k] = (double)(long long)wacc[k] * kGradInv;
+    a.wave_part[(size_t)k * n_waves + wave] = (double)(long long)wacc[k] * kGradInv;
 }
 
-// Fixed-order sum over waves: one thread per output value.
+// Fixed-order sum over waves: one block per output value, coalesced reads
+// of the value's row.
 __global__ __launch_bounds__(256) void k_adjoint_reduce(int n_waves, int n_vals,
                                                         const double *wave_part,
                                                         double *out) {
   __shared__ double red[256];
   const int v = blockIdx.x;
   double s = 0.0;
-  for (int w = threadIdx.x; w < n_waves; w += blockDim.x)
-    s += wave_part[(size_t)w * n_vals + v];
+  const double *row = wave_part + (size_t)v * n_waves;
+  for (int w = threadIdx.x; w < n_waves; w += blockDim.x) s += row[w];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
@@ -141,6 +144,8 @@ __global__ __launch_bounds__(256) void k_adjoint_reduce(int n_waves, int n_vals,
 // Geometric integrals of one species profile: F = sum over bins/images of
 // rho*(G(s1)-G(s0)) split into the sigma^6 and sigma^12 parts, likewise the
 // energy integral of P, and their derivatives with respect to R.
+constexpr int kTailAdjBlock = 128;  // threads per tail bin in the adjoint
+
 struct TailX {
   double F6, F12, E6, E12, dF6, dF12, dE6, dE12;
 };
@@ -182,38 +187,49 @@ __device__ inline void v_parts(double a, double R, double *v6, double *v12,
   }
 }
 
-__device__ TailX tail_x(double z, double R, const double *rb, int nz,
-                        double Lz) {
+// Sums of the planar tail integrals and their rc derivatives over the
+// (image, source bin) pairs idx = t0, t0 + stride, ... (acc[0..7] in TailX
+// order) and of the density over bins t0, t0 + stride, ... (acc[8]); a block
+// splits the work over its threads and tail_x_finish adds the mean-density
+// remainder outside the image window.
+__device__ void tail_x_part(double z, double R, const double *rb, int nz,
+                            double Lz, int t0, int stride, double acc[9]) {
   const int M = max(1, (int)ceil(30.0 / Lz));
   const double w = Lz / nz;
   const double two_pi = 6.283185307179586;
-  TailX x{0, 0, 0, 0, 0, 0, 0, 0};
-  double rbar = 0.0;
-  for (int k = 0; k < nz; ++k) rbar += rb[k];
-  rbar /= nz;
-  for (int img = -M; img <= M; ++img)
-    for (int k = 0; k < nz; ++k) {
-      const double r = rb[k];
-      if (r == 0.0) continue;
-      const double s0 = k * w + img * Lz - z, s1 = s0 + w;
-      double a6, a12, da6, da12, b6, b12, db6, db12;
-      g_parts(s1, R, &a6, &a12, &da6, &da12);
-      g_parts(s0, R, &b6, &b12, &db6, &db12);
-      x.F6 += r * (a6 - b6);
-      x.F12 += r * (a12 - b12);
-      x.dF6 += r * (da6 - db6);
-      x.dF12 += r * (da12 - db12);
-      double v6, v12, dv6, dv12, u6, u12, du6, du12;
-      v_parts(fabs(s1), R, &v6, &v12, &dv6, &dv12);
-      v_parts(fabs(s0), R, &u6, &u12, &du6, &du12);
-      const double g1 = s1 < 0.0 ? two_pi : -two_pi;
-      const double g0 = s0 < 0.0 ? two_pi : -two_pi;
-      x.E6 += r * (g1 * v6 - g0 * u6);
-      x.E12 += r * (g1 * v12 - g0 * u12);
-      x.dE6 += r * (g1 * dv6 - g0 * du6);
-      x.dE12 += r * (g1 * dv12 - g0 * du12);
-    }
-  // outside the image window: mean density
+  for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+  const int total = (2 * M + 1) * nz;
+  for (int idx = t0; idx < total; idx += stride) {
+    const int img = idx / nz - M, k = idx - (idx / nz) * nz;
+    const double r = rb[k];
+    if (r == 0.0) continue;
+    const double s0 = k * w + img * Lz - z, s1 = s0 + w;
+    double a6, a12, da6, da12, b6, b12, db6, db12;
+    g_parts(s1, R, &a6, &a12, &da6, &da12);
+    g_parts(s0, R, &b6, &b12, &db6, &db12);
+    acc[0] += r * (a6 - b6);
+    acc[1] += r * (a12 - b12);
+    acc[4] += r * (da6 - db6);
+    acc[5] += r * (da12 - db12);
+    double v6, v12, dv6, dv12, u6, u12, du6, du12;
+    v_parts(fabs(s1), R, &v6, &v12, &dv6, &dv12);
+    v_parts(fabs(s0), R, &u6, &u12, &du6, &du12);
+    const double g1 = s1 < 0.0 ? two_pi : -two_pi;
+    const double g0 = s0 < 0.0 ? two_pi : -two_pi;
+    acc[2] += r * (g1 * v6 - g0 * u6);
+    acc[3] += r * (g1 * v12 - g0 * u12);
+    acc[6] += r * (g1 * dv6 - g0 * du6);
+    acc[7] += r * (g1 * dv12 - g0 * du12);
+  }
+  for (int k = t0; k < nz; k += stride) acc[8] += rb[k];
+}
+
+__device__ TailX tail_x_finish(double z, double R, int nz, double Lz,
+                               const double acc[9]) {
+  const int M = max(1, (int)ceil(30.0 / Lz));
+  const double two_pi = 6.283185307179586;
+  TailX x{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7]};
+  const double rbar = acc[8] / nz;
   const double sL = -M * Lz - z, sU = (M + 1) * Lz - z;
   double a6, a12, da6, da12, b6, b12, db6, db12;
   g_parts(sL, R, &a6, &a12, &da6, &da12);
@@ -235,16 +251,18 @@ __device__ TailX tail_x(double z, double R, const double *rb, int nz,
   return x;
 }
 
-// One thread per tail bin (over all replicas): combines the per-entry weights
-// W_F = sum g_z and W_E = sum dLdE with the derivatives of the table entries.
-// Writes d/drc of the bin to g_rc (fixed point) and the eps/sigma parts to
+// One block per tail bin (over all replicas), its threads splitting the
+// planar integrals (fixed-order tree reduction): combines the per-entry
+// weights W_F = sum g_z and W_E = sum dLdE with the derivatives of the table
+// entries.  Writes d/drc of the bin to g_rc and the eps/sigma parts to
 // contrib[bin][2*npt].
-__global__ __launch_bounds__(64) void k_adjoint_tail(
+__global__ __launch_bounds__(kTailAdjBlock) void k_adjoint_tail(
     int n_bins, const int *entry_rep, const RepInfo *reps, int n_types,
     const PairParamD *params, const float *rc_table, const double *rho,
     const int64_t *wts, double *contrib, double *g_rc) {
-  const int bg = blockIdx.x * blockDim.x + threadIdx.x;
-  if (bg >= n_bins) return;
+  __shared__ double red[9][kTailAdjBlock];
+  const int bg = blockIdx.x;
+  const int t = threadIdx.x;
   const int npt = n_types * n_types;
   const int r = entry_rep[bg];
   const RepInfo &R = reps[r];
@@ -254,32 +272,46 @@ __global__ __launch_bounds__(64) void k_adjoint_tail(
   const double z = (bin + 0.5) * Lz / nz;
   const double rc = (double)rc_table[R.rc_off + bin];
   double *c = contrib + (size_t)bg * 2 * npt;
-  for (int k = 0; k < 2 * npt; ++k) c[k] = 0.0;
+  if (t == 0)
+    for (int k = 0; k < 2 * npt; ++k) c[k] = 0.0;
   double grc = 0.0;
   for (int ta = 0; ta < n_types; ++ta) {
     const int te = bg * n_types + ta;
     const double WF = (double)wts[2 * (size_t)te] * kFixInv;
     const double WE = (double)wts[2 * (size_t)te + 1] * kFixInv;
-    if (WF == 0.0 && WE == 0.0) continue;
+    if (WF == 0.0 && WE == 0.0) continue;  // block-uniform
     for (int tb = 0; tb < n_types; ++tb) {
-      const PairParamD &P = params[ta * n_types + tb];
-      const double sg = P.sig, ep = P.eps;
-      const double s2 = sg * sg, s6 = s2 * s2 * s2, s12 = s6 * s6;
-      const TailX x = tail_x(z, rc, rho + R.rho_off + (size_t)tb * R.rho_nz,
-                             R.rho_nz, Lz);
-      // T_F = -2 pi eps (s6 F6 + s12 F12),  T_E = eps/2 (s6 E6 + s12 E12)
-      const double cF = -6.283185307179586, cE = 0.5;
-      const double tF = s6 * x.F6 + s12 * x.F12, tE = s6 * x.E6 + s12 * x.E12;
-      const double ds6 = 6.0 * s6 / sg, ds12 = 12.0 * s12 / sg;
-      const int p = ta * n_types + tb;
-      c[p] += WF * cF * tF + WE * cE * tE;
-      c[npt + p] += ep * (WF * cF * (ds6 * x.F6 + ds12 * x.F12) +
-                          WE * cE * (ds6 * x.E6 + ds12 * x.E12));
-      grc += ep * (WF * cF * (s6 * x.dF6 + s12 * x.dF12) +
-                   WE * cE * (s6 * x.dE6 + s12 * x.dE12));
+      double acc[9];
+      tail_x_part(z, rc, rho + R.rho_off + (size_t)tb * R.rho_nz, R.rho_nz, Lz,
+                  t, kTailAdjBlock, acc);
+      for (int k = 0; k < 9; ++k) red[k][t] = acc[k];
+      __syncthreads();
+      for (int o = kTailAdjBlock / 2; o > 0; o >>= 1) {
+        if (t < o)
+          for (int k = 0; k < 9; ++k) red[k][t] += red[k][t + o];
+        __syncthreads();
+      }
+      if (t == 0) {
+        for (int k = 0; k < 9; ++k) acc[k] = red[k][0];
+        const TailX x = tail_x_finish(z, rc, R.rho_nz, Lz, acc);
+        const PairParamD &P = params[ta * n_types + tb];
+        const double sg = P.sig, ep = P.eps;
+        const double s2 = sg * sg, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        // T_F = -2 pi eps (s6 F6 + s12 F12),  T_E = eps/2 (s6 E6 + s12 E12)
+        const double cF = -6.283185307179586, cE = 0.5;
+        const double tF = s6 * x.F6 + s12 * x.F12, tE = s6 * x.E6 + s12 * x.E12;
+        const double ds6 = 6.0 * s6 / sg, ds12 = 12.0 * s12 / sg;
+        const int p = ta * n_types + tb;
+        c[p] += WF * cF * tF + WE * cE * tE;
+        c[npt + p] += ep * (WF * cF * (ds6 * x.F6 + ds12 * x.F12) +
+                            WE * cE * (ds6 * x.E6 + ds12 * x.E12));
+        grc += ep * (WF * cF * (s6 * x.dF6 + s12 * x.dF12) +
+                     WE * cE * (s6 * x.dE6 + s12 * x.dE12));
+      }
+      __syncthreads();  // red[] reused
     }
   }
-  g_rc[R.rc_off + bin] = grc;
+  if (t == 0) g_rc[R.rc_off + bin] = grc;
 }
 
 __global__ __launch_bounds__(256) void k_adjoint_tail_reduce(
@@ -340,7 +372,7 @@ void launch_adjoint_tail(int n_bins, const int *entry_rep, const RepInfo *reps,
                          double *out, hipStream_t st) {
   if (n_bins <= 0) return;
   const int npt = n_types * n_types;
-  hipLaunchKernelGGL(k_adjoint_tail, dim3((n_bins + 63) / 64), dim3(64), 0, st,
+  hipLaunchKernelGGL(k_adjoint_tail, dim3(n_bins), dim3(kTailAdjBlock), 0, st,
                      n_bins, entry_rep, reps, n_types, params, rc_table, rho,
                      tail_w, contrib, g_rc);
   hipLaunchKernelGGL(k_adjoint_tail_reduce, dim3(2 * npt), dim3(256), 0, st,

@@ -302,7 +302,7 @@ struct AdjointArgs {
   const double *dLdF;      // caller order [N][3] or NULL
   const int *tail_idx;
   int use_tail;
-  double *wave_part;       // per launched wave: 2 * n_types^2 doubles
+  double *wave_part;       // value-major: [2 * n_types^2][launched waves]
   int64_t *tail_w;         // per tail entry: (sum g_z, sum dLdE), scale 2^32
 };
 int adjoint_grid_waves(int n, int lanes);
