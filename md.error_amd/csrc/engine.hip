@@ -170,6 +170,10 @@ struct sre_engine {
   bool built = false;
   bool system_set = false;
   int graph_epoch = 0;
+  double *d_adj = nullptr;         // adjoint workspace (grown on demand)
+  size_t adj_cap = 0;
+  int64_t *d_adj_tw = nullptr;
+  size_t adj_tw_cap = 0;
   // step graphs: K steps of [refresh + force] each, cached by (flags, K)
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
@@ -205,6 +209,7 @@ sre_engine::~sre_engine() {
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
+  dfree(d_adj); dfree(d_adj_tw);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1173,68 +1178,64 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
   const int gw = adjoint_grid_waves(e->n_atoms, e->list_lanes);
   const int n_entries = std::max(1, e->tail_bins * nt);
   const int rc_n = std::max(1, e->rc_total);
-  double *d_wp = nullptr, *d_dLdE = nullptr, *d_out = nullptr,
-         *d_tout = nullptr, *d_contrib = nullptr, *d_grc = nullptr;
-  int64_t *d_tw = nullptr;
+  // one persistent workspace: [wave partials | dLdE | out | tout | g_rc |
+  // tail contributions] doubles + tail weights (fixed point); the results
+  // (out, tout, g_rc) are contiguous for one readback
+  const size_t n_wp = (size_t)gw * 2 * npt, n_res = (size_t)4 * npt + rc_n;
+  const size_t n_ctb = (size_t)std::max(1, e->tail_bins) * 2 * npt;
+  const size_t need = n_wp + e->n_rep + n_res + n_ctb;
+  if (e->adj_cap < need) {
+    dfree(e->d_adj);
+    HIP_TRY(dalloc(&e->d_adj, need));
+    e->adj_cap = need;
+  }
+  if (e->adj_tw_cap < (size_t)2 * n_entries) {
+    dfree(e->d_adj_tw);
+    HIP_TRY(dalloc(&e->d_adj_tw, (size_t)2 * n_entries));
+    e->adj_tw_cap = (size_t)2 * n_entries;
+  }
+  double *d_wp = e->d_adj, *d_dLdE = d_wp + n_wp, *d_out = d_dLdE + e->n_rep;
+  double *d_tout = d_out + 2 * npt, *d_grc = d_tout + 2 * npt;
+  double *d_contrib = d_grc + rc_n;
+  int64_t *d_tw = e->d_adj_tw;
   sre_status s = SRE_OK;
-  hipError_t r = dalloc(&d_wp, (size_t)gw * 2 * npt);
-  if (r == hipSuccess) r = dalloc(&d_dLdE, e->n_rep);
-  if (r == hipSuccess) r = dalloc(&d_out, 2 * npt);
-  if (r == hipSuccess) r = dalloc(&d_tout, 2 * npt);
-  if (r == hipSuccess) r = dalloc(&d_contrib, (size_t)std::max(1, e->tail_bins) * 2 * npt);
-  if (r == hipSuccess) r = dalloc(&d_grc, rc_n);
-  if (r == hipSuccess) r = dalloc(&d_tw, (size_t)2 * n_entries);
   std::vector<double> hE(e->n_rep, 0.0);
   if (dLdE) hE.assign(dLdE, dLdE + e->n_rep);
-  std::vector<double> zero(std::max(rc_n, 2 * npt), 0.0);
-  if (r == hipSuccess)
-    r = hipMemcpyAsync(d_dLdE, hE.data(), sizeof(double) * e->n_rep,
-                       hipMemcpyHostToDevice, e->stream);
-  if (r == hipSuccess)
-    r = hipMemcpyAsync(d_grc, zero.data(), sizeof(double) * rc_n,
-                       hipMemcpyHostToDevice, e->stream);
-  if (r == hipSuccess)
-    r = hipMemcpyAsync(d_tout, zero.data(), sizeof(double) * 2 * npt,
-                       hipMemcpyHostToDevice, e->stream);
-  std::vector<double> out(2 * npt), tout(2 * npt), grc(rc_n);
-  if (r == hipSuccess) {
-    launch_fill_i64(d_tw, 0, (size_t)2 * n_entries, e->stream);
-    AdjointArgs a{};
-    a.n = e->n_atoms;
-    a.n_types = nt;
-    a.cap = e->cap;
-    a.n_rep = e->n_rep;
-    a.pos4 = e->d_pos4;
-    a.rcut = e->d_rcut;
-    a.nnb = e->d_nnb;
-    a.list = e->d_list;
-    a.perm = e->sb.val_a;
-    a.reps = e->d_reps;
-    a.params = e->d_params;
-    a.dLdE = d_dLdE;
-    a.dLdF = d_dLdF;
-    a.tail_idx = e->d_tail_idx;
-    a.use_tail = use_tail ? 1 : 0;
-    a.wave_part = d_wp;
-    a.tail_w = d_tw;
-    launch_adjoint_pairs(a, e->precision, e->list_lanes, d_out, e->stream);
-    if (use_tail)
-      launch_adjoint_tail(e->tail_bins, e->d_tail_entry_rep, e->d_reps, nt,
-                          e->d_paramsd, e->d_rc, e->d_rho, d_tw, d_contrib,
-                          d_grc, d_tout, e->stream);
-    r = hipMemcpyAsync(out.data(), d_out, sizeof(double) * 2 * npt,
-                       hipMemcpyDeviceToHost, e->stream);
-  }
-  if (r == hipSuccess)
-    r = hipMemcpyAsync(tout.data(), d_tout, sizeof(double) * 2 * npt,
-                       hipMemcpyDeviceToHost, e->stream);
-  if (r == hipSuccess)
-    r = hipMemcpyAsync(grc.data(), d_grc, sizeof(double) * rc_n,
-                       hipMemcpyDeviceToHost, e->stream);
-  if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
-  if (r != hipSuccess) s = fail(SRE_ERR_HIP, "adjoint: %s", hipGetErrorString(r));
-  dfree(d_wp); dfree(d_dLdE); dfree(d_out); dfree(d_tout); dfree(d_contrib);
-  dfree(d_grc); dfree(d_tw);
+  HIP_TRY(hipMemcpyAsync(d_dLdE, hE.data(), sizeof(double) * e->n_rep,
+                         hipMemcpyHostToDevice, e->stream));
+  // zero tout and g_rc (all-zero bits), and the tail weights
+  launch_fill_i64(reinterpret_cast<int64_t *>(d_tout), 0, (size_t)2 * npt + rc_n,
+                  e->stream);
+  launch_fill_i64(d_tw, 0, (size_t)2 * n_entries, e->stream);
+  AdjointArgs a{};
+  a.n = e->n_atoms;
+  a.n_types = nt;
+  a.cap = e->cap;
+  a.n_rep = e->n_rep;
+  a.pos4 = e->d_pos4;
+  a.rcut = e->d_rcut;
+  a.nnb = e->d_nnb;
+  a.list = e->d_list;
+  a.perm = e->sb.val_a;
+  a.reps = e->d_reps;
+  a.params = e->d_params;
+  a.dLdE = d_dLdE;
+  a.dLdF = d_dLdF;
+  a.tail_idx = e->d_tail_idx;
+  a.use_tail = use_tail ? 1 : 0;
+  a.wave_part = d_wp;
+  a.tail_w = d_tw;
+  launch_adjoint_pairs(a, e->precision, e->list_lanes, d_out, e->stream);
+  if (use_tail)
+    launch_adjoint_tail(e->tail_bins, e->d_tail_entry_rep, e->d_reps, nt,
+                        e->d_paramsd, e->d_rc, e->d_rho, d_tw, d_contrib, d_grc,
+                        d_tout, e->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> res(n_res);
+  HIP_TRY(hipMemcpyAsync(res.data(), d_out, sizeof(double) * n_res,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  const double *out = res.data(), *tout = out + 2 * npt, *grc = tout + 2 * npt;
   if (s != SRE_OK) return s;
   // Values are per ordered type pair; a symmetric parameter (a,b) = (b,a)
   // collects both orders.
