@@ -261,6 +261,51 @@ sre_status sre_time_compute(sre_engine *e, unsigned flags, int n_iter,
 sre_status sre_time_rebuild(sre_engine *e, int n_iter, double *ms_per_iter);
 sre_status sre_synchronize(sre_engine *e);
 
+/* ---- MD integrator ------------------------------------------------------- */
+/* The MD loop around the force path (H/app/lj.nhc-xtc.cu:158-196), device-
+ * resident: per step the centre-of-mass velocity removal
+ * (TranslationalFreedomRemover::remove, H/src/Integrator_interface.cu:40-68),
+ * a half step of the two-link Nose-Hoover chain (NoseHoover_Chains2::
+ * operator_L(dt/2), H/src/Thermostat.cu:255-276), velocity Verlet part 1
+ * (H/src/Integrator.cu:273-324), the force, velocity Verlet part 2
+ * (H/src/Integrator.cu:327-347) and the second chain half step.  Positions
+ * are the bound ones (wrapped into [0, L) as they move), velocities and
+ * masses are held by the engine, in caller order, FP32 as in the reference;
+ * the chain variables and the kinetic-energy sums are FP64.  One replica. */
+#define SRE_THERMO_NONE 0   /* NVE */
+#define SRE_THERMO_NHC2 1   /* two-link Nose-Hoover chain */
+typedef struct sre_md_state {
+  double kinetic;       /* sum m v^2 / 2 at the end of the last step */
+  double xi1, xi2;      /* chain positions (start at 1) */
+  double vxi1, vxi2;    /* chain velocities (start at 0) */
+  double nhc_energy;    /* HamiltonianContribution: Q1 vxi1^2/2 + Q2 vxi2^2/2
+                           + 3N T xi1 + T xi2 (0 without thermostat) */
+  double Q1, Q2;        /* chain masses, tau^2 T 3N / (4 pi^2) and Q1 / 3N */
+} sre_md_state;
+/* Per-type masses, n_types values > 0 (default 1). */
+sre_status sre_set_masses(sre_engine *e, const float *mass);
+/* Velocities v[3N] (host, caller order; xyz per atom). */
+sre_status sre_set_velocities(sre_engine *e, const float *v);
+sre_status sre_get_velocities(sre_engine *e, float *v);
+/* Time step, thermostat (SRE_THERMO_*; NoseHoover_Chains2::reinit with
+ * reference temperature ref_T and period tau_T), remove_com 1 = remove the
+ * centre-of-mass velocity at the start of every step.  Resets the chain. */
+sre_status sre_set_integrator(sre_engine *e, double dt, int thermostat,
+                              double ref_T, double tau_T, int remove_com);
+/* n_steps MD steps.  rebuild_every > 0: the lists are rebuilt every that many
+ * steps and the steps in between run as captured graphs; SRE_ERR_SKIN if an
+ * atom moved more than skin/2 between two rebuilds (the forces of that
+ * stretch may have missed pairs).  rebuild_every = 0: the reference's rule,
+ * a rebuild whenever the largest displacement since the last one exceeds
+ * skin/2 (one host synchronisation per step).  flags: SRE_TAIL adds the tail
+ * force every step; SRE_STATS computes energy/virial on the last step (read
+ * by sre_get_stats).  Forces are recomputed at the start of every call. */
+sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
+                      unsigned flags, int *n_rebuilds);
+sre_status sre_get_md_state(sre_engine *e, sre_md_state *out);
+/* Restores the chain positions and velocities (checkpoint restart). */
+sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in);
+
 /* ---- synthetic configurations (host only, no GPU use) --------------------- */
 
 /* FCC lattice of ncell^3 unit cells with lattice constant a, every coordinate

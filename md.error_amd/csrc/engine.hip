@@ -60,6 +60,9 @@ static void dfree(T *&p) {
 
 constexpr int kStepGraphSlots = 4;   // cached step graphs (flags, step count)
 constexpr int kMaxGraphSteps = 64;   // steps per captured graph
+// internal step-graph flag: the K steps are MD steps (thermostat factors,
+// kick + drift refresh, force, kick) and SRE_STATS applies to the last only
+constexpr unsigned kGraphMD = 1u << 16;
 
 struct sre_engine {
   int device = 0;
@@ -79,6 +82,7 @@ struct sre_engine {
   RepInfo *d_reps = nullptr;
   int *d_rep_of = nullptr;
   int *d_types = nullptr;
+  std::vector<int> h_types;
   bool types_set = false;
 
   // force field
@@ -188,6 +192,17 @@ struct sre_engine {
   hipGraphExec_t rebuild_exec = nullptr;
   std::vector<unsigned char> rebuild_sig;
 
+  // MD integrator (caller order, one replica)
+  std::vector<float> mass;         // per type (default 1)
+  float *d_mass = nullptr;
+  float *d_vel = nullptr;          // 3 per atom
+  float *d_fcall = nullptr;        // force of the last kick, 3 per atom
+  double *d_md_part = nullptr;     // kick partial sums (K, Px, Py, Pz) per block
+  MDState *d_md = nullptr;
+  MDState md{};                    // host copy of the parameters
+  bool vel_set = false, md_set = false;
+  int md_since_rebuild = 0;        // MD steps since the last rebuild
+
   ~sre_engine();
 };
 
@@ -210,6 +225,7 @@ sre_engine::~sre_engine() {
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   dfree(d_adj); dfree(d_adj_tw);
+  dfree(d_mass); dfree(d_vel); dfree(d_fcall); dfree(d_md_part); dfree(d_md);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -342,6 +358,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_stats); dfree(e->d_nin); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
   dfree(e->d_stat_part);
   dfree(e->d_skin_part);
+  dfree(e->d_vel); dfree(e->d_fcall); dfree(e->d_md_part);
+  e->vel_set = false;
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
   HIP_TRY(dalloc(&e->d_rep_of, N));
@@ -394,6 +412,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   return SRE_OK;
 }
 
+static sre_status md_update_total_mass(sre_engine *e);
+
 extern "C" sre_status sre_set_types(sre_engine *e, const int32_t *types) {
   if (!e || !types || !e->system_set)
     return fail(SRE_ERR_INVALID, "sre_set_types: system not set");
@@ -404,8 +424,10 @@ extern "C" sre_status sre_set_types(sre_engine *e, const int32_t *types) {
   HIP_TRY(hipSetDevice(e->device));
   HIP_TRY(hipMemcpy(e->d_types, types, sizeof(int) * e->n_atoms,
                     hipMemcpyHostToDevice));
+  e->h_types.assign(types, types + e->n_atoms);
   e->types_set = true;
   e->built = false;
+  if (e->md_set) return md_update_total_mass(e);
   return SRE_OK;
 }
 
@@ -878,6 +900,7 @@ static sre_status do_rebuild(sre_engine *e) {
     HIP_TRY(hipStreamSynchronize(st));
   }
   e->built = true;
+  e->md_since_rebuild = 0;
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   return SRE_OK;
 }
@@ -893,7 +916,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   HIP_TRY(hipSetDevice(e->device));
   launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
                           e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
-                          e->d_res4, e->d_skin_part, nullptr, e->stream);
+                          e->d_res4, e->d_skin_part, nullptr, nullptr, e->stream);
   HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
@@ -1463,6 +1486,31 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
 
 // ----------------------------------------------------------------- graphs --
 
+// MD step pieces (k_md.hip): the factors of the step from the last kick's
+// sums, then kick + drift + refresh, and after the force the second kick
+static void enqueue_md_drift(sre_engine *e) {
+  launch_md_factors(e->n_atoms, e->d_md_part, e->d_md, e->stream);
+  DriftArgs d{e->d_vel, e->d_fcall, e->d_mass, e->d_types, e->d_md,
+              (float)e->md.dt};
+  launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
+                          e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
+                          e->d_res4, e->d_skin_part, nullptr, &d, e->stream);
+}
+
+static void enqueue_md_kick(sre_engine *e, float hdt) {
+  launch_md_kick(e->n_atoms, e->d_inv_perm, e->d_types, e->d_fx, e->d_fy,
+                 e->d_fz, e->d_fxd, e->d_fyd, e->d_fzd, e->d_mass, hdt,
+                 e->d_vel, e->d_fcall, e->d_md_part, e->stream);
+}
+
+static sre_status enqueue_md_step(sre_engine *e, unsigned flags) {
+  enqueue_md_drift(e);
+  sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL), nullptr);
+  if (s != SRE_OK) return s;
+  enqueue_md_kick(e, (float)(0.5 * e->md.dt));
+  return SRE_OK;
+}
+
 // Captures K steps of [refresh (+ perturbation) + force] into one graph, or
 // returns the cached one; the graph reads the engine's buffers by address, so
 // every reallocation or layout change bumps graph_epoch and invalidates it.
@@ -1487,12 +1535,18 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     PerturbArgs pa{e->pert_sigma, e->pert_seed,
                    e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
+    const bool md = (flags & kGraphMD) != 0;
     sre_status s = SRE_OK;
     for (int k = 0; k < K && s == SRE_OK; ++k) {
+      if (md) {
+        s = enqueue_md_step(e, k == K - 1 ? flags : flags & ~SRE_STATS);
+        continue;
+      }
       launch_update_positions(e->n_atoms, e->d_inv_perm,
                               const_cast<float *>(e->d_pos), e->pos_stride,
                               e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
-                              e->d_skin_part, pert ? &pa : nullptr, e->stream);
+                              e->d_skin_part, pert ? &pa : nullptr, nullptr,
+                              e->stream);
       s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL),
                          pert ? e->d_step_ctr : nullptr);
     }
@@ -1612,5 +1666,306 @@ extern "C" sre_status sre_set_perturbation(sre_engine *e, float sigma,
   if (e->pert_sigma != sigma || e->pert_seed != seed) e->graph_epoch++;
   e->pert_sigma = sigma;
   e->pert_seed = seed;
+  return SRE_OK;
+}
+
+// ------------------------------------------------------------- integrator --
+
+static sre_status md_buffers(sre_engine *e) {
+  const int N = e->n_atoms;
+  if (!e->d_vel) {
+    HIP_TRY(dalloc(&e->d_vel, (size_t)3 * N));
+    HIP_TRY(dalloc(&e->d_fcall, (size_t)3 * N));
+    HIP_TRY(dalloc(&e->d_md_part, (size_t)4 * md_part_count(N)));
+    e->graph_epoch++;
+  }
+  if (!e->d_md) {
+    HIP_TRY(dalloc(&e->d_md, 1));
+    HIP_TRY(dalloc(&e->d_mass, kMaxTypes));
+    e->graph_epoch++;
+  }
+  if (e->mass.empty()) {
+    e->mass.assign(kMaxTypes, 1.0f);
+    HIP_TRY(hipMemcpyAsync(e->d_mass, e->mass.data(), sizeof(float) * kMaxTypes,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  return SRE_OK;
+}
+
+// read-modify-write of the device MD state (stream-ordered)
+template <typename Fn>
+static sre_status md_patch(sre_engine *e, Fn fn) {
+  MDState m;
+  HIP_TRY(hipMemcpyAsync(&m, e->d_md, sizeof m, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  fn(m);
+  HIP_TRY(hipMemcpyAsync(e->d_md, &m, sizeof m, hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return SRE_OK;
+}
+
+static double md_total_mass(const sre_engine *e) {
+  double M = 0.0;
+  for (int t : e->h_types) M += e->mass[t];
+  return M;
+}
+
+static sre_status md_update_total_mass(sre_engine *e) {
+  const double M = md_total_mass(e);
+  e->md.total_mass = M;
+  return md_patch(e, [M](MDState &m) { m.total_mass = M; });
+}
+
+extern "C" sre_status sre_set_masses(sre_engine *e, const float *mass) {
+  if (!e || !mass || !e->system_set)
+    return fail(SRE_ERR_INVALID, "sre_set_masses: system not set");
+  for (int t = 0; t < e->n_types; ++t)
+    if (!(mass[t] > 0.f) || !std::isfinite(mass[t]))
+      return fail(SRE_ERR_INVALID, "mass of type %d is %g", t, (double)mass[t]);
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = md_buffers(e);
+  if (s != SRE_OK) return s;
+  std::copy(mass, mass + e->n_types, e->mass.begin());
+  HIP_TRY(hipMemcpyAsync(e->d_mass, e->mass.data(), sizeof(float) * kMaxTypes,
+                         hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  if (e->md_set) return md_update_total_mass(e);
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_velocities(sre_engine *e, const float *v) {
+  if (!e || !v || !e->system_set)
+    return fail(SRE_ERR_INVALID, "sre_set_velocities: system not set");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = md_buffers(e);
+  if (s != SRE_OK) return s;
+  HIP_TRY(hipMemcpyAsync(e->d_vel, v, sizeof(float) * 3 * e->n_atoms,
+                         hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->vel_set = true;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_get_velocities(sre_engine *e, float *v) {
+  if (!e || !v) return fail(SRE_ERR_INVALID, "sre_get_velocities: null argument");
+  if (!e->vel_set) return fail(SRE_ERR_NOT_READY, "velocities not set");
+  HIP_TRY(hipSetDevice(e->device));
+  HIP_TRY(hipMemcpyAsync(v, e->d_vel, sizeof(float) * 3 * e->n_atoms,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_integrator(sre_engine *e, double dt,
+                                         int thermostat, double ref_T,
+                                         double tau_T, int remove_com) {
+  if (!e || !e->system_set || !e->types_set)
+    return fail(SRE_ERR_INVALID, "sre_set_integrator: system or types not set");
+  if (e->n_rep != 1)
+    return fail(SRE_ERR_INVALID, "the integrator runs one replica (have %d)",
+                e->n_rep);
+  if (!(dt > 0.0) || !std::isfinite(dt) ||
+      (thermostat != SRE_THERMO_NONE && thermostat != SRE_THERMO_NHC2) ||
+      (remove_com != 0 && remove_com != 1))
+    return fail(SRE_ERR_INVALID, "sre_set_integrator: bad arguments");
+  if (thermostat == SRE_THERMO_NHC2 &&
+      !(ref_T > 0.0 && tau_T > 0.0 && std::isfinite(ref_T) && std::isfinite(tau_T)))
+    return fail(SRE_ERR_INVALID, "NHC needs ref_T > 0 and tau_T > 0");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = md_buffers(e);
+  if (s != SRE_OK) return s;
+  MDState m{};
+  // NoseHoover_Chains2::reinit (H/src/Thermostat.cu:23-53)
+  m.LL = 3.0 * e->n_atoms;
+  m.refT = thermostat == SRE_THERMO_NHC2 ? ref_T : 0.0;
+  m.Q1 = thermostat == SRE_THERMO_NHC2
+             ? tau_T * tau_T * ref_T / (4.0 * M_PI * M_PI) * m.LL : 1.0;
+  m.Q2 = m.Q1 / m.LL;
+  m.xi1 = m.xi2 = 1.0;
+  m.vxi1 = m.vxi2 = 0.0;
+  m.a1 = m.a2 = 1.0;
+  m.dt = dt;
+  m.total_mass = md_total_mass(e);
+  m.thermostat = thermostat;
+  m.remove_com = remove_com;
+  m.have_end = 0;
+  if (e->md_set && e->md.dt != dt) e->graph_epoch++;  // dt is a kernel argument
+  e->md = m;
+  HIP_TRY(hipMemcpyAsync(e->d_md, &m, sizeof(MDState), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->md_set = true;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_get_md_state(sre_engine *e, sre_md_state *out) {
+  if (!e || !out) return fail(SRE_ERR_INVALID, "sre_get_md_state: null argument");
+  if (!e->md_set) return fail(SRE_ERR_NOT_READY, "integrator not set");
+  HIP_TRY(hipSetDevice(e->device));
+  MDState m;
+  HIP_TRY(hipMemcpyAsync(&m, e->d_md, sizeof m, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  out->kinetic = m.K;
+  out->xi1 = m.xi1;
+  out->xi2 = m.xi2;
+  out->vxi1 = m.vxi1;
+  out->vxi2 = m.vxi2;
+  // NoseHoover_Chains2::HamiltonianContribution (H/src/Thermostat.cu:246-253)
+  out->nhc_energy = m.thermostat
+                        ? 0.5 * m.Q1 * m.vxi1 * m.vxi1 +
+                              0.5 * m.Q2 * m.vxi2 * m.vxi2 +
+                              m.LL * m.refT * m.xi1 + m.refT * m.xi2
+                        : 0.0;
+  out->Q1 = m.Q1;
+  out->Q2 = m.Q2;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in) {
+  if (!e || !in) return fail(SRE_ERR_INVALID, "sre_set_md_state: null argument");
+  if (!e->md_set) return fail(SRE_ERR_NOT_READY, "integrator not set");
+  HIP_TRY(hipSetDevice(e->device));
+  const sre_md_state c = *in;
+  return md_patch(e, [c](MDState &m) {
+    m.xi1 = c.xi1;
+    m.xi2 = c.xi2;
+    m.vxi1 = c.vxi1;
+    m.vxi2 = c.vxi2;
+  });
+}
+
+// largest displacement^2 since the last rebuild against skin/2 (*ok), and
+// the force-range flag, in one readback.  A skin violation is reported first:
+// missed pairs are the likelier cause of a range error that follows it.
+static sre_status md_skin_ok(sre_engine *e, bool *ok) {
+  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+                      e->d_skin_part, 0, e->d_flags, e->stream);
+  unsigned bits[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(bits, e->d_flags, 2 * sizeof(unsigned),
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  float d2;
+  memcpy(&d2, &bits[0], sizeof d2);
+  const float half = 0.5f * e->skin;
+  *ok = !(d2 > half * half);
+  if (*ok && (bits[1] & 1u))
+    return fail(SRE_ERR_FORCE_RANGE,
+                "a pair force component exceeded 2^19 (overlapping atoms?)");
+  return SRE_OK;
+}
+
+// H/app/lj.nhc-xtc.cu:158-196: n_steps of [COM removal, NHC(dt/2), kick +
+// drift, (rebuild), force, kick, NHC(dt/2)].
+extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
+                                 unsigned flags, int *n_rebuilds) {
+  if (!e || n_steps < 0 || rebuild_every < 0 ||
+      (flags & ~(unsigned)(SRE_STATS | SRE_TAIL)))
+    return fail(SRE_ERR_INVALID, "sre_md_run: bad arguments");
+  if (!e->md_set) return fail(SRE_ERR_NOT_READY, "integrator not set");
+  if (!e->vel_set) return fail(SRE_ERR_NOT_READY, "velocities not set");
+  if (n_rebuilds) *n_rebuilds = 0;
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s;
+  bool fresh = !e->built;
+  if (e->built) {
+    // the bound positions may have been changed since the last step:
+    // refresh the lattice copy and rebuild if they moved beyond skin/2
+    launch_update_positions(e->n_atoms, e->d_inv_perm,
+                            const_cast<float *>(e->d_pos), e->pos_stride,
+                            e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                            e->d_skin_part, nullptr, nullptr, e->stream);
+    HIP_TRY(hipGetLastError());
+    bool ok;
+    s = md_skin_ok(e, &ok);
+    if (s != SRE_OK) return s;
+    fresh = !ok;
+  }
+  if (fresh) {
+    s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+    if (n_rebuilds) ++*n_rebuilds;
+  }
+  refresh_tail_values(e);
+  const unsigned fl = flags & SRE_TAIL;
+  // forces at the current positions, then a kick of length 0: saves them in
+  // caller order and leaves the kinetic energy and momentum sums
+  s = launch_compute(e, fl);
+  if (s != SRE_OK) return s;
+  enqueue_md_kick(e, 0.f);
+  HIP_TRY(hipGetLastError());
+  if (n_steps == 0) {
+    launch_md_finish(e->n_atoms, e->d_md_part, e->d_md, e->d_vel, e->stream);
+    HIP_TRY(hipGetLastError());
+    e->forces_valid = true;
+    e->stats_valid = e->atom_stats_valid = false;
+    e->stats_pending = false;
+    bool ok;
+    s = md_skin_ok(e, &ok);
+    if (s != SRE_OK) return s;
+    if (!ok)
+      return fail(SRE_ERR_SKIN,
+                  "an atom moved more than skin/2 since the last rebuild");
+    return SRE_OK;
+  }
+  int left = n_steps;
+  while (left > 0) {
+    if (rebuild_every == 0) {
+      // the reference's rule: rebuild when the largest displacement since
+      // the last rebuild exceeds skin/2 (one readback per step)
+      enqueue_md_drift(e);
+      HIP_TRY(hipGetLastError());
+      bool ok;
+      s = md_skin_ok(e, &ok);
+      if (s != SRE_OK) return s;
+      if (!ok) {
+        // (a range flag raised before this point is seen by the next check)
+        s = do_rebuild(e);
+        if (s != SRE_OK) return s;
+        if (n_rebuilds) ++*n_rebuilds;
+      }
+      s = launch_compute(e, left == 1 ? flags : fl);
+      if (s != SRE_OK) return s;
+      enqueue_md_kick(e, (float)(0.5 * e->md.dt));
+      HIP_TRY(hipGetLastError());
+      ++e->md_since_rebuild;
+      --left;
+      continue;
+    }
+    if (e->md_since_rebuild >= rebuild_every) {
+      // fixed interval: every step since the last rebuild must have stayed
+      // within skin/2 (the rebuild reads the largest displacement back),
+      // else the lists of that stretch were incomplete
+      s = do_rebuild(e);
+      if (s != SRE_OK) return s;
+      if (n_rebuilds) ++*n_rebuilds;
+      const float half = 0.5f * e->skin;
+      if (e->last_disp2 > half * half)
+        return fail(SRE_ERR_SKIN,
+                    "an atom moved %g > skin/2 within %d steps: lower "
+                    "rebuild_every or raise the skin",
+                    sqrt((double)e->last_disp2), rebuild_every);
+    }
+    const int K = std::min({left, rebuild_every - e->md_since_rebuild,
+                            kMaxGraphSteps});
+    const unsigned gflags = kGraphMD | (K == left ? flags : fl);
+    hipGraphExec_t ex;
+    s = step_graph(e, gflags, K, &ex);
+    if (s != SRE_OK) return s;
+    HIP_TRY(hipGraphLaunch(ex, e->stream));
+    e->md_since_rebuild += K;
+    left -= K;
+  }
+  launch_md_finish(e->n_atoms, e->d_md_part, e->d_md, e->d_vel, e->stream);
+  HIP_TRY(hipGetLastError());
+  e->forces_valid = true;
+  e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
+  e->stats_pending = e->stats_valid;
+  bool ok;
+  s = md_skin_ok(e, &ok);
+  if (s != SRE_OK) return s;
+  if (!ok)
+    return fail(SRE_ERR_SKIN,
+                "an atom moved more than skin/2 since the last rebuild: lower "
+                "rebuild_every or raise the skin");
   return SRE_OK;
 }

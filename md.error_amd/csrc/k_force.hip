@@ -24,11 +24,16 @@ namespace sre {
 // inv_perm[i].  The skin test compares with the rebuild-time lattice position
 // backup[i]; each wavefront keeps the largest squared displacement it has
 // seen since the last rebuild in its own slot of skin_part (no atomics).
+// With drift.vel set the refresh is the first half of a velocity Verlet step
+// (H/src/Integrator.cu:273-324, velocityVerlet_part1) after the thermostat
+// and centre-of-mass factors of the step: v <- a2 (a1 v - vcom) + F dt/(2m),
+// x <- x + v dt, then x is wrapped back into [0, L) by one add or subtract of
+// L (the reference wraps at each rebuild, H/src/MDSystem_interface.cu:19-36).
 __global__ __launch_bounds__(256) void k_update_positions(
     int n, const uint32_t *__restrict__ inv_perm, float *__restrict__ pos,
     int stride, const RepInfo *__restrict__ reps, uint4 *__restrict__ pos4,
     const uint4 *__restrict__ backup, float4 *__restrict__ res4,
-    float *__restrict__ skin_part, PerturbArgs pert) {
+    float *__restrict__ skin_part, PerturbArgs pert, DriftArgs drift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   float d2 = 0.f;
   if (i < n) {
@@ -42,6 +47,28 @@ __global__ __launch_bounds__(256) void k_update_positions(
       x = o[0] + pert.sigma * g.x;
       y = o[1] + pert.sigma * g.y;
       z = o[2] + pert.sigma * g.z;
+      p[0] = x;
+      p[1] = y;
+      p[2] = z;
+    } else if (drift.vel) {
+      const MDState &m = *drift.state;
+      const float a1 = (float)m.a1, a2 = (float)m.a2;
+      const float c = 0.5f * drift.dt / drift.mass[drift.types[i]];
+      float *v = drift.vel + (size_t)i * 3;
+      const float *f = drift.fcall + (size_t)i * 3;
+      const float vx = a2 * (a1 * v[0] - (float)m.vcom[0]) + c * f[0];
+      const float vy = a2 * (a1 * v[1] - (float)m.vcom[1]) + c * f[1];
+      const float vz = a2 * (a1 * v[2] - (float)m.vcom[2]) + c * f[2];
+      v[0] = vx;
+      v[1] = vy;
+      v[2] = vz;
+      const RepInfo &R0 = reps[b.w >> 8];
+      x = p[0] + vx * drift.dt;
+      y = p[1] + vy * drift.dt;
+      z = p[2] + vz * drift.dt;
+      x = x < 0.f ? x + R0.L[0] : (x >= R0.L[0] ? x - R0.L[0] : x);
+      y = y < 0.f ? y + R0.L[1] : (y >= R0.L[1] ? y - R0.L[1] : y);
+      z = z < 0.f ? z + R0.L[2] : (z >= R0.L[2] ? z - R0.L[2] : z);
       p[0] = x;
       p[1] = y;
       p[2] = z;
@@ -74,13 +101,15 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
-                             hipStream_t st) {
+                             const DriftArgs *drift, hipStream_t st) {
   if (n <= 0) return;
   PerturbArgs p{};
   if (pert) p = *pert;
+  DriftArgs d{};
+  if (drift) d = *drift;
   hipLaunchKernelGGL(k_update_positions, dim3((n + 255) / 256), dim3(256), 0,
                      st, n, inv_perm, pos, stride, reps, pos4, backup, res4,
-                     skin_part, p);
+                     skin_part, p, d);
 }
 
 // ------------------------------------------------------- rebuild info --

@@ -217,6 +217,41 @@ struct PerturbArgs {
   const float *anchor;
   const uint64_t *step_ctr;
 };
+
+// MD integrator state on the device (k_md.hip): two-link Nose-Hoover chain
+// (reference NoseHoover_Chains2: xi init 1, vxi init 0, Q1 = tau^2 T LL /
+// (4 pi^2), Q2 = Q1 / LL, LL = 3N) and the velocity factors of the next
+// drift: v <- a2 (a1 v - vcom).
+struct MDState {
+  double xi1, xi2, vxi1, vxi2;
+  double Q1, Q2, LL, refT;
+  double a1, a2;
+  double vcom[3];
+  double K;            // kinetic energy after the last factors/finish
+  double dt, total_mass;
+  int thermostat, remove_com, have_end, pad;
+};
+// Optional velocity Verlet kick + drift folded into the position refresh:
+// v <- a2 (a1 v - vcom) + F dt / (2 m); x <- x + v dt (caller order; F from
+// fcall, saved by the previous kick).
+struct DriftArgs {
+  float *vel;
+  const float *fcall;
+  const float *mass;   // per type
+  const int *types;
+  const MDState *state;
+  float dt;
+};
+int md_part_count(int n);
+void launch_md_kick(int n, const uint32_t *inv_perm, const int *types,
+                    const int64_t *fx, const int64_t *fy, const int64_t *fz,
+                    const double *fxd, const double *fyd, const double *fzd,
+                    const float *mass, float hdt, float *vel, float *fcall,
+                    double *part, hipStream_t st);
+void launch_md_factors(int n, const double *part, MDState *state,
+                       hipStream_t st);
+void launch_md_finish(int n, const double *part, MDState *state, float *vel,
+                      hipStream_t st);
 // Position refresh + skin test: each wavefront keeps the largest squared
 // displacement it has seen since the last rebuild in skin_part[wave]
 // (skin_part_count(n) entries).
@@ -226,7 +261,7 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              int stride, const RepInfo *reps, uint4 *pos4,
                              const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
-                             hipStream_t st);
+                             const DriftArgs *drift, hipStream_t st);
 
 struct ForceArgs {
   int n;

@@ -28,6 +28,8 @@ TAIL = 2
 PERTURB = 4
 TAIL_BIN = 0
 TAIL_EXACT = 1
+THERMO_NONE = 0
+THERMO_NHC2 = 1
 
 
 class Options(C.Structure):
@@ -39,6 +41,15 @@ class Tuning(C.Structure):
     _fields_ = [("force_lanes", C.c_int), ("build_lanes", C.c_int), ("block", C.c_int),
                 ("cell_xy", C.c_float), ("cell_z", C.c_float), ("build_mode", C.c_int),
                 ("xcd_remap", C.c_int), ("force_unroll", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MDState(C.Structure):
+    _fields_ = [("kinetic", C.c_double), ("xi1", C.c_double), ("xi2", C.c_double),
+                ("vxi1", C.c_double), ("vxi2", C.c_double), ("nhc_energy", C.c_double),
+                ("Q1", C.c_double), ("Q2", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,6 +97,14 @@ SIGNATURES = {
     "sre_time_compute": (_st, [_vp, C.c_uint, C.c_int, _f64p]),
     "sre_time_rebuild": (_st, [_vp, C.c_int, _f64p]),
     "sre_synchronize": (_st, [_vp]),
+    "sre_set_masses": (_st, [_vp, _f32p]),
+    "sre_set_velocities": (_st, [_vp, _f32p]),
+    "sre_get_velocities": (_st, [_vp, _f32p]),
+    "sre_set_integrator": (_st, [_vp, C.c_double, C.c_int, C.c_double, C.c_double,
+                                 C.c_int]),
+    "sre_md_run": (_st, [_vp, C.c_int, C.c_int, C.c_uint, C.POINTER(C.c_int)]),
+    "sre_get_md_state": (_st, [_vp, C.POINTER(MDState)]),
+    "sre_set_md_state": (_st, [_vp, C.POINTER(MDState)]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_double, C.c_double, C.c_uint64, _f32p]),

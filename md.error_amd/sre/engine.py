@@ -319,6 +319,49 @@ class Engine:
     def synchronize(self):
         check(lib().sre_synchronize(self._h))
 
+    # -- MD integrator -------------------------------------------------------
+    def set_masses(self, mass):
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32),
+                                                 (self.n_types,)))
+        check(lib().sre_set_masses(self._h, _ptr(m, C.c_float)))
+
+    def set_velocities(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(self.n_atoms, 3)
+        check(lib().sre_set_velocities(self._h, _ptr(v, C.c_float)))
+
+    def velocities(self):
+        out = np.empty((self.n_atoms, 3), np.float32)
+        check(lib().sre_get_velocities(self._h, _ptr(out, C.c_float)))
+        return out
+
+    def set_integrator(self, dt, thermostat=None, ref_T=0.0, tau_T=0.0, remove_com=True):
+        """Velocity Verlet with time step dt; thermostat None (NVE) or "nhc2"
+        (two-link Nose-Hoover chain at ref_T, period tau_T); remove_com removes
+        the centre-of-mass velocity at the start of every step."""
+        th = {None: _lib.THERMO_NONE, "nve": _lib.THERMO_NONE,
+              "nhc2": _lib.THERMO_NHC2}[thermostat]
+        check(lib().sre_set_integrator(self._h, float(dt), th, float(ref_T), float(tau_T),
+                                       1 if remove_com else 0))
+
+    def md_run(self, n_steps, rebuild_every=10, stats=False, tail=False):
+        """n_steps MD steps; returns the number of list rebuilds."""
+        nr = C.c_int(0)
+        check(lib().sre_md_run(self._h, int(n_steps), int(rebuild_every),
+                               _flags(stats, tail), C.byref(nr)))
+        return nr.value
+
+    def md_state(self):
+        s = _lib.MDState()
+        check(lib().sre_get_md_state(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def set_md_state(self, **kw):
+        s = _lib.MDState()
+        check(lib().sre_get_md_state(self._h, C.byref(s)))
+        for k, v in kw.items():
+            setattr(s, k, v)
+        check(lib().sre_set_md_state(self._h, C.byref(s)))
+
 
 def gen_fcc(ncell, a, jitter, seed):
     n = 4 * ncell ** 3
