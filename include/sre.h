@@ -281,6 +281,8 @@ typedef struct sre_md_state {
   double nhc_energy;    /* HamiltonianContribution: Q1 vxi1^2/2 + Q2 vxi2^2/2
                            + 3N T xi1 + T xi2 (0 without thermostat) */
   double Q1, Q2;        /* chain masses, tau^2 T 3N / (4 pi^2) and Q1 / 3N */
+  int adapt_steps;      /* length of the next SRE_MD_ADAPTIVE stretch (read only) */
+  int rollbacks;        /* adaptive stretches redone so far (read only) */
 } sre_md_state;
 /* Per-type masses, n_types values > 0 (default 1). */
 sre_status sre_set_masses(sre_engine *e, const float *mass);
@@ -297,9 +299,14 @@ sre_status sre_set_integrator(sre_engine *e, double dt, int thermostat,
  * atom moved more than skin/2 between two rebuilds (the forces of that
  * stretch may have missed pairs).  rebuild_every = 0: the reference's rule,
  * a rebuild whenever the largest displacement since the last one exceeds
- * skin/2 (one host synchronisation per step).  flags: SRE_TAIL adds the tail
+ * skin/2 (one host synchronisation per step).  rebuild_every =
+ * SRE_MD_ADAPTIVE: stretches of K steps run as graphs from a snapshot and are
+ * kept only if no atom moved more than skin/2 (else restored and redone with
+ * K/2); K adapts to the measured displacement.  Forces are exact fixed-point
+ * sums, so all three give the same trajectory.  flags: SRE_TAIL adds the tail
  * force every step; SRE_STATS computes energy/virial on the last step (read
  * by sre_get_stats).  Forces are recomputed at the start of every call. */
+#define SRE_MD_ADAPTIVE (-1)
 sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
                       unsigned flags, int *n_rebuilds);
 sre_status sre_get_md_state(sre_engine *e, sre_md_state *out);

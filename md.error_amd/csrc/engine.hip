@@ -202,6 +202,13 @@ struct sre_engine {
   MDState md{};                    // host copy of the parameters
   bool vel_set = false, md_set = false;
   int md_since_rebuild = 0;        // MD steps since the last rebuild
+  // adaptive stretches: state at the start of the running stretch
+  float *d_snap_pos = nullptr, *d_snap_vel = nullptr, *d_snap_fcall = nullptr;
+  double *d_snap_part = nullptr;
+  MDState *d_snap_md = nullptr;
+  size_t snap_pos_cap = 0;
+  int md_adapt_k = 8;              // steps of the next adaptive stretch
+  int md_rollbacks = 0;            // stretches redone after a skin violation
 
   ~sre_engine();
 };
@@ -226,6 +233,8 @@ sre_engine::~sre_engine() {
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   dfree(d_adj); dfree(d_adj_tw);
   dfree(d_mass); dfree(d_vel); dfree(d_fcall); dfree(d_md_part); dfree(d_md);
+  dfree(d_snap_pos); dfree(d_snap_vel); dfree(d_snap_fcall); dfree(d_snap_part);
+  dfree(d_snap_md);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -359,6 +368,9 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_stat_part);
   dfree(e->d_skin_part);
   dfree(e->d_vel); dfree(e->d_fcall); dfree(e->d_md_part);
+  dfree(e->d_snap_pos); dfree(e->d_snap_vel); dfree(e->d_snap_fcall);
+  dfree(e->d_snap_part);
+  e->snap_pos_cap = 0;
   e->vel_set = false;
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
@@ -1818,6 +1830,8 @@ extern "C" sre_status sre_get_md_state(sre_engine *e, sre_md_state *out) {
                         : 0.0;
   out->Q1 = m.Q1;
   out->Q2 = m.Q2;
+  out->adapt_steps = e->md_adapt_k;
+  out->rollbacks = e->md_rollbacks;
   return SRE_OK;
 }
 
@@ -1837,7 +1851,7 @@ extern "C" sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in) {
 // largest displacement^2 since the last rebuild against skin/2 (*ok), and
 // the force-range flag, in one readback.  A skin violation is reported first:
 // missed pairs are the likelier cause of a range error that follows it.
-static sre_status md_skin_ok(sre_engine *e, bool *ok) {
+static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
   launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
                       e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits[2] = {0, 0};
@@ -1848,9 +1862,39 @@ static sre_status md_skin_ok(sre_engine *e, bool *ok) {
   memcpy(&d2, &bits[0], sizeof d2);
   const float half = 0.5f * e->skin;
   *ok = !(d2 > half * half);
+  if (disp2) *disp2 = d2;
   if (*ok && (bits[1] & 1u))
     return fail(SRE_ERR_FORCE_RANGE,
                 "a pair force component exceeded 2^19 (overlapping atoms?)");
+  return SRE_OK;
+}
+
+// Saves (restore = false) or restores the state an MD stretch starts from:
+// bound positions, velocities, saved forces, kick sums and the chain.
+static sre_status md_snapshot(sre_engine *e, bool restore) {
+  const int N = e->n_atoms;
+  const size_t npos = (size_t)N * e->pos_stride;
+  if (!e->d_snap_vel || e->snap_pos_cap < npos) {
+    dfree(e->d_snap_pos);
+    HIP_TRY(dalloc(&e->d_snap_pos, npos));
+    e->snap_pos_cap = npos;
+    if (!e->d_snap_vel) {
+      HIP_TRY(dalloc(&e->d_snap_vel, (size_t)3 * N));
+      HIP_TRY(dalloc(&e->d_snap_fcall, (size_t)3 * N));
+      HIP_TRY(dalloc(&e->d_snap_part, (size_t)4 * md_part_count(N)));
+    }
+    if (!e->d_snap_md) HIP_TRY(dalloc(&e->d_snap_md, 1));
+  }
+  struct Pair { void *live; void *snap; size_t bytes; };
+  const Pair pairs[] = {
+      {const_cast<float *>(e->d_pos), e->d_snap_pos, sizeof(float) * npos},
+      {e->d_vel, e->d_snap_vel, sizeof(float) * 3 * N},
+      {e->d_fcall, e->d_snap_fcall, sizeof(float) * 3 * N},
+      {e->d_md_part, e->d_snap_part, sizeof(double) * 4 * md_part_count(N)},
+      {e->d_md, e->d_snap_md, sizeof(MDState)}};
+  for (const Pair &p : pairs)
+    HIP_TRY(hipMemcpyAsync(restore ? p.live : p.snap, restore ? p.snap : p.live,
+                           p.bytes, hipMemcpyDeviceToDevice, e->stream));
   return SRE_OK;
 }
 
@@ -1858,7 +1902,7 @@ static sre_status md_skin_ok(sre_engine *e, bool *ok) {
 // drift, (rebuild), force, kick, NHC(dt/2)].
 extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
                                  unsigned flags, int *n_rebuilds) {
-  if (!e || n_steps < 0 || rebuild_every < 0 ||
+  if (!e || n_steps < 0 || rebuild_every < SRE_MD_ADAPTIVE ||
       (flags & ~(unsigned)(SRE_STATS | SRE_TAIL)))
     return fail(SRE_ERR_INVALID, "sre_md_run: bad arguments");
   if (!e->md_set) return fail(SRE_ERR_NOT_READY, "integrator not set");
@@ -1907,8 +1951,62 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
                   "an atom moved more than skin/2 since the last rebuild");
     return SRE_OK;
   }
+  const float half = 0.5f * e->skin;
+  bool verified = false;
+  if (rebuild_every == SRE_MD_ADAPTIVE && e->md_since_rebuild > 0) {
+    // every speculative stretch starts from a list built at its first step
+    s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+    if (n_rebuilds) ++*n_rebuilds;
+  }
   int left = n_steps;
   while (left > 0) {
+    if (rebuild_every == SRE_MD_ADAPTIVE) {
+      // a stretch of K steps as one graph from a snapshot; the displacement
+      // read back at its end decides: kept (next K aims at 80 % of skin/2,
+      // displacements growing about linearly) or rolled back and redone
+      // with K/2.  Each kept stretch used complete lists throughout.
+      const int K = std::min({left, e->md_adapt_k, kMaxGraphSteps});
+      const bool last = K == left;
+      s = md_snapshot(e, false);
+      if (s != SRE_OK) return s;
+      hipGraphExec_t ex;
+      s = step_graph(e, kGraphMD | (last ? flags : fl), K, &ex);
+      if (s != SRE_OK) return s;
+      HIP_TRY(hipGraphLaunch(ex, e->stream));
+      e->md_since_rebuild += K;
+      bool ok;
+      float d2;
+      if (!last) {
+        s = do_rebuild(e);   // reads the stretch's displacement back
+        if (s != SRE_OK) return s;
+        if (n_rebuilds) ++*n_rebuilds;
+        d2 = e->last_disp2;
+        ok = !(d2 > half * half);
+      } else {
+        s = md_skin_ok(e, &ok, &d2);
+        if (s != SRE_OK) return s;
+      }
+      if (!ok) {
+        if (K == 1)
+          return fail(SRE_ERR_SKIN, "an atom moved %g > skin/2 in one step",
+                      sqrt((double)d2));
+        s = md_snapshot(e, true);
+        if (s != SRE_OK) return s;
+        s = do_rebuild(e);
+        if (s != SRE_OK) return s;
+        if (n_rebuilds) ++*n_rebuilds;
+        e->md_adapt_k = std::max(1, K / 2);
+        ++e->md_rollbacks;
+        continue;
+      }
+      left -= K;
+      const double d = sqrt((double)d2);
+      const double k_next = d > 0.0 ? K * 0.8 * half / d : (double)kMaxGraphSteps;
+      e->md_adapt_k = (int)std::max(1.0, std::min((double)kMaxGraphSteps, floor(k_next)));
+      verified = last;
+      continue;
+    }
     if (rebuild_every == 0) {
       // the reference's rule: rebuild when the largest displacement since
       // the last rebuild exceeds skin/2 (one readback per step)
@@ -1938,7 +2036,6 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
       s = do_rebuild(e);
       if (s != SRE_OK) return s;
       if (n_rebuilds) ++*n_rebuilds;
-      const float half = 0.5f * e->skin;
       if (e->last_disp2 > half * half)
         return fail(SRE_ERR_SKIN,
                     "an atom moved %g > skin/2 within %d steps: lower "
@@ -1960,6 +2057,7 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
   e->forces_valid = true;
   e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
   e->stats_pending = e->stats_valid;
+  if (verified) return SRE_OK;
   bool ok;
   s = md_skin_ok(e, &ok);
   if (s != SRE_OK) return s;

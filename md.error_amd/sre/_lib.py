@@ -49,7 +49,8 @@ class Tuning(C.Structure):
 class MDState(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("xi1", C.c_double), ("xi2", C.c_double),
                 ("vxi1", C.c_double), ("vxi2", C.c_double), ("nhc_energy", C.c_double),
-                ("Q1", C.c_double), ("Q2", C.c_double)]
+                ("Q1", C.c_double), ("Q2", C.c_double), ("adapt_steps", C.c_int),
+                ("rollbacks", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

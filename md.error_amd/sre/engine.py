@@ -344,7 +344,12 @@ class Engine:
                                        1 if remove_com else 0))
 
     def md_run(self, n_steps, rebuild_every=10, stats=False, tail=False):
-        """n_steps MD steps; returns the number of list rebuilds."""
+        """n_steps MD steps; returns the number of list rebuilds.
+        rebuild_every: a fixed interval (> 0), 0 = check the displacement every
+        step (the reference's rule), "adaptive" = speculative graph stretches
+        with rollback."""
+        if rebuild_every == "adaptive":
+            rebuild_every = -1
         nr = C.c_int(0)
         check(lib().sre_md_run(self._h, int(n_steps), int(rebuild_every),
                                _flags(stats, tail), C.byref(nr)))

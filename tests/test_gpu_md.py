@@ -4,7 +4,8 @@
   forces the device used, and those forces match the FP64 oracle;
 - one NHC + COM-removal step matches the FP64 oracle step (FP32 round-off);
 - trajectories do not depend on how the run is cut into calls, on the rebuild
-  interval, or on graph vs eager execution (bit-exact, NVE);
+  policy (fixed interval, per-step check, adaptive stretches with rollback)
+  or on graph vs eager execution (bit-exact, NVE);
 - invariants over longer runs (NVE energy, momentum) and a 400-step NHC run
   that follows the oracle's extended energy and temperature.
 """
@@ -117,11 +118,14 @@ def test_nhc_com_steps_match_oracle(torch_cuda, fcc, steps):
     assert abs(st["nhc_energy"] - nhc.hamiltonian()) < 1e-6 * abs(nhc.hamiltonian())
 
 
-def _trajectory(torch, w, v0, chunks, rebuild_every, com, precision="mixed"):
-    e, pos = md_engine(torch, w, v0, com=com, precision=precision)
+def _trajectory(torch, w, v0, chunks, rebuild_every, com, precision="mixed", skin=0.4,
+                engine_out=None):
+    e, pos = md_engine(torch, w, v0, com=com, precision=precision, skin=skin)
     nr = 0
     for c in chunks:
         nr += e.md_run(c, rebuild_every=rebuild_every)
+    if engine_out is not None:
+        engine_out.append(e)
     return pos_of(pos), e.velocities(), nr
 
 
@@ -135,9 +139,26 @@ def test_trajectory_independent_of_segmentation(torch_cuda, fcc, com):
     xa, va, na = _trajectory(torch_cuda, fcc, v0, [24], 6, com)
     xb, vb, nb = _trajectory(torch_cuda, fcc, v0, [5, 11, 8], 4, com)
     xc, vc, nc = _trajectory(torch_cuda, fcc, v0, [24], 0, com)
+    xd, vd, nd = _trajectory(torch_cuda, fcc, v0, [9, 15], "adaptive", com)
     assert na >= 4 and nb >= 6
     assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     assert np.array_equal(xa, xc) and np.array_equal(va, vc)
+    assert np.array_equal(xa, xd) and np.array_equal(va, vd)
+
+
+def test_adaptive_rollback_keeps_trajectory(torch_cuda, fcc):
+    """With a thin skin the first adaptive stretches overrun skin/2 and are
+    rolled back; the kept trajectory is still the eager rule's, bit for bit."""
+    v0 = maxwell(fcc.n, 2.0, 14)
+    eng = []
+    xa, va, na = _trajectory(torch_cuda, fcc, v0, [40], "adaptive", True, skin=0.1,
+                             engine_out=eng)
+    xe, ve, ne = _trajectory(torch_cuda, fcc, v0, [40], 0, True, skin=0.1)
+    st = eng[0].md_state()
+    assert st["rollbacks"] >= 1
+    assert 1 <= st["adapt_steps"] < 8
+    assert ne >= 5
+    assert np.array_equal(xa, xe) and np.array_equal(va, ve)
 
 
 def test_fp64_form_trajectory_close(torch_cuda, fcc):
