@@ -18,7 +18,9 @@ Prints one JSON line:  value = pair evaluations per second (list entries
 processed by the force kernel per second of whole-job wall time, rebuilds
 included), ms_per_step, the FP64 form measured the same way, the force
 kernel's HBM roofline fraction, force/energy error of the mixed path against
-the FP64 form, and the CPU path (the FP64 oracle, OpenMP) timed on the host.
+the FP64 form, the CPU path (the FP64 oracle, OpenMP) timed on the host, and
+under "md" the device MD loop (velocity Verlet + Nose-Hoover chain, the same
+force path) timed on the same slab.
 """
 import argparse
 import json
@@ -238,6 +240,70 @@ def force_time_distribution(e, torch, stream, n=200):
             "p90": round(float(np.percentile(t, 90)), 3)}
 
 
+MD_DT, MD_REF_T, MD_TAU_T = 0.005, 1.10, 1.0   # H/app/lj.nhc-xtc.cu:42-46
+
+
+def run_md(w, steps, warmup, torch, Engine):
+    """Times the device MD loop: velocity Verlet + two-link Nose-Hoover chain
+    + COM removal, the reference driver's step (H/app/lj.nhc-xtc.cu:158-196,
+    dt 0.005, T 1.1, tau 1), with the force path of the metric (rc(z) + tail)
+    and adaptive rebuild stretches; thermo output (energy, virial, kinetic
+    energy and chain state read back) every 100 steps.  The generated slab
+    holds pairs down to dmin = 0.85, whose release would heat it far above
+    1.1, so the warmup first relaxes it under a tight chain (tau 0.05), then
+    draws fresh velocities at 1.1 and restarts the chain with tau 1."""
+    import numpy as np
+    stream = torch.cuda.Stream()
+    e = Engine(n_types=w.n_types, skin=w.skin, precision="mixed", stream=stream.cuda_stream)
+    w.setup(e, tail=True)
+    pos = torch.from_numpy(w.xyz.copy()).to("cuda")
+    torch.cuda.synchronize()
+    e.bind_positions(pos)
+    e.set_masses(np.ones(w.n_types, np.float32))
+    rng = np.random.default_rng(SEED)
+
+    def draw():
+        e.set_velocities(rng.normal(0.0, np.sqrt(MD_REF_T), (w.n, 3)).astype(np.float32))
+
+    def loop(n_steps):
+        done, pairs, rebuilds = 0, 0, 0
+        while done < n_steps:
+            n = min(THERMO_EVERY, n_steps - done)
+            rebuilds += e.md_run(n, rebuild_every="adaptive", stats=True, tail=True)
+            e.stats()
+            e.md_state()
+            pairs += e.neighbor_count(in_rc=False)[0] * n
+            done += n
+        return pairs, rebuilds
+
+    draw()
+    e.set_integrator(MD_DT, "nhc2", MD_REF_T, 0.05, remove_com=True)
+    loop(warmup)
+    draw()
+    e.set_integrator(MD_DT, "nhc2", MD_REF_T, MD_TAU_T, remove_com=True)
+    loop(100)
+    r0 = e.md_state()["rollbacks"]
+    t_start = 2.0 * e.md_state()["kinetic"] / (3.0 * w.n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pairs, rebuilds = loop(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = e.md_state()
+    return {"ms_per_step": round(1e3 * wall / steps, 5),
+            "steps_per_s": round(steps / wall, 1),
+            "pair_evals_per_s": round(pairs / wall, 1),
+            "steps": steps, "warmup": warmup, "rebuilds": rebuilds,
+            "mean_steps_per_rebuild": round(steps / max(rebuilds, 1), 2),
+            "rollbacks": st["rollbacks"] - r0,
+            "temperature_start": round(t_start, 4),
+            "temperature_end": round(2.0 * st["kinetic"] / (3.0 * w.n), 4),
+            "mean_list_entries": round(pairs / steps),
+            "dt": MD_DT, "thermostat": f"NHC2 T={MD_REF_T} tau={MD_TAU_T}",
+            "pairs_def": "list entries after each 100-step call x 100",
+            "rebuild_policy": "adaptive stretches (snapshot + rollback), skin/2 rule"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +321,8 @@ def main():
                     help="pick tile/block/cell sizes with sre_autotune before timing")
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the adjoint (parameter gradients) of the mixed form")
+    ap.add_argument("--no-md", action="store_true",
+                    help="skip timing the device MD loop (VV + NHC2 + COM removal)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -346,6 +414,12 @@ def main():
         result["autotune_s"] = round(mixed["autotune_s"], 2)
     if args.adjoint:
         result["adjoint"] = time_adjoint(mixed["engine"], w, torch)
+    if not args.no_md and len(w.n_atoms) == 1:
+        try:
+            result["md"] = run_md(w, args.steps, max(args.warmup, 300), torch, Engine)
+        except Exception as ex:          # reported in the line, never silent
+            result["md"] = {"error": f"{type(ex).__name__}: {ex}"}
+        log(f"[bench] md: {result['md']}")
     if fp64 is not None:
         result["fp64_form"] = {
             "value": round(fp64["pair_evals_per_s"], 1), "ms_per_step": round(fp64["ms_per_step"], 5),
