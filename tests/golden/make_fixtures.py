@@ -10,6 +10,12 @@ positions and boxes as .npz files next to this script:
                   (16000-atom liquid-vapour slab, 150 x 21 x 21, T* = 0.85; the
                   system of PRE Table I)
   conf10000.npz   shortRange.inhomo/template/confs/N00010000-T0.7-100x020x020.gro
+  arc_t085.npz    the adaptive-cutoff outputs of the reference's 16000-atom slab run
+                  (T* = 0.85, target precision 0.0045; shortRange.inhomo/alltext/
+                  pubilsh.pre/resubmit/fig/t0.85-n16000-adapt-e0.0045/): the
+                  run-averaged density along x (a.density.x.out), the rc averaged
+                  over y, z (a.rcut.x.out), and the estimated error and rc of the
+                  y = z = 0 column at the last update (error.x.out: x, error, rc)
 GROMACS .gro layout: title line, atom count, one fixed-column line per atom
 (positions in columns 21-44, 8 characters each), box line.
 """
@@ -39,7 +45,25 @@ def read_gro(path):
     return xyz, box
 
 
+ARC_DIR = ("/root/reference/shortRange.inhomo/alltext/pubilsh.pre/resubmit/fig/"
+           "t0.85-n16000-adapt-e0.0045")
+
+
+def arc_fixture():
+    tables = {}
+    for key, name in (("a_density", "a.density.x.out"), ("a_rcut", "a.rcut.x.out"),
+                      ("error_x", "error.x.out")):
+        src = os.path.join(ARC_DIR, name)
+        if not os.path.exists(src):
+            print(f"missing {src}", file=sys.stderr)
+            return
+        tables[key] = np.loadtxt(src)
+    np.savez_compressed(os.path.join(HERE, "arc_t085.npz"), **tables)
+    print("arc_t085.npz", {k: v.shape for k, v in tables.items()})
+
+
 def main():
+    arc_fixture()
     for out, rel in FILES.items():
         src = os.path.join(REF, rel)
         if not os.path.exists(src):
