@@ -313,6 +313,50 @@ sre_status sre_get_md_state(sre_engine *e, sre_md_state *out);
 /* Restores the chain positions and velocities (checkpoint restart). */
 sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in);
 
+/* ---- adaptive-cutoff producer -------------------------------------------- */
+/* The rc lattice producer of the adaptive method on the GPU: a density grid
+ * fed by deposits of positions (DensityProfile_PiecewiseConst,
+ * H/src/DensityProfile.cu:27-99), the error estimate of every rc candidate
+ * (AdaptRCut::reinit/makeS1k/makeS2k/calError, H/src/AdaptRCut.cu:96-416),
+ * the per-cell choice of rc (calRCut, :418-468) and its widening (refineRCut,
+ * :470-508).  One grid per box; LJ units with eps = sigma = 1 as in the
+ * reference (makeS1k(1., 1.)). */
+typedef struct sre_arc sre_arc;
+/* Grid of n_d = int(L_d / refh) cells per edge, made odd (reinit).  device -1:
+ * the current device; stream NULL: a stream of its own. */
+sre_status sre_arc_create(sre_arc **out, const double box[3], double refh,
+                          int device, void *stream);
+void sre_arc_destroy(sre_arc *a);
+sre_status sre_arc_dims(sre_arc *a, int *dims /* [3] */, int *n_rc);
+/* Candidates rcmin, rcmin + rcstep, ... <= rcmax (rcmax < 30) and their error
+ * kernels (AdaptRCut::reinit; computed on the device). */
+sre_status sre_arc_set_candidates(sre_arc *a, double rcmin, double rcmax,
+                                  double rcstep);
+sre_status sre_arc_rc_list(sre_arc *a, double *rcs /* [n_rc] */);
+/* Density samples: clearData, deposite (device positions, n atoms, stride >= 3
+ * floats), the same from an engine replica's bound positions, a host density
+ * in place of samples, and calculate() read back. */
+sre_status sre_arc_clear(sre_arc *a);
+sre_status sre_arc_deposit(sre_arc *a, const float *d_pos, int n, int stride);
+sre_status sre_arc_deposit_engine(sre_arc *a, sre_engine *e, int replica);
+sre_status sre_arc_set_density(sre_arc *a, const double *rho /* [nele] */);
+sre_status sre_arc_density(sre_arc *a, double *rho /* [nele] */);
+/* calError; keep_parts = 1 also keeps (e1, e2x, e2y, e2z) per candidate. */
+sre_status sre_arc_cal_error(sre_arc *a, int keep_parts);
+/* calRCut(prec): per cell the candidate index by the reference's bisection. */
+sre_status sre_arc_cal_rcut(sre_arc *a, double prec);
+/* refineRCut: index = max over the periodic 3x3x3 neighbourhood. */
+sre_status sre_arc_refine(sre_arc *a);
+/* Result per cell rc[iz + nz*(iy + ny*ix)] (FP32 as AssignRCut::getRCut
+ * copies it), its error and candidate index; any pointer may be NULL. */
+sre_status sre_arc_result(sre_arc *a, float *rc, double *err, int *idx);
+/* error[c][cell]; parts[c][cell][4] (needs keep_parts). */
+sre_status sre_arc_errors(sre_arc *a, double *err, double *parts);
+/* Kernel tables S1[c][cell] and G[c][cell] (S2_a = i k_a G). */
+sre_status sre_arc_kernels(sre_arc *a, double *S1, double *G);
+/* AssignRCut::getRCut: installs the result as the rc lattice of a replica. */
+sre_status sre_arc_apply(sre_arc *a, sre_engine *e, int replica);
+
 /* ---- synthetic configurations (host only, no GPU use) --------------------- */
 
 /* FCC lattice of ncell^3 unit cells with lattice constant a, every coordinate

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/sre.h"
+#include "sre_host.h"
 #include "sre_internal.h"
 
 using namespace sre;
@@ -26,7 +27,7 @@ using namespace sre;
 
 static thread_local std::string g_err;
 
-static sre_status fail(sre_status s, const char *fmt, ...) {
+sre_status sre::fail(sre_status s, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -34,26 +35,6 @@ static sre_status fail(sre_status s, const char *fmt, ...) {
   va_end(ap);
   g_err = buf;
   return s;
-}
-
-#define HIP_TRY(call)                                                      \
-  do {                                                                     \
-    hipError_t _e = (call);                                                \
-    if (_e != hipSuccess)                                                  \
-      return fail(SRE_ERR_HIP, "%s: %s (%s:%d)", #call,                    \
-                  hipGetErrorString(_e), __FILE__, __LINE__);              \
-  } while (0)
-
-template <typename T>
-static hipError_t dalloc(T **p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  return hipMalloc((void **)p, count * sizeof(T));
-}
-template <typename T>
-static void dfree(T *&p) {
-  if (p) (void)hipFree((void *)p);
-  p = nullptr;
 }
 
 // ----------------------------------------------------------------- engine --
@@ -2065,5 +2046,19 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
     return fail(SRE_ERR_SKIN,
                 "an atom moved more than skin/2 since the last rebuild: lower "
                 "rebuild_every or raise the skin");
+  return SRE_OK;
+}
+
+sre_status sre::engine_replica_positions(sre_engine *e, int replica,
+                                         const float **pos, int *n, int *stride,
+                                         float box[3]) {
+  if (!e || !e->system_set || replica < 0 || replica >= e->n_rep)
+    return fail(SRE_ERR_INVALID, "no replica %d", replica);
+  if (!e->d_pos) return fail(SRE_ERR_INVALID, "positions not bound");
+  const RepInfo &R = e->reps[replica];
+  *pos = e->d_pos + (size_t)R.atom_off * e->pos_stride;
+  *n = R.n_atoms;
+  *stride = e->pos_stride;
+  for (int d = 0; d < 3; ++d) box[d] = R.L[d];
   return SRE_OK;
 }

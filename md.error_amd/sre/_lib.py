@@ -106,6 +106,23 @@ SIGNATURES = {
     "sre_md_run": (_st, [_vp, C.c_int, C.c_int, C.c_uint, C.POINTER(C.c_int)]),
     "sre_get_md_state": (_st, [_vp, C.POINTER(MDState)]),
     "sre_set_md_state": (_st, [_vp, C.POINTER(MDState)]),
+    "sre_arc_create": (_st, [C.POINTER(_vp), _f64p, C.c_double, C.c_int, _vp]),
+    "sre_arc_destroy": (None, [_vp]),
+    "sre_arc_dims": (_st, [_vp, _i32p, C.POINTER(C.c_int)]),
+    "sre_arc_set_candidates": (_st, [_vp, C.c_double, C.c_double, C.c_double]),
+    "sre_arc_rc_list": (_st, [_vp, _f64p]),
+    "sre_arc_clear": (_st, [_vp]),
+    "sre_arc_deposit": (_st, [_vp, _vp, C.c_int, C.c_int]),
+    "sre_arc_deposit_engine": (_st, [_vp, _vp, C.c_int]),
+    "sre_arc_set_density": (_st, [_vp, _f64p]),
+    "sre_arc_density": (_st, [_vp, _f64p]),
+    "sre_arc_cal_error": (_st, [_vp, C.c_int]),
+    "sre_arc_cal_rcut": (_st, [_vp, C.c_double]),
+    "sre_arc_refine": (_st, [_vp]),
+    "sre_arc_result": (_st, [_vp, _f32p, _f64p, _i32p]),
+    "sre_arc_errors": (_st, [_vp, _f64p, _f64p]),
+    "sre_arc_kernels": (_st, [_vp, _f64p, _f64p]),
+    "sre_arc_apply": (_st, [_vp, _vp, C.c_int]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_double, C.c_double, C.c_uint64, _f32p]),

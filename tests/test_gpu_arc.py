@@ -1,0 +1,159 @@
+"""GPU parity of the adaptive-cutoff producer (sre_arc_*) with the oracle.
+
+Kernel tables, the error of every candidate (and its parts), the chosen and
+widened candidate per cell, the deposited density, and the full pipeline on
+the reference's own grid (151 x 21 x 21, 19 candidates) against both the
+oracle and the reference's slab-run output.
+"""
+import os
+import warnings
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import arc as A
+from oracle import oracle as O
+from sre import Arc, Engine
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(autouse=True)
+def _quiet():
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", integrate.IntegrationWarning)
+        yield
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def small(torch_cuda):
+    box = (12.0, 9.0, 15.0)
+    arc = Arc(box, 1.0)
+    rcs = arc.set_candidates(3.0, 5.0, 0.5)
+    assert arc.dims == A.grid_dims(box, 1.0) == (13, 9, 15)
+    assert np.allclose(rcs, A.rc_list(3.0, 5.0, 0.5))
+    S1, G, kv = A.kernels(box, arc.dims, rcs)
+    rng = np.random.default_rng(0)
+    x, y, z = np.meshgrid(*[np.arange(n) for n in arc.dims], indexing="ij")
+    rho = (0.4 + 0.3 * np.tanh(np.sin(2 * np.pi * z / arc.dims[2]) * 3)
+           + 0.05 * rng.random(arc.dims))
+    return dict(box=box, arc=arc, rcs=rcs, S1=S1, G=G, kv=kv, rho=rho)
+
+
+def test_kernel_tables(small):
+    S1, G = small["arc"].kernels()
+    for c in range(len(small["rcs"])):
+        assert rel(S1[c], small["S1"][c]) < 1e-10
+        assert rel(G[c], small["G"][c]) < 1e-10
+
+
+def test_errors_and_parts(small):
+    arc = small["arc"]
+    arc.set_density(small["rho"])
+    assert np.array_equal(arc.density(), small["rho"])
+    arc.cal_error(keep_parts=True)
+    err, parts = arc.errors(parts=True)
+    ref, e1, e2 = A.cal_error(small["rho"], small["box"], small["S1"], small["G"],
+                              small["kv"], parts=True)
+    assert rel(err, ref) < 1e-10
+    assert rel(parts[..., 0], e1) < 1e-10
+    for a in range(3):
+        assert rel(parts[..., 1 + a], e2[:, a]) < 1e-9
+
+
+def test_rcut_and_refine(small):
+    arc = small["arc"]
+    arc.set_density(small["rho"])
+    arc.cal_error()
+    ref = A.cal_error(small["rho"], small["box"], small["S1"], small["G"], small["kv"])
+    for prec in (0.004, 0.01, 0.03):
+        arc.cal_rcut(prec)
+        want = A.cal_rcut(ref, prec)
+        rc, err, idx = arc.result()
+        assert np.array_equal(idx, want)
+        for n in (1, 2):
+            arc.refine()
+            want = A.refine(want)
+            rc, err, idx = arc.result()
+            assert np.array_equal(idx, want)
+        assert np.array_equal(rc, small["rcs"][want].astype(np.float32))
+        assert rel(err, np.take_along_axis(ref, want[None], 0)[0]) < 1e-10
+
+
+def test_deposit_matches_oracle(torch_cuda):
+    g = np.load(os.path.join(GOLDEN, "conf16000.npz"))
+    xyz, box = g["xyz"], g["box"].astype(np.float64)
+    arc = Arc(box, 1.0)
+    pos = torch_cuda.from_numpy(np.c_[xyz, np.zeros(len(xyz), np.float32)]).cuda()
+    arc.deposit(pos)
+    arc.deposit(pos)
+    counts = np.zeros(arc.dims)
+    A.deposit(counts, xyz, box, arc.dims)
+    A.deposit(counts, xyz, box, arc.dims)
+    got, want = arc.density(), A.density(counts, 2, box, arc.dims)
+    assert np.array_equal(got, want), (np.count_nonzero(got != want),
+                                       np.abs(got - want).max())
+    arc.clear()
+    with pytest.raises(Exception):
+        arc.density()
+
+
+def test_reference_grid_pipeline(torch_cuda):
+    """The reference's own grid (151 x 21 x 21, rc 3.0..7.5 step 0.25, target
+    0.0045, two refinements), fed its run-averaged density along x: the same
+    candidates as the oracle's 1-D computation in every cell, and the
+    reference's last rc update in >= 85 % of the liquid/interface bins."""
+    fx = np.load(os.path.join(GOLDEN, "arc_t085.npz"))
+    d, ex = fx["a_density"], fx["error_x"]
+    box = (150.0, 21.0, 21.0)
+    arc = Arc(box, 1.0)
+    rcs = arc.set_candidates(3.0, 7.5, 0.25)
+    nx = arc.dims[0]
+    xc = (np.arange(nx) + 0.5) * 150.0 / nx
+    prof = np.interp(xc, d[:, 0], d[:, 1], period=150.0)
+    arc.set_density(np.broadcast_to(prof[:, None, None], arc.dims))
+    arc.cal_error()
+    arc.cal_rcut(0.0045)
+    arc.refine(2)
+    rc, err, idx = arc.result()
+    dims1 = (nx, 1, 1)
+    S1, G, kv = A.kernels(box, dims1, rcs)
+    ref = A.cal_error(prof.reshape(dims1), box, S1, G, kv)
+    want = A.refine(A.refine(A.cal_rcut(ref, 0.0045)))
+    assert np.array_equal(idx, np.broadcast_to(want, arc.dims))
+    e_col = np.take_along_axis(ref, want[None], 0)[0, :, 0, 0]
+    assert rel(err[:, 3, 7], e_col) < 1e-9
+    dense = prof > 0.05
+    assert np.mean(rc[dense, 0, 0] == ex[dense, 2]) >= 0.85
+
+
+def test_apply_installs_rc_lattice(torch_cuda):
+    g = np.load(os.path.join(GOLDEN, "conf16000.npz"))
+    xyz, box = g["xyz"], g["box"]
+    e = Engine(n_types=1, skin=0.3)
+    e.set_system([len(xyz)], box.reshape(1, 3))
+    e.set_types(np.zeros(len(xyz), np.int32))
+    e.set_pair_params(np.ones((1, 1)), np.ones((1, 1)))
+    pos = torch_cuda.from_numpy(np.c_[xyz, np.zeros(len(xyz), np.float32)]).cuda()
+    torch_cuda.cuda.synchronize()
+    e.bind_positions(pos)
+    arc = Arc(box.astype(np.float64), 1.0)
+    arc.set_candidates(3.0, 6.0, 0.5)
+    for _ in range(3):
+        arc.deposit(e)
+    arc.cal_error()
+    arc.cal_rcut(0.01)
+    arc.refine(1)
+    rc_tab, _, _ = arc.result()
+    arc.apply(e)
+    e.rebuild()
+    rc_atoms = e.rcut()
+    want, _ = O.assign_rc(xyz, box, rc_tab)
+    assert np.array_equal(rc_atoms, want)
+    assert rc_tab.min() < rc_tab.max()          # the slab is inhomogeneous
