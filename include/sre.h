@@ -350,6 +350,8 @@ sre_status sre_arc_refine(sre_arc *a);
 /* Result per cell rc[iz + nz*(iy + ny*ix)] (FP32 as AssignRCut::getRCut
  * copies it), its error and candidate index; any pointer may be NULL. */
 sre_status sre_arc_result(sre_arc *a, float *rc, double *err, int *idx);
+/* load_rc: installs a saved candidate index per cell (values in [0, n_rc)). */
+sre_status sre_arc_set_index(sre_arc *a, const int *idx /* [nele] */);
 /* error[c][cell]; parts[c][cell][4] (needs keep_parts). */
 sre_status sre_arc_errors(sre_arc *a, double *err, double *parts);
 /* Kernel tables S1[c][cell] and G[c][cell] (S2_a = i k_a G). */

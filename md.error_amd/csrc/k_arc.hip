@@ -615,6 +615,25 @@ extern "C" sre_status sre_arc_refine(sre_arc *a) {
   return SRE_OK;
 }
 
+// load_rc: a saved candidate index per cell (the errors are not restored)
+extern "C" sre_status sre_arc_set_index(sre_arc *a, const int *idx) {
+  if (!a || !idx) return fail(SRE_ERR_INVALID, "null argument");
+  if (a->rcs.empty()) return fail(SRE_ERR_NOT_READY, "sre_arc_set_candidates first");
+  const int nrc = (int)a->rcs.size();
+  for (int i = 0; i < a->g.nele; ++i)
+    if (idx[i] < 0 || idx[i] >= nrc)
+      return fail(SRE_ERR_INVALID, "index %d of cell %d outside [0, %d)", idx[i], i, nrc);
+  HIP_TRY(hipSetDevice(a->device));
+  HIP_TRY(hipMemcpyAsync(a->d_idx, idx, sizeof(int) * a->g.nele, hipMemcpyHostToDevice,
+                         a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  if (!a->err_valid)   // no error estimate behind this index: report zeros
+    HIP_TRY(hipMemsetAsync(a->d_err, 0, sizeof(double) * a->g.nele * nrc, a->stream));
+  a->err_valid = true;
+  a->rc_valid = true;
+  return SRE_OK;
+}
+
 extern "C" sre_status sre_arc_result(sre_arc *a, float *rc, double *err, int *idx) {
   if (!a) return fail(SRE_ERR_INVALID, "null arc");
   if (!a->rc_valid) return fail(SRE_ERR_NOT_READY, "sre_arc_cal_rcut first");

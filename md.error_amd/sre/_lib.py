@@ -120,6 +120,7 @@ SIGNATURES = {
     "sre_arc_cal_rcut": (_st, [_vp, C.c_double]),
     "sre_arc_refine": (_st, [_vp]),
     "sre_arc_result": (_st, [_vp, _f32p, _f64p, _i32p]),
+    "sre_arc_set_index": (_st, [_vp, _i32p]),
     "sre_arc_errors": (_st, [_vp, _f64p, _f64p]),
     "sre_arc_kernels": (_st, [_vp, _f64p, _f64p]),
     "sre_arc_apply": (_st, [_vp, _vp, C.c_int]),

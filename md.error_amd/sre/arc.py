@@ -95,6 +95,10 @@ class Arc:
                                    _ptr(idx, C.c_int32)))
         return rc, err, idx
 
+    def set_index(self, idx):
+        i = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(self.dims))
+        check(lib().sre_arc_set_index(self._h, _ptr(i, C.c_int32)))
+
     def errors(self, parts=False):
         nrc = len(self.rcs)
         err = np.empty((nrc,) + self.dims, np.float64)

@@ -26,6 +26,7 @@ exact), and rc assignment happens at every list rebuild.
 """
 import numpy as np
 
+from .arc import Arc
 from .engine import Engine
 
 
@@ -136,6 +137,157 @@ class MDSystem:
         return self.engine
 
 
+class DensityProfile_PiecewiseConst:
+    """Density on a grid of odd edges n = int(L/refh) (H/include/DensityProfile.h:14-59),
+    accumulated on the device (deposite) and averaged by calculate()."""
+
+    def __init__(self):
+        self.arc = None
+        self.profile = None
+
+    def reinit(self, bx, by, bz, refh):
+        self.arc = Arc((bx, by, bz), refh)
+        self.nx, self.ny, self.nz = self.arc.dims
+        self.profile = None
+
+    def getNx(self):
+        return self.nx
+
+    def getNy(self):
+        return self.ny
+
+    def getNz(self):
+        return self.nz
+
+    def getBox(self):
+        return list(self.arc.box)
+
+    def clearData(self):
+        self.arc.clear()
+
+    def deposite(self, coord, numAtom=None):
+        """coord: sys.coord (device (N, 4) float32) or an MDSystem."""
+        if isinstance(coord, MDSystem):
+            coord = coord.coord
+        self.arc.deposit(coord if numAtom is None else coord[:numAtom])
+
+    def calculate(self):
+        self.profile = self.arc.density()
+
+    def getProfile(self, i, j=None, k=None):
+        if j is None:
+            return self.profile.ravel()[i]
+        return self.profile[i, j, k]
+
+    def print_x(self, path):
+        hx = self.arc.box[0] / self.nx
+        with open(path, "w") as f:
+            for i in range(self.nx):
+                f.write("%f %f\n" % ((i + 0.5) * hx, self.profile[i, 0, 0]))
+
+
+class AdaptRCut:
+    """Per-cell cutoff from the estimated force error (H/include/AdaptRCut.h:31-108),
+    computed on the device over the grid of a DensityProfile_PiecewiseConst."""
+
+    def __init__(self):
+        self.dp = None
+        self.rcut = None
+        self.rcutIndex = None
+        self.result_error = None
+
+    def reinit(self, rcmin, rcmax, rcstep, dp):
+        self.dp = dp
+        self.rcList = dp.arc.set_candidates(rcmin, rcmax, rcstep)
+
+    def getNx(self):
+        return self.dp.nx
+
+    def getNy(self):
+        return self.dp.ny
+
+    def getNz(self):
+        return self.dp.nz
+
+    def getRcList(self):
+        return self.rcList
+
+    def calError(self, dp=None):
+        self.dp.arc.cal_error()
+
+    def calRCut(self, prec):
+        self.dp.arc.cal_rcut(prec)
+        self._fetch()
+
+    def refineRCut(self):
+        self.dp.arc.refine()
+        self._fetch()
+
+    def _fetch(self):
+        rc, err, idx = self.dp.arc.result()
+        self.rcutIndex = idx
+        self.rcut = self.rcList[idx]
+        self.result_error = err
+
+    def getRCut(self):
+        return self.rcut
+
+    def getProfileIndex(self):
+        return self.rcutIndex
+
+    def print_x(self, path):
+        hx = self.dp.arc.box[0] / self.dp.nx
+        with open(path, "w") as f:
+            for i in range(self.dp.nx):
+                f.write("%f %e %f\n" % ((i + 0.5) * hx, self.result_error[i, 0, 0],
+                                        self.rcut[i, 0, 0]))
+
+    def print_rc_avg(self, path):
+        hx = self.dp.arc.box[0] / self.dp.nx
+        avg = self.rcut.mean(axis=(1, 2))
+        with open(path, "w") as f:
+            for i in range(self.dp.nx):
+                f.write("%f %e\n" % ((i + 0.5) * hx, avg[i]))
+
+    def print_error_avg(self, dp, path):
+        """sqrt(sum_yz err^2 rho / sum_yz rho) along x."""
+        hx = self.dp.arc.box[0] / self.dp.nx
+        rho = dp.profile
+        num = (self.result_error ** 2 * rho).sum(axis=(1, 2))
+        den = rho.sum(axis=(1, 2))
+        with open(path, "w") as f:
+            for i in range(self.dp.nx):
+                f.write("%f %e\n" % ((i + 0.5) * hx, np.sqrt(num[i] / den[i])))
+
+    def save_rc(self, path):
+        """Text state of H/src/AdaptRCut.cu:698-726: candidates, box, grid, indices."""
+        b = self.dp.arc.box
+        with open(path, "w") as f:
+            f.write("%d " % len(self.rcList) + "".join("%f " % r for r in self.rcList))
+            f.write("\n%f %f %f\n" % tuple(b))
+            f.write("%d %d %d\n" % self.dp.arc.dims)
+            f.write("".join("%d " % v for v in self.rcutIndex.ravel()) + "\n")
+
+    def load_rc(self, path):
+        """Reads save_rc's text back; the candidates must be equally spaced."""
+        with open(path) as f:
+            tok = f.read().split()
+        nrc = int(tok[0])
+        rcs = np.array([float(v) for v in tok[1:1 + nrc]])
+        box = [float(v) for v in tok[1 + nrc:4 + nrc]]
+        dims = tuple(int(v) for v in tok[4 + nrc:7 + nrc])
+        idx = np.array([int(v) for v in tok[7 + nrc:]], np.int32).reshape(dims)
+        if self.dp is None or tuple(self.dp.arc.dims) != dims:
+            raise ValueError(f"{path}: grid {dims} differs from the density grid")
+        step = rcs[1] - rcs[0] if nrc > 1 else 1.0
+        self.rcList = self.dp.arc.set_candidates(rcs[0], rcs[-1] + 0.5 * step, step)
+        if len(self.rcList) != nrc or np.abs(self.rcList - rcs).max() > 1e-5:
+            raise ValueError(f"{path}: candidates are not equally spaced")
+        self.dp.arc.set_index(idx)
+        self._fetch()
+        self.box = box
+
+
 class AssignRCut:
     """rc lattice assigned per atom (H/include/AssignRCut.h:29-37)."""
 
@@ -143,11 +295,16 @@ class AssignRCut:
         self.dims = (1, 1, 1)
         self.table = None
 
-    def reinit(self, sys, dims, NThread=96):
-        self.dims = tuple(int(v) for v in dims)
+    def reinit(self, sys, dims_or_arc, NThread=96):
+        if isinstance(dims_or_arc, AdaptRCut):
+            dims_or_arc = (dims_or_arc.getNx(), dims_or_arc.getNy(), dims_or_arc.getNz())
+        self.dims = tuple(int(v) for v in dims_or_arc)
         self.sys = sys
 
     def getRCut(self, table):
+        """table: an rc lattice or an AdaptRCut (its current result, as FP32)."""
+        if isinstance(table, AdaptRCut):
+            table = table.getRCut()
         self.table = np.asarray(table, np.float32).reshape(self.dims)
         self.sys._rc = self.table
 

@@ -95,3 +95,71 @@ def test_autograd_bridge_matches_finite_differences(torch_cuda):
             fd = (vals[0][0] - vals[1][0]) / (vals[0][1] - vals[1][1])
             g = float(grad[a, b])
             assert abs(fd - g) <= 2e-4 * max(1.0, abs(g)), (which, a, b, fd, g)
+
+
+def test_reference_adaptive_cutoff_sequence(torch_cuda, tmp_path):
+    """The adaptive part of the reference driver (H/app/lj.nhc-xtc.cu:84-100,
+    228-245) through the reference-named API: density grid, rc candidates,
+    deposits, calError / calRCut / refineRCut x2, getRCut into AssignRCut,
+    then the force path with the new lattice; and save_rc / load_rc."""
+    from oracle import arc as A
+    d = np.load(os.path.join(GOLD, "conf16000.npz"))
+    xyz, box = d["xyz"], d["box"]
+    sys = R.MDSystem()
+    sys.initConfig(xyz, box)
+    nb = R.SystemNonBondedInteraction()
+    nb.add(0, 0, R.LennardJones6_12Parameter(1.0, 1.0, 0.0, 7.5))
+    sys.initTopology()
+    sys.initDeviceData()
+    dp = R.DensityProfile_PiecewiseConst()
+    dp.reinit(*box, 1.0)
+    arc = R.AdaptRCut()
+    arc.reinit(3.0, 7.5, 0.25, dp)
+    assign = R.AssignRCut()
+    assign.reinit(sys, arc)
+    assign.uniform(5.0)
+    assign.assign(sys)
+    for _ in range(4):
+        dp.deposite(sys.coord, sys.numAtom)
+    dp.calculate()
+    arc.calError(dp)
+    arc.calRCut(0.0045)
+    for _ in range(2):
+        arc.refineRCut()
+    assign.getRCut(arc)
+    assign.assign(sys)
+    nlist = R.NeighborList(nb, sys, 7.5 + 0.49, 0.49)
+    nlist.rebuild(sys)
+    inter = R.InteractionEngine(sys)
+    inter.registNonBondedInteraction(nb)
+    st = R.MDStatistic(sys)
+    inter.clearInteraction(sys)
+    inter.applyNonBondedInteraction(sys, nlist, st)
+    st.updateHost()
+
+    # the oracle's choice from the same density
+    dims = (dp.getNx(), dp.getNy(), dp.getNz())
+    counts = np.zeros(dims)
+    for _ in range(4):
+        A.deposit(counts, xyz, box, dims)
+    rho = A.density(counts, 4, box, dims)
+    assert np.array_equal(dp.profile, rho)
+    rcs = A.rc_list(3.0, 7.5, 0.25)
+    table = arc.getRCut()
+    assert set(np.unique(table)) <= set(rcs)
+    rc_atoms = sys.engine.rcut()
+    want, _ = O.assign_rc(xyz, box, table.astype(np.float32))
+    assert np.array_equal(rc_atoms, want)
+    off, j = O.build_pairs(xyz, box, want, 0.49)
+    F, E, _, _ = O.forces(xyz, box, None, np.ones((1, 1)), np.ones((1, 1)), want, off, j)
+    assert abs(st.nonBondedEnergy() - E.sum()) / np.abs(E).sum() < 1e-6
+    # save_rc / load_rc round trip (H/src/AdaptRCut.cu:627-726 text format)
+    path = str(tmp_path / "rcut.save")
+    arc.save_rc(path)
+    arc2 = R.AdaptRCut()
+    dp2 = R.DensityProfile_PiecewiseConst()
+    dp2.reinit(*box, 1.0)
+    arc2.reinit(3.0, 7.5, 0.25, dp2)
+    arc2.load_rc(path)
+    assert np.array_equal(arc2.getProfileIndex(), arc.getProfileIndex())
+    assert np.array_equal(arc2.getRCut(), table)
