@@ -20,7 +20,8 @@ included), ms_per_step, the FP64 form measured the same way, the force
 kernel's HBM roofline fraction, force/energy error of the mixed path against
 the FP64 form, the CPU path (the FP64 oracle, OpenMP) timed on the host, and
 under "md" the device MD loop (velocity Verlet + Nose-Hoover chain, the same
-force path) timed on the same slab.
+force path) timed on the same slab, and under "arc" the adaptive-cutoff
+producer (density grid -> error estimate -> rc lattice) on the slab's box.
 """
 import argparse
 import json
@@ -304,6 +305,54 @@ def run_md(w, steps, warmup, torch, Engine):
             "rebuild_policy": "adaptive stretches (snapshot + rollback), skin/2 rule"}
 
 
+def time_arc(w, torch, cpu=True):
+    """Times the adaptive-cutoff producer on the workload's box (grid of ~1 sigma
+    cells, rc candidates rc_min..rc_max step 0.25): the kernel tables (once per
+    grid), then one update = deposit + error estimate of every candidate +
+    choice + two widenings + read-back of the lattice (the reference's
+    every-rcutUpdateFeq work).  cpu: numpy's FFT route with the same kernel
+    tables (the oracle, one thread) on the same grid."""
+    import numpy as np
+    from sre import Arc
+    box = [float(v) for v in w.boxes[0]]
+    rt = w.rc_tables[0]
+    rcmin, rcmax = float(rt.min()), float(rt.max())
+    arc = Arc(box, 1.0)
+    t0 = time.perf_counter()
+    rcs = arc.set_candidates(rcmin, rcmax, 0.25)
+    t_tab = time.perf_counter() - t0
+    pos = torch.from_numpy(w.xyz.copy()).to("cuda")
+    torch.cuda.synchronize()
+
+    def update():
+        arc.clear()
+        arc.deposit(pos)
+        arc.cal_error()
+        arc.cal_rcut(0.0045)
+        arc.refine(2)
+        return arc.result()
+
+    update()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        update()
+        ts.append(time.perf_counter() - t0)
+    out = {"grid": list(arc.dims), "cells": arc.nele, "candidates": len(rcs),
+           "kernel_tables_ms": round(1e3 * t_tab, 2),
+           "update_ms": round(1e3 * float(np.median(ts)), 3)}
+    if cpu:
+        from oracle import arc as A
+        S1, G = arc.kernels()
+        rho = arc.density()
+        kv = A.k_grid(box, arc.dims)
+        t0 = time.perf_counter()
+        A.cal_error(rho, box, S1, G, kv)
+        out["cpu_numpy_fft_error_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+    arc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +463,12 @@ def main():
         result["autotune_s"] = round(mixed["autotune_s"], 2)
     if args.adjoint:
         result["adjoint"] = time_adjoint(mixed["engine"], w, torch)
+    if len(w.n_atoms) == 1:
+        try:
+            result["arc"] = time_arc(w, torch, cpu=not args.no_cpu)
+        except Exception as ex:          # reported in the line, never silent
+            result["arc"] = {"error": f"{type(ex).__name__}: {ex}"}
+        log(f"[bench] arc: {result['arc']}")
     if not args.no_md and len(w.n_atoms) == 1:
         try:
             result["md"] = run_md(w, args.steps, max(args.warmup, 300), torch, Engine)
