@@ -356,6 +356,12 @@ sre_status sre_arc_set_index(sre_arc *a, const int *idx /* [nele] */);
 sre_status sre_arc_errors(sre_arc *a, double *err, double *parts);
 /* Kernel tables S1[c][cell] and G[c][cell] (S2_a = i k_a G). */
 sre_status sre_arc_kernels(sre_arc *a, double *S1, double *G);
+/* PressureCorrection::correction (H/src/PressureCorrection.cu:176-211): the
+ * diagonal pressure correction p[3] = (pxx, pyy, pzz) of the pairs beyond each
+ * cell's chosen rc, from the current density and candidates. */
+sre_status sre_arc_pressure(sre_arc *a, double *p /* [3] */);
+/* Its kernel integrals i1, i5[c][cell] = int_rc^30 -24 x^-4.5 J_{1/2, 5/2}(2 pi k x) dx. */
+sre_status sre_arc_pressure_integrals(sre_arc *a, double *I1, double *I5);
 /* AssignRCut::getRCut: installs the result as the rc lattice of a replica. */
 sre_status sre_arc_apply(sre_arc *a, sre_engine *e, int replica);
 

@@ -6,6 +6,8 @@
 //       (H/src/DensityProfile.cu:27-99)
 //   AdaptRCut::{reinit, makeS1k, makeS2k, calError, calRCut, refineRCut}
 //       (H/src/AdaptRCut.cu:96-163, 265-416, 182-236, 418-468, 470-508)
+//   PressureCorrection::{makeKernel, correction}
+//       (H/src/PressureCorrection.cu:262-355, 176-211)
 //
 // GPU formulation:
 //  * kernel tables: one thread per grid cell integrates x^-13 sin(2 pi k x)
@@ -59,10 +61,31 @@ __device__ inline void arc_k(const ArcGrid &g, const int i[3], double k[3]) {
   }
 }
 
-// makeS1k / makeS2k for every candidate, layout [c][cell]
+// sqrt(2/(pi z)) sin z and sqrt(2z/pi) j2(z): J_{1/2}(z), J_{5/2}(z)
+// (half-integer Bessel functions in closed form; a series for j2 at small z)
+__device__ inline void bessel_half(double z, double sz, double cz, double *j1,
+                                   double *j5) {
+  const double r = sqrt(2.0 / (M_PI * z));
+  *j1 = r * sz;
+  double j2;
+  if (z < 0.5) {
+    const double z2 = z * z;
+    j2 = z2 / 15.0 *
+         (1.0 - z2 / 14.0 * (1.0 - z2 / 36.0 * (1.0 - z2 / 66.0 * (1.0 - z2 / 104.0))));
+  } else {
+    const double iz = 1.0 / z;
+    j2 = (3.0 * iz * iz * iz - iz) * sz - 3.0 * cz * iz * iz;
+  }
+  *j5 = j2 * z * r;   // sqrt(2z/pi) j2 = z sqrt(2/(pi z)) j2
+}
+
+// makeS1k / makeS2k for every candidate, layout [c][cell]; with I1 != NULL
+// also the integrals i1, i5 = int_rc^30 -24 x^-4.5 J_{1/2, 5/2}(2 pi k x) dx
+// of PressureCorrection::makeKernel
 __global__ __launch_bounds__(kArcBlock) void k_arc_kernels(
     ArcGrid g, int nrc, const double *__restrict__ rcs, ArcQuad q,
-    double *__restrict__ S1, double *__restrict__ G) {
+    double *__restrict__ S1, double *__restrict__ G, double *__restrict__ I1,
+    double *__restrict__ I5) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= g.nele) return;
   int i[3];
@@ -76,30 +99,40 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_kernels(
       const double r2 = rc * rc, r4 = r2 * r2, r8 = r4 * r4;
       S1[(size_t)c * g.nele + cell] = 4.0 * M_PI * kArcP1 / 11.0 / (r8 * r2 * rc);
       G[(size_t)c * g.nele + cell] = 0.0;
+      I1[(size_t)c * g.nele + cell] = 0.0;
+      I5[(size_t)c * g.nele + cell] = 0.0;
     }
     return;
   }
   const double hmax = fmin(0.125, 0.125 / k);
-  double acc13 = 0.0, acc5 = 0.0, upper = kArcXMax;
+  double acc13 = 0.0, acc5 = 0.0, accb1 = 0.0, accb5 = 0.0, upper = kArcXMax;
   for (int c = nrc - 1; c >= 0; --c) {
     const double lo = rcs[c];
     const int m = max(1, (int)ceil((upper - lo) / hmax));
     const double hp = (upper - lo) / m;
     for (int p = 0; p < m; ++p) {
       const double mid = lo + (p + 0.5) * hp;
-      double s13 = 0.0, s5 = 0.0;
+      double s13 = 0.0, s5 = 0.0, b1 = 0.0, b5 = 0.0;
 #pragma unroll
       for (int j = 0; j < kGL; ++j) {
         const double x = mid + 0.5 * hp * q.x[j];
-        const double s = sinpi(2.0 * k * x);
+        double s, cs;
+        sincospi(2.0 * k * x, &s, &cs);
         const double inv = 1.0 / x;
         const double i2 = inv * inv, i4 = i2 * i2;
         const double i5 = i4 * inv;
         s13 += q.w[j] * (i5 * i4 * i4) * s;
         s5 += q.w[j] * i5 * s;
+        double j1, j5;
+        bessel_half(2.0 * M_PI * k * x, s, cs, &j1, &j5);
+        const double f = -24.0 * i4 * sqrt(inv);
+        b1 += q.w[j] * f * j1;
+        b5 += q.w[j] * f * j5;
       }
       acc13 += 0.5 * hp * s13;
       acc5 += 0.5 * hp * s5;
+      accb1 += 0.5 * hp * b1;
+      accb5 += 0.5 * hp * b5;
     }
     upper = lo;
     const double rc = lo;
@@ -109,6 +142,71 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_kernels(
                         (2.0 * cos(phi) / phi - 2.0 * sin(phi) / (phi * phi)) / k;
     S1[(size_t)c * g.nele + cell] = 2.0 / k * kArcP1 * acc13;
     G[(size_t)c * g.nele + cell] = 2.0 * M_PI * kArcP2 / k * acc5 + size;
+    I1[(size_t)c * g.nele + cell] = accb1;
+    I5[(size_t)c * g.nele + cell] = accb5;
+  }
+}
+
+// K_aa(k) = (pi / sqrt(k)) (2/3 i1 + (2/3 - 2 khat_a^2) i5), K(0) = 0
+__device__ inline void pressure_kernel(const double kv[3], double i1, double i5,
+                                       double K[3]) {
+  const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+  if (k2 == 0.0) {
+    K[0] = K[1] = K[2] = 0.0;
+    return;
+  }
+  const double k = sqrt(k2);
+  const double sc = M_PI / sqrt(k);
+  for (int a = 0; a < 3; ++a)
+    K[a] = sc * (2.0 / 3.0 * i1 + (2.0 / 3.0 - 2.0 * kv[a] * kv[a] / k2) * i5);
+}
+
+// Z[2c] = (Kxx + i Kyy) rho_k, Z[2c+1] = Kzz rho_k   (rho_k scaled)
+__global__ __launch_bounds__(kArcBlock) void k_arc_pack_pressure(
+    ArcGrid g, int nrc, double scale, const double *__restrict__ I1,
+    const double *__restrict__ I5, const double2 *__restrict__ rhok,
+    double2 *__restrict__ Z) {
+  const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= (size_t)nrc * g.nele) return;
+  const int c = (int)(gi / g.nele);
+  const int cell = (int)(gi % g.nele);
+  int i[3];
+  double kv[3], K[3];
+  arc_cell(g, cell, i);
+  arc_k(g, i, kv);
+  pressure_kernel(kv, I1[gi], I5[gi], K);
+  const double2 r = make_double2(rhok[cell].x * scale, rhok[cell].y * scale);
+  Z[(size_t)(2 * c) * g.nele + cell] =
+      make_double2(K[0] * r.x - K[1] * r.y, K[0] * r.y + K[1] * r.x);
+  Z[(size_t)(2 * c + 1) * g.nele + cell] = make_double2(K[2] * r.x, K[2] * r.y);
+}
+
+// per-block partial sums of 1/2 conv_aa[idx](cell) rho(cell), fixed order
+__global__ __launch_bounds__(kArcBlock) void k_arc_pressure_sum(
+    ArcGrid g, const int *__restrict__ idx, const double *__restrict__ rho,
+    const double2 *__restrict__ Z, double *__restrict__ part) {
+  __shared__ double red[kArcBlock / kWave][3];
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (cell < g.nele) {
+    const int c = idx[cell];
+    const double2 z1 = Z[(size_t)(2 * c) * g.nele + cell];
+    const double2 z2 = Z[(size_t)(2 * c + 1) * g.nele + cell];
+    const double w = 0.5 * rho[cell];
+    v[0] = w * z1.x;
+    v[1] = w * z1.y;
+    v[2] = w * z2.x;
+  }
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v[a] += __shfl_xor(v[a], o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    for (int a = 0; a < 3; ++a) red[threadIdx.x / kWave][a] = v[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kArcBlock / kWave; ++q)
+      for (int a = 0; a < 3; ++a) v[a] += red[q][a];
+    for (int a = 0; a < 3; ++a) part[(size_t)blockIdx.x * 3 + a] = v[a];
   }
 }
 
@@ -330,6 +428,8 @@ struct sre_arc {
   std::vector<double> rcs;
   double *d_rcs = nullptr;
   double *d_S1 = nullptr, *d_G = nullptr;   // [nrc][nele]
+  double *d_I1 = nullptr, *d_I5 = nullptr;  // pressure-kernel integrals [nrc][nele]
+  double *d_psum = nullptr;                 // pressure partial sums, 3 per block
   double2 *d_tw[3][2] = {};                 // per axis: forward, backward
   unsigned *d_counts = nullptr;
   int ndata = 0;
@@ -346,7 +446,7 @@ struct sre_arc {
 };
 
 sre_arc::~sre_arc() {
-  dfree(d_rcs); dfree(d_S1); dfree(d_G);
+  dfree(d_rcs); dfree(d_S1); dfree(d_G); dfree(d_I1); dfree(d_I5); dfree(d_psum);
   for (auto &a : d_tw) { dfree(a[0]); dfree(a[1]); }
   dfree(d_counts); dfree(d_rho); dfree(d_rhok); dfree(d_zr); dfree(d_za);
   dfree(d_zb); dfree(d_err); dfree(d_parts); dfree(d_idx); dfree(d_idx2);
@@ -417,6 +517,7 @@ extern "C" sre_status sre_arc_create(sre_arc **out, const double box[3],
     HIP_TRY(dalloc(&a->d_idx2, nele));
     HIP_TRY(dalloc(&a->d_rc, nele));
     HIP_TRY(dalloc(&a->d_res_err, nele));
+    HIP_TRY(dalloc(&a->d_psum, (size_t)3 * arc_blocks(nele)));
     for (int d = 0; d < 3; ++d)
       for (int dir = 0; dir < 2; ++dir) {
         HIP_TRY(dalloc(&a->d_tw[d][dir], a->g.n[d]));
@@ -450,6 +551,7 @@ extern "C" sre_status sre_arc_set_candidates(sre_arc *a, double rcmin, double rc
   HIP_TRY(hipSetDevice(a->device));
   HIP_TRY(hipStreamSynchronize(a->stream));
   dfree(a->d_rcs); dfree(a->d_S1); dfree(a->d_G); dfree(a->d_za); dfree(a->d_zb);
+  dfree(a->d_I1); dfree(a->d_I5);
   dfree(a->d_err); dfree(a->d_parts);
   a->rcs = rcs;
   a->err_valid = a->rc_valid = false;
@@ -458,13 +560,16 @@ extern "C" sre_status sre_arc_set_candidates(sre_arc *a, double rcmin, double rc
   HIP_TRY(dalloc(&a->d_rcs, nrc));
   HIP_TRY(dalloc(&a->d_S1, tab));
   HIP_TRY(dalloc(&a->d_G, tab));
+  HIP_TRY(dalloc(&a->d_I1, tab));
+  HIP_TRY(dalloc(&a->d_I5, tab));
   HIP_TRY(dalloc(&a->d_za, 2 * tab));
   HIP_TRY(dalloc(&a->d_zb, 2 * tab));
   HIP_TRY(dalloc(&a->d_err, tab));
   HIP_TRY(hipMemcpyAsync(a->d_rcs, rcs.data(), sizeof(double) * nrc,
                          hipMemcpyHostToDevice, a->stream));
   hipLaunchKernelGGL(k_arc_kernels, dim3(arc_blocks(a->g.nele)), dim3(kArcBlock), 0,
-                     a->stream, a->g, nrc, a->d_rcs, gauss_legendre(), a->d_S1, a->d_G);
+                     a->stream, a->g, nrc, a->d_rcs, gauss_legendre(), a->d_S1, a->d_G,
+                     a->d_I1, a->d_I5);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(a->stream));
   return SRE_OK;
@@ -683,6 +788,59 @@ extern "C" sre_status sre_arc_kernels(sre_arc *a, double *S1, double *G) {
                            a->stream));
   if (G)
     HIP_TRY(hipMemcpyAsync(G, a->d_G, sizeof(double) * tab, hipMemcpyDeviceToHost,
+                           a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  return SRE_OK;
+}
+
+// PressureCorrection::correction: p_aa = sum_cells 1/2 conv_aa[idx](cell)
+// rho(cell) dV / V^2 with the current density and candidate per cell
+extern "C" sre_status sre_arc_pressure(sre_arc *a, double *p) {
+  if (!a || !p) return fail(SRE_ERR_INVALID, "null argument");
+  if (!a->rc_valid) return fail(SRE_ERR_NOT_READY, "sre_arc_cal_rcut first");
+  HIP_TRY(hipSetDevice(a->device));
+  sre_status s = arc_calculate(a);
+  if (s != SRE_OK) return s;
+  const ArcGrid &g = a->g;
+  const int nrc = (int)a->rcs.size();
+  const size_t tab = (size_t)nrc * g.nele;
+  const double vol = g.L[0] * g.L[1] * g.L[2];
+  hipLaunchKernelGGL(k_arc_real_to_complex, dim3(arc_blocks(g.nele)), dim3(kArcBlock),
+                     0, a->stream, g.nele, a->d_rho, a->d_rhok);
+  s = arc_dft3(a, 1, 0, a->d_rhok, a->d_zr);
+  if (s != SRE_OK) return s;
+  hipLaunchKernelGGL(k_arc_pack_pressure, dim3(arc_blocks(tab)), dim3(kArcBlock), 0,
+                     a->stream, g, nrc, vol / g.nele, a->d_I1, a->d_I5, a->d_rhok,
+                     a->d_za);
+  s = arc_dft3(a, 2 * nrc, 1, a->d_za, a->d_zb);
+  if (s != SRE_OK) return s;
+  const unsigned nb = arc_blocks(g.nele);
+  hipLaunchKernelGGL(k_arc_pressure_sum, dim3(nb), dim3(kArcBlock), 0, a->stream, g,
+                     a->d_idx, a->d_rho, a->d_za, a->d_psum);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> part((size_t)3 * nb);
+  HIP_TRY(hipMemcpyAsync(part.data(), a->d_psum, sizeof(double) * part.size(),
+                         hipMemcpyDeviceToHost, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  const double dv = (g.L[0] / g.n[0]) * (g.L[1] / g.n[1]) * (g.L[2] / g.n[2]);
+  for (int d = 0; d < 3; ++d) {
+    double acc = 0.0;
+    for (unsigned b = 0; b < nb; ++b) acc += part[(size_t)b * 3 + d];
+    p[d] = acc / vol * dv / vol;
+  }
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_arc_pressure_integrals(sre_arc *a, double *I1, double *I5) {
+  if (!a) return fail(SRE_ERR_INVALID, "null arc");
+  if (a->rcs.empty()) return fail(SRE_ERR_NOT_READY, "sre_arc_set_candidates first");
+  HIP_TRY(hipSetDevice(a->device));
+  const size_t tab = a->rcs.size() * (size_t)a->g.nele;
+  if (I1)
+    HIP_TRY(hipMemcpyAsync(I1, a->d_I1, sizeof(double) * tab, hipMemcpyDeviceToHost,
+                           a->stream));
+  if (I5)
+    HIP_TRY(hipMemcpyAsync(I5, a->d_I5, sizeof(double) * tab, hipMemcpyDeviceToHost,
                            a->stream));
   HIP_TRY(hipStreamSynchronize(a->stream));
   return SRE_OK;

@@ -124,6 +124,8 @@ SIGNATURES = {
     "sre_arc_errors": (_st, [_vp, _f64p, _f64p]),
     "sre_arc_kernels": (_st, [_vp, _f64p, _f64p]),
     "sre_arc_apply": (_st, [_vp, _vp, C.c_int]),
+    "sre_arc_pressure": (_st, [_vp, _f64p]),
+    "sre_arc_pressure_integrals": (_st, [_vp, _f64p, _f64p]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_double, C.c_double, C.c_uint64, _f32p]),

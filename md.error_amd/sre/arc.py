@@ -114,5 +114,19 @@ class Arc:
         check(lib().sre_arc_kernels(self._h, _ptr(S1, C.c_double), _ptr(G, C.c_double)))
         return S1, G
 
+    def pressure(self):
+        """(pxx, pyy, pzz) correction for the current density and candidates."""
+        p = np.zeros(3, np.float64)
+        check(lib().sre_arc_pressure(self._h, _ptr(p, C.c_double)))
+        return p
+
+    def pressure_integrals(self):
+        nrc = len(self.rcs)
+        I1 = np.empty((nrc,) + self.dims, np.float64)
+        I5 = np.empty((nrc,) + self.dims, np.float64)
+        check(lib().sre_arc_pressure_integrals(self._h, _ptr(I1, C.c_double),
+                                               _ptr(I5, C.c_double)))
+        return I1, I5
+
     def apply(self, engine, replica=0):
         check(lib().sre_arc_apply(self._h, engine._h, int(replica)))

@@ -288,6 +288,22 @@ class AdaptRCut:
         self.box = box
 
 
+class PressureCorrection:
+    """Pressure-tensor correction of the pairs beyond each cell's rc
+    (H/include/PressureCorrection.h:54-99), on the device over the AdaptRCut grid."""
+
+    def __init__(self, arc=None, dp=None):
+        self.pxx = self.pyy = self.pzz = 0.0
+        if arc is not None:
+            self.reinit(arc, dp)
+
+    def reinit(self, arc, dp=None):
+        self.arc = arc
+
+    def correction(self, arc=None, dp=None):
+        self.pxx, self.pyy, self.pzz = (float(v) for v in (arc or self.arc).dp.arc.pressure())
+
+
 class AssignRCut:
     """rc lattice assigned per atom (H/include/AssignRCut.h:29-37)."""
 

@@ -35,7 +35,9 @@ e1 = rho0 S1(0) + a S1(k0) cos(2 pi k0 x) exactly; for a planar profile the
 mean error force equals the closed-form planar tail force of oracle.oracle
 (independent formulation); the refine equals scipy.ndimage's wrapped maximum
 filter; the bisection equals a brute-force search when the error decreases
-with rc.
+with rc.  Pressure correction: the trace of the kernels equals the transform of
+the isotropic virial -24 r^-6, and for rho(z) the FFT route equals the direct
+planar double integral with closed-form transverse kernels (both components).
 """
 import numpy as np
 from scipy import integrate
@@ -176,6 +178,57 @@ def refine(idx):
         for dy in (-1, 0, 1):
             for dz in (-1, 0, 1):
                 out = np.maximum(out, np.roll(idx, (dx, dy, dz), axis=(0, 1, 2)))
+    return out
+
+
+def _int_bessel(nu, k, a, b):
+    """int_a^b -24 x^-4.5 J_nu(2 pi k x) dx (FF_I1 / FF_I5, exact Bessel)."""
+    from scipy import special
+    return integrate.quad(lambda x: -24.0 * x ** -4.5 * special.jv(nu, 2 * np.pi * k * x),
+                          a, b, epsabs=1e-17, epsrel=1e-13, limit=800)[0]
+
+
+def pressure_kernels(box, dims, rcs):
+    """Kxx, Kyy, Kzz[c] on the k grid (PressureCorrection::makeKernel,
+    H/src/PressureCorrection.cu:262-355): with i1, i5 = int_rc^30 -24 x^-4.5
+    J_{1/2 or 5/2}(2 pi k x) dx and the direction of k,
+    K_aa = (pi / sqrt(k)) (2/3 i1 + (2/3 - 2 khat_a^2) i5);  K(0) = 0."""
+    kx, ky, kz = k_grid(box, dims)
+    k = np.sqrt(kx * kx + ky * ky + kz * kz)
+    K = np.zeros((3, len(rcs)) + k.shape)
+    uk, inv = np.unique(k.ravel(), return_inverse=True)
+    for c, rc in enumerate(rcs):
+        i1 = np.zeros(uk.size)
+        i5 = np.zeros(uk.size)
+        for q, kk in enumerate(uk):
+            if kk == 0.0:
+                continue
+            i1[q] = _int_bessel(0.5, kk, rc, X_MAX)
+            i5[q] = _int_bessel(2.5, kk, rc, X_MAX)
+        I1 = i1[inv].reshape(k.shape)
+        I5 = i5[inv].reshape(k.shape)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            scale = np.where(k > 0, np.pi / np.sqrt(np.where(k > 0, k, 1.0)), 0.0)
+            for a, ka in enumerate((kx, ky, kz)):
+                kh2 = np.where(k > 0, (ka / np.where(k > 0, k, 1.0)) ** 2, 0.0)
+                K[a, c] = scale * (2.0 / 3.0 * I1 + (2.0 / 3.0 - 2.0 * kh2) * I5)
+    return K
+
+
+def pressure_correction(rho, box, K, idx):
+    """PressureCorrection::correction (H/src/PressureCorrection.cu:176-211):
+    p_aa = sum_cells 1/2 conv_aa[idx](cell) rho(cell) dV / V^2,
+    conv = IFFT(K rho_k) (unnormalised), rho_k = FFT(rho) V / n."""
+    b = np.asarray([float(np.float32(v)) for v in box])
+    V = float(np.prod(b))
+    n = rho.size
+    dv = np.prod(b / np.asarray(rho.shape))
+    rhok = np.fft.fftn(rho) * (V / n)
+    out = np.zeros(3)
+    for a in range(3):
+        conv = np.array([np.fft.ifftn(K[a, c] * rhok).real * n for c in range(K.shape[1])])
+        sel = np.take_along_axis(conv, idx[None], 0)[0]
+        out[a] = (0.5 * sel * rho / V * dv).sum() / V
     return out
 
 

@@ -157,3 +157,52 @@ def test_apply_installs_rc_lattice(torch_cuda):
     want, _ = O.assign_rc(xyz, box, rc_tab)
     assert np.array_equal(rc_atoms, want)
     assert rc_tab.min() < rc_tab.max()          # the slab is inhomogeneous
+
+
+def test_pressure_correction(small):
+    arc = small["arc"]
+    box, rcs = small["box"], small["rcs"]
+    I1, I5 = arc.pressure_integrals()
+    # spot-check the integrals against the oracle's exact-Bessel quadrature
+    kx, ky, kz = A.k_grid(box, arc.dims)
+    k = np.sqrt(kx * kx + ky * ky + kz * kz)
+    for cell in [(1, 0, 0), (3, 2, 5), (6, 4, 7)]:
+        for c in (0, len(rcs) - 1):
+            assert abs(I1[(c,) + cell] / A._int_bessel(0.5, k[cell], rcs[c], 30.0) - 1) < 1e-10
+            assert abs(I5[(c,) + cell] / A._int_bessel(2.5, k[cell], rcs[c], 30.0) - 1) < 1e-9
+    K = A.pressure_kernels(box, arc.dims, rcs)
+    arc.set_density(small["rho"])
+    arc.cal_error()
+    for prec, n_ref in ((0.01, 0), (0.004, 2)):
+        arc.cal_rcut(prec)
+        arc.refine(n_ref)
+        _, _, idx = arc.result()
+        want = A.pressure_correction(small["rho"], box, K, idx.astype(np.int64))
+        got = arc.pressure()
+        assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
+
+
+def test_reference_pressure_correction_class(torch_cuda):
+    from sre import reference_api as R
+    g = np.load(os.path.join(GOLDEN, "conf16000.npz"))
+    box = g["box"]
+    sys = R.MDSystem()
+    sys.initConfig(g["xyz"], box)
+    sys.initTopology()
+    sys.initDeviceData()
+    dp = R.DensityProfile_PiecewiseConst()
+    dp.reinit(*box, 1.0)
+    arc = R.AdaptRCut()
+    arc.reinit(3.0, 4.0, 0.5, dp)        # few candidates: the oracle's kernels are slow
+    pc = R.PressureCorrection(arc, dp)
+    dp.deposite(sys)
+    dp.calculate()
+    arc.calError(dp)
+    arc.calRCut(0.01)
+    arc.refineRCut()
+    pc.correction(arc, dp)
+    K = A.pressure_kernels(box, dp.arc.dims, arc.getRcList())
+    want = A.pressure_correction(dp.profile, box, K, arc.getProfileIndex().astype(np.int64))
+    got = np.array([pc.pxx, pc.pyy, pc.pzz])
+    assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
+    assert got[0] != got[1]                      # the configuration is not planar-symmetric

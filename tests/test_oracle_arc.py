@@ -178,3 +178,45 @@ def test_deposit_and_density():
     hist = np.histogram(np.mod(xyz[:, 0].astype(np.float64), box[0]), bins=dims[0],
                         range=(0, float(box[0])))[0]
     assert np.abs(counts.sum(axis=(1, 2)) / 2 - hist).max() <= 2
+
+
+def test_pressure_kernel_trace_is_isotropic_transform():
+    """Kxx + Kyy + Kzz = FT of the isotropic virial -24 r^-6 on rc < r < 30."""
+    k, rc = 0.25, 3.0
+    box, dims = (1.0 / k,) * 3, (3, 1, 1)
+    K = A.pressure_kernels(box, dims, np.array([rc]))
+    ft = integrate.quad(lambda r: 4 * np.pi * r * r * (-24.0 * r ** -6)
+                        * special.spherical_jn(0, 2 * np.pi * k * r), rc, A.X_MAX,
+                        limit=2000, epsabs=1e-18, epsrel=1e-13)[0]
+    assert abs(K[:, 0, 1, 0, 0].sum() / ft - 1) < 1e-10
+    assert np.all(K[:, 0, 0, 0, 0] == 0.0)
+
+
+def test_planar_pressure_correction_matches_direct_integral():
+    """For rho(z) and uniform rc the FFT route equals the direct planar double
+    integral (1/2V) A int int rho(z) (rho(z') - rho_mean) W_aa(z - z') with the
+    closed-form transverse integrals W_zz(s) = -8 pi s^2 R^-6 and
+    W_xx(s) = -24 pi (R^-4/4 - s^2 R^-6/6), R = max(rc, |s|) (the mean is
+    removed because the reference's kernel is zero at k = 0)."""
+    Lz, nz, rc = 60.0, 481, 3.0
+    box, dims = (12.0, 12.0, Lz), (1, 1, nz)
+    z = np.arange(nz) * Lz / nz
+    rho = 0.02 + 0.37 * (np.tanh((z - 20) / 2.5) - np.tanh((z - 40) / 2.5))
+    K = A.pressure_kernels(box, dims, np.array([rc]))
+    p = A.pressure_correction(rho.reshape(dims), box, K, np.zeros(dims, np.int64))
+    V, area, h = 12.0 * 12.0 * Lz, 144.0, Lz / nz
+    dr = rho - rho.mean()
+    s = z[:, None] - z[None, :]
+    direct = {}
+    for comp in ("xx", "zz"):
+        acc = np.zeros(nz)
+        for img in range(-2, 3):
+            t = s + img * Lz
+            R = np.maximum(rc, np.abs(t))
+            W = (-8 * np.pi * t * t * R ** -6 if comp == "zz"
+                 else -24 * np.pi * (R ** -4 / 4 - t * t * R ** -6 / 6))
+            acc += (W * dr[None, :]).sum(1) * h
+        direct[comp] = 0.5 * area * (rho * acc).sum() * h / V
+    assert abs(p[0] / direct["xx"] - 1) < 2e-3 and p[0] == p[1]
+    assert abs(p[2] / direct["zz"] - 1) < 2e-3
+    assert abs(p[2] - p[0]) > 1e-3 * abs(p[0])       # anisotropic at the interfaces
