@@ -64,6 +64,7 @@ typedef enum {
 /* compute flags */
 #define SRE_STATS 1u        /* also energy and virial (the statistic kernel variant) */
 #define SRE_TAIL 2u         /* add the per-atom inhomogeneous tail correction */
+#define SRE_FCORR 8u        /* add the grid force correction (sre_set_force_grid) */
 
 /* tail-correction evaluation modes (sre_set_tail_mode) */
 #define SRE_TAIL_BIN 0      /* value at the rc-bin centre, assigned per atom by bin (FC/src/AssignForceCorr.cu:95-128) */
@@ -261,6 +262,15 @@ sre_status sre_time_compute(sre_engine *e, unsigned flags, int n_iter,
 sre_status sre_time_rebuild(sre_engine *e, int n_iter, double *ms_per_iter);
 sre_status sre_synchronize(sre_engine *e);
 
+/* Grid force correction of replica r: f[3*(iz + nz*(iy + ny*ix)) + a] (host,
+ * FP32).  With SRE_FCORR every force evaluation adds to each atom the value
+ * of the cell its position falls in, by the reference's FP32 rule
+ * (AssignForceCorr::getForceCorr + assign, FC/src/AssignForceCorr.cu:44-145;
+ * the add FC/src/InteractionEngine_interface.cu:1118).  Energy and virial are
+ * not changed.  f = NULL removes the grid. */
+sre_status sre_set_force_grid(sre_engine *e, int replica, int nx, int ny, int nz,
+                              const float *f);
+
 /* ---- MD integrator ------------------------------------------------------- */
 /* The MD loop around the force path (H/app/lj.nhc-xtc.cu:158-196), device-
  * resident: per step the centre-of-mass velocity removal
@@ -362,6 +372,12 @@ sre_status sre_arc_kernels(sre_arc *a, double *S1, double *G);
 sre_status sre_arc_pressure(sre_arc *a, double *p /* [3] */);
 /* Its kernel integrals i1, i5[c][cell] = int_rc^30 -24 x^-4.5 J_{1/2, 5/2}(2 pi k x) dx. */
 sre_status sre_arc_pressure_integrals(sre_arc *a, double *I1, double *I5);
+/* ForceCorr (FC/src/ForceCorr.cu:131-162): the mean error force of each cell
+ * at its chosen candidate, f[3*cell + a] (needs sre_arc_cal_error(a, 1)); a
+ * single candidate (rcmin = rcmax) gives the uniform-rc force correction. */
+sre_status sre_arc_force_grid(sre_arc *a, float *f);
+/* AssignForceCorr::getForceCorr: sre_set_force_grid of that grid. */
+sre_status sre_arc_apply_force_correction(sre_arc *a, sre_engine *e, int replica);
 /* AssignRCut::getRCut: installs the result as the rc lattice of a replica. */
 sre_status sre_arc_apply(sre_arc *a, sre_engine *e, int replica);
 

@@ -88,6 +88,12 @@ struct sre_engine {
   int tail_bins = 0;
   double2 *d_tail_atom = nullptr;
   int tail_mode = SRE_TAIL_BIN;
+  // grid force correction, per replica f[3 * cell + a]; concatenated float4
+  std::vector<std::vector<float>> fc_tab;
+  std::vector<int> fc_dims;
+  float4 *d_fc = nullptr;
+  size_t fc_cap = 0;
+  int fc_cells = 0;
   bool tail_dirty = true;          // profile/rc layout changed: rebuild tables
   bool tail_values_dirty = false;  // only the LJ parameters changed
 
@@ -214,6 +220,7 @@ sre_engine::~sre_engine() {
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
   dfree(d_adj); dfree(d_adj_tw);
   dfree(d_mass); dfree(d_vel); dfree(d_fcall); dfree(d_md_part); dfree(d_md);
+  dfree(d_fc);
   dfree(d_snap_pos); dfree(d_snap_vel); dfree(d_snap_fcall); dfree(d_snap_part);
   dfree(d_snap_md);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -332,8 +339,12 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
     R.n_atoms = n_atoms[r];
     R.tail_off = -1;
     R.rho_off = -1;
+    R.fc_off = -1;
     off += n_atoms[r];
   }
+  e->fc_tab.assign(n_replicas, std::vector<float>());
+  e->fc_dims.assign(3 * n_replicas, 0);
+  e->fc_cells = 0;
   // host-side replica map, uploaded once
   std::vector<int> rep_of(e->n_atoms);
   for (int r = 0; r < n_replicas; ++r)
@@ -489,6 +500,62 @@ extern "C" sre_status sre_set_rc_table(sre_engine *e, int replica, int nx,
   e->tail_dirty = true;
   e->built = false;
   return SRE_OK;
+}
+
+// Concatenates the replicas' force grids into d_fc (float4 per cell), sets
+// fc_off / fc_n and uploads the replica table (stream-ordered, synchronous).
+static sre_status upload_force_grids(sre_engine *e) {
+  std::vector<float4> all;
+  for (int r = 0; r < e->n_rep; ++r) {
+    RepInfo &R = e->reps[r];
+    if (e->fc_tab[r].empty()) {
+      R.fc_off = -1;
+      continue;
+    }
+    R.fc_off = (int)all.size();
+    for (int d = 0; d < 3; ++d) R.fc_n[d] = e->fc_dims[3 * r + d];
+    const std::vector<float> &t = e->fc_tab[r];
+    for (size_t c = 0; c < t.size() / 3; ++c)
+      all.push_back(make_float4(t[3 * c], t[3 * c + 1], t[3 * c + 2], 0.f));
+  }
+  if (all.size() > e->fc_cap) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    dfree(e->d_fc);
+    HIP_TRY(dalloc(&e->d_fc, all.size()));
+    e->fc_cap = all.size();
+    e->graph_epoch++;   // captured launches read d_fc by address
+  }
+  e->fc_cells = (int)all.size();
+  if (!all.empty())
+    HIP_TRY(hipMemcpyAsync(e->d_fc, all.data(), sizeof(float4) * all.size(),
+                           hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_reps, e->reps.data(), sizeof(RepInfo) * e->n_rep,
+                         hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->reps_uploaded = e->reps;
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_set_force_grid(sre_engine *e, int replica, int nx, int ny,
+                                         int nz, const float *f) {
+  if (!e || !e->system_set || replica < 0 || replica >= e->n_rep)
+    return fail(SRE_ERR_INVALID, "sre_set_force_grid: bad replica");
+  HIP_TRY(hipSetDevice(e->device));
+  if (!f) {
+    e->fc_tab[replica].clear();
+    return upload_force_grids(e);
+  }
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz > (1ll << 26))
+    return fail(SRE_ERR_INVALID, "sre_set_force_grid: bad grid %d x %d x %d", nx, ny, nz);
+  const size_t n = (size_t)3 * nx * ny * nz;
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(f[i]))
+      return fail(SRE_ERR_INVALID, "sre_set_force_grid: value %zu is not finite", i);
+  e->fc_tab[replica].assign(f, f + n);
+  e->fc_dims[3 * replica] = nx;
+  e->fc_dims[3 * replica + 1] = ny;
+  e->fc_dims[3 * replica + 2] = nz;
+  return upload_force_grids(e);
 }
 
 extern "C" sre_status sre_set_density_profile(sre_engine *e, int replica,
@@ -966,6 +1033,10 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
   a.step_ctr = step_ctr;
   launch_force(e->precision, T, e->tune.force_unroll, e->tune.block, stats, tail,
                a, e->stream);
+  if ((flags & SRE_FCORR) && e->fc_cells > 0)
+    launch_add_force_grid(e->n_atoms, e->d_pos, e->pos_stride, e->d_backup,
+                          e->d_inv_perm, e->d_reps, e->d_fc, e->d_fx, e->d_fy,
+                          e->d_fz, e->d_fxd, e->d_fyd, e->d_fzd, e->stream);
   if (stats) e->stats_pending = true;
   HIP_TRY(hipGetLastError());  // launch-configuration errors
   return SRE_OK;
@@ -1329,7 +1400,7 @@ static sre_status time_region(sre_engine *e, int n_iter, double *ms,
 
 // one force launch (the statistic reduction is deferred to the read)
 static sre_status body_compute(sre_engine *e, unsigned flags) {
-  return launch_compute(e, flags & (SRE_STATS | SRE_TAIL));
+  return launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR));
 }
 
 extern "C" sre_status sre_time_compute(sre_engine *e, unsigned flags,
@@ -1498,7 +1569,7 @@ static void enqueue_md_kick(sre_engine *e, float hdt) {
 
 static sre_status enqueue_md_step(sre_engine *e, unsigned flags) {
   enqueue_md_drift(e);
-  sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL), nullptr);
+  sre_status s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR), nullptr);
   if (s != SRE_OK) return s;
   enqueue_md_kick(e, (float)(0.5 * e->md.dt));
   return SRE_OK;
@@ -1540,7 +1611,7 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
                               e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
                               e->d_skin_part, pert ? &pa : nullptr, nullptr,
                               e->stream);
-      s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL),
+      s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
                          pert ? e->d_step_ctr : nullptr);
     }
     hipError_t r = hipStreamEndCapture(e->stream, &g);
@@ -1884,7 +1955,7 @@ static sre_status md_snapshot(sre_engine *e, bool restore) {
 extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
                                  unsigned flags, int *n_rebuilds) {
   if (!e || n_steps < 0 || rebuild_every < SRE_MD_ADAPTIVE ||
-      (flags & ~(unsigned)(SRE_STATS | SRE_TAIL)))
+      (flags & ~(unsigned)(SRE_STATS | SRE_TAIL | SRE_FCORR)))
     return fail(SRE_ERR_INVALID, "sre_md_run: bad arguments");
   if (!e->md_set) return fail(SRE_ERR_NOT_READY, "integrator not set");
   if (!e->vel_set) return fail(SRE_ERR_NOT_READY, "velocities not set");
@@ -1911,7 +1982,7 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
     if (n_rebuilds) ++*n_rebuilds;
   }
   refresh_tail_values(e);
-  const unsigned fl = flags & SRE_TAIL;
+  const unsigned fl = flags & (SRE_TAIL | SRE_FCORR);
   // forces at the current positions, then a kick of length 0: saves them in
   // caller order and leaves the kinetic energy and momentum sums
   s = launch_compute(e, fl);

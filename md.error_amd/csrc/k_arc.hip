@@ -372,6 +372,18 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_deposit(
   atomicAdd(&counts[ix[2] + g.n[2] * (ix[1] + g.n[1] * ix[0])], 1u);
 }
 
+// the mean error force (e2x, e2y, e2z) of each cell at its chosen candidate
+__global__ __launch_bounds__(kArcBlock) void k_arc_force_grid(
+    int nele, const int *__restrict__ idx, const double *__restrict__ parts,
+    float *__restrict__ out) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= nele) return;
+  const double *p = parts + ((size_t)idx[cell] * nele + cell) * 4;
+  out[3 * (size_t)cell] = (float)p[1];
+  out[3 * (size_t)cell + 1] = (float)p[2];
+  out[3 * (size_t)cell + 2] = (float)p[3];
+}
+
 __global__ void k_arc_density(int nele, double norm, const unsigned *counts,
                               double *rho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,6 +453,8 @@ struct sre_arc {
   int *d_idx = nullptr, *d_idx2 = nullptr;
   float *d_rc = nullptr;
   double *d_res_err = nullptr;
+  float *d_fgrid = nullptr;                 // [nele][3], on demand
+  bool parts_valid = false;
   bool err_valid = false, rc_valid = false;
   ~sre_arc();
 };
@@ -450,7 +464,7 @@ sre_arc::~sre_arc() {
   for (auto &a : d_tw) { dfree(a[0]); dfree(a[1]); }
   dfree(d_counts); dfree(d_rho); dfree(d_rhok); dfree(d_zr); dfree(d_za);
   dfree(d_zb); dfree(d_err); dfree(d_parts); dfree(d_idx); dfree(d_idx2);
-  dfree(d_rc); dfree(d_res_err);
+  dfree(d_rc); dfree(d_res_err); dfree(d_fgrid);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -693,6 +707,7 @@ extern "C" sre_status sre_arc_cal_error(sre_arc *a, int keep_parts) {
                      keep_parts ? a->d_parts : nullptr);
   HIP_TRY(hipGetLastError());
   a->err_valid = true;
+  a->parts_valid = keep_parts != 0;
   a->rc_valid = false;
   return SRE_OK;
 }
@@ -844,6 +859,35 @@ extern "C" sre_status sre_arc_pressure_integrals(sre_arc *a, double *I1, double 
                            a->stream));
   HIP_TRY(hipStreamSynchronize(a->stream));
   return SRE_OK;
+}
+
+// ForceCorr::getForceCorr{X,Y,Z}: the mean error force at each cell's chosen
+// candidate, f[3 * cell + a] (FP32, as AssignForceCorr::getForceCorr copies it)
+extern "C" sre_status sre_arc_force_grid(sre_arc *a, float *f) {
+  if (!a || !f) return fail(SRE_ERR_INVALID, "null argument");
+  if (!a->rc_valid) return fail(SRE_ERR_NOT_READY, "sre_arc_cal_rcut first");
+  if (!a->parts_valid)
+    return fail(SRE_ERR_NOT_READY, "the force grid needs sre_arc_cal_error(a, 1)");
+  HIP_TRY(hipSetDevice(a->device));
+  const int nele = a->g.nele;
+  if (!a->d_fgrid) HIP_TRY(dalloc(&a->d_fgrid, (size_t)3 * nele));
+  hipLaunchKernelGGL(k_arc_force_grid, dim3(arc_blocks(nele)), dim3(kArcBlock), 0,
+                     a->stream, nele, a->d_idx, a->d_parts, a->d_fgrid);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(f, a->d_fgrid, sizeof(float) * 3 * nele, hipMemcpyDeviceToHost,
+                         a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  return SRE_OK;
+}
+
+// AssignForceCorr::getForceCorr: installs the force grid into an engine replica
+extern "C" sre_status sre_arc_apply_force_correction(sre_arc *a, sre_engine *e,
+                                                     int replica) {
+  if (!a || !e) return fail(SRE_ERR_INVALID, "null argument");
+  std::vector<float> f((size_t)3 * a->g.nele);
+  sre_status s = sre_arc_force_grid(a, f.data());
+  if (s != SRE_OK) return s;
+  return sre_set_force_grid(e, replica, a->g.n[0], a->g.n[1], a->g.n[2], f.data());
 }
 
 extern "C" sre_status sre_arc_apply(sre_arc *a, sre_engine *e, int replica) {

@@ -54,6 +54,8 @@ struct RepInfo {
   int rho_off;      // offset of rho[n_types][rho_nz], -1 if none
   int rho_nz;
   float rc_min, rc_max;
+  int fc_off;       // first cell of the force-correction grid, -1 if none
+  int fc_n[3];      // force-correction grid dims
 };
 
 // LJ pair constants in the form used by the kernels:
@@ -242,6 +244,13 @@ struct DriftArgs {
   const MDState *state;
   float dt;
 };
+// Grid force correction (AssignForceCorr::assign + the force add of the
+// force-correction engine): per atom the cell of its current position by the
+// reference's FP32 rule, F += grid[cell].
+void launch_add_force_grid(int n, const float *pos, int stride, const uint4 *backup,
+                           const uint32_t *inv_perm, const RepInfo *reps,
+                           const float4 *grid, int64_t *fx, int64_t *fy, int64_t *fz,
+                           double *fxd, double *fyd, double *fzd, hipStream_t st);
 int md_part_count(int n);
 void launch_md_kick(int n, const uint32_t *inv_perm, const int *types,
                     const int64_t *fx, const int64_t *fy, const int64_t *fz,

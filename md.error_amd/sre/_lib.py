@@ -26,6 +26,7 @@ PREC_FP64 = 1
 STATS = 1
 TAIL = 2
 PERTURB = 4
+FCORR = 8
 TAIL_BIN = 0
 TAIL_EXACT = 1
 THERMO_NONE = 0
@@ -125,6 +126,9 @@ SIGNATURES = {
     "sre_arc_kernels": (_st, [_vp, _f64p, _f64p]),
     "sre_arc_apply": (_st, [_vp, _vp, C.c_int]),
     "sre_arc_pressure": (_st, [_vp, _f64p]),
+    "sre_arc_force_grid": (_st, [_vp, _f32p]),
+    "sre_arc_apply_force_correction": (_st, [_vp, _vp, C.c_int]),
+    "sre_set_force_grid": (_st, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p]),
     "sre_arc_pressure_integrals": (_st, [_vp, _f64p, _f64p]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,

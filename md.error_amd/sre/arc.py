@@ -128,5 +128,15 @@ class Arc:
                                                _ptr(I5, C.c_double)))
         return I1, I5
 
+    def force_grid(self):
+        """Mean error force (nx, ny, nz, 3) at each cell's chosen candidate
+        (needs cal_error(keep_parts=True))."""
+        f = np.empty(self.dims + (3,), np.float32)
+        check(lib().sre_arc_force_grid(self._h, _ptr(f, C.c_float)))
+        return f
+
+    def apply_force_correction(self, engine, replica=0):
+        check(lib().sre_arc_apply_force_correction(self._h, engine._h, int(replica)))
+
     def apply(self, engine, replica=0):
         check(lib().sre_arc_apply(self._h, engine._h, int(replica)))

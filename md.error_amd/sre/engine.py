@@ -75,8 +75,9 @@ def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-def _flags(stats, tail):
-    return (_lib.STATS if stats else 0) | (_lib.TAIL if tail else 0)
+def _flags(stats, tail, fcorr=False):
+    return ((_lib.STATS if stats else 0) | (_lib.TAIL if tail else 0)
+            | (_lib.FCORR if fcorr else 0))
 
 
 class Engine:
@@ -144,6 +145,17 @@ class Engine:
         """Total number of rc lattice values over all replicas."""
         return int(sum(int(np.prod(s)) for s in self._rc_layout if s is not None))
 
+    def set_force_grid(self, f, replica=0):
+        """Grid force correction (nx, ny, nz, 3) added with fcorr=True; None removes it."""
+        if f is None:
+            check(lib().sre_set_force_grid(self._h, int(replica), 0, 0, 0, None))
+            return
+        a = np.ascontiguousarray(np.asarray(f, np.float32))
+        assert a.ndim == 4 and a.shape[3] == 3
+        nx, ny, nz = a.shape[:3]
+        check(lib().sre_set_force_grid(self._h, int(replica), nx, ny, nz,
+                                       _ptr(a, C.c_float)))
+
     def set_density_profile(self, rho, replica=0):
         if rho is None:
             check(lib().sre_set_density_profile(self._h, int(replica), 0, None))
@@ -182,12 +194,12 @@ class Engine:
         check(lib().sre_update_positions(self._h, C.byref(need) if check_skin else None))
         return bool(need.value)
 
-    def compute(self, stats=False, tail=False):
-        check(lib().sre_compute(self._h, _flags(stats, tail)))
+    def compute(self, stats=False, tail=False, fcorr=False):
+        check(lib().sre_compute(self._h, _flags(stats, tail, fcorr)))
 
-    def step(self, stats=False, tail=False):
+    def step(self, stats=False, tail=False, fcorr=False):
         rebuilt = C.c_int(0)
-        check(lib().sre_step(self._h, _flags(stats, tail), C.byref(rebuilt)))
+        check(lib().sre_step(self._h, _flags(stats, tail, fcorr), C.byref(rebuilt)))
         return bool(rebuilt.value)
 
     # -- results -----------------------------------------------------------
@@ -282,9 +294,9 @@ class Engine:
                                  cache_path.encode() if cache_path else None))
         return self.tuning()
 
-    def graph_run(self, n_steps, stats=False, tail=False, perturb=False):
+    def graph_run(self, n_steps, stats=False, tail=False, perturb=False, fcorr=False):
         """Replays [perturb +] update + compute n_steps times as one hipGraph."""
-        f = _flags(stats, tail) | (_lib.PERTURB if perturb else 0)
+        f = _flags(stats, tail, fcorr) | (_lib.PERTURB if perturb else 0)
         check(lib().sre_graph_run(self._h, f, int(n_steps)))
 
     def set_perturbation(self, sigma, seed, first_step=0, anchored=False):
@@ -343,7 +355,7 @@ class Engine:
         check(lib().sre_set_integrator(self._h, float(dt), th, float(ref_T), float(tau_T),
                                        1 if remove_com else 0))
 
-    def md_run(self, n_steps, rebuild_every=10, stats=False, tail=False):
+    def md_run(self, n_steps, rebuild_every=10, stats=False, tail=False, fcorr=False):
         """n_steps MD steps; returns the number of list rebuilds.
         rebuild_every: a fixed interval (> 0), 0 = check the displacement every
         step (the reference's rule), "adaptive" = speculative graph stretches
@@ -352,7 +364,7 @@ class Engine:
             rebuild_every = -1
         nr = C.c_int(0)
         check(lib().sre_md_run(self._h, int(n_steps), int(rebuild_every),
-                               _flags(stats, tail), C.byref(nr)))
+                               _flags(stats, tail, fcorr), C.byref(nr)))
         return nr.value
 
     def md_state(self):

@@ -288,6 +288,71 @@ class AdaptRCut:
         self.box = box
 
 
+class ForceCorr:
+    """Mean error force of the pairs beyond a uniform rc on the density grid
+    (FC/include/ForceCorr.h; reinit/calError FC/src/ForceCorr.cu:73-162), on
+    the device: a one-candidate AdaptRCut grid of its own."""
+
+    def __init__(self):
+        self.arc = None
+        self.f = None
+
+    def reinit(self, rc, dp):
+        self.dp = dp
+        self.arc = Arc(dp.arc.box, 1.0)
+        if self.arc.dims != dp.arc.dims:        # refh is not kept: rebuild from dims
+            raise ValueError("ForceCorr needs the density grid of refh 1")
+        self.arc.set_candidates(rc, rc, 1.0)
+
+    def calError(self, dp=None):
+        dp = dp or self.dp
+        if dp.profile is None:
+            dp.calculate()
+        self.arc.set_density(dp.profile)
+        self.arc.cal_error(keep_parts=True)
+        self.arc.cal_rcut(1.0)                   # one candidate: index 0 everywhere
+        self.f = self.arc.force_grid()
+
+    def getForceCorrX(self):
+        return self.f[..., 0]
+
+    def getForceCorrY(self):
+        return self.f[..., 1]
+
+    def getForceCorrZ(self):
+        return self.f[..., 2]
+
+    def getNx(self):
+        return self.arc.dims[0]
+
+    def getNy(self):
+        return self.arc.dims[1]
+
+    def getNz(self):
+        return self.arc.dims[2]
+
+
+class AssignForceCorr:
+    """Per-atom force correction from a ForceCorr grid (FC/include/AssignForceCorr.h;
+    FC/src/AssignForceCorr.cu:15-145): applied by every force evaluation."""
+
+    def __init__(self):
+        self.grid = None
+
+    def reinit(self, sys, fc, NThread=96):
+        self.sys = sys
+        self.dims = (fc.getNx(), fc.getNy(), fc.getNz())
+
+    def getForceCorr(self, fc):
+        self.grid = np.array(fc.f, np.float32)
+
+    def zero(self):
+        self.grid = np.zeros(self.dims + (3,), np.float32)
+
+    def assign(self, sys):
+        sys._fgrid = self.grid          # uploaded by the next force evaluation
+
+
 class PressureCorrection:
     """Pressure-tensor correction of the pairs beyond each cell's rc
     (H/include/PressureCorrection.h:54-99), on the device over the AdaptRCut grid."""
@@ -402,15 +467,23 @@ class InteractionEngine:
         self.nb = sysNbInter
 
     def clearInteraction(self, sys):
+        import torch
         sys.forcx.zero_()
         sys.forcy.zero_()
         sys.forcz.zero_()
+        # the engine adds on its own stream: the zeros must land first
+        torch.cuda.current_stream().synchronize()
 
     def applyNonBondedInteraction(self, sys, nlist, st=None, excl=None, timer=None):
         e = sys.engine
         e.update_positions(check_skin=False)
-        e.compute(stats=st is not None, tail=self.tail)
+        fgrid = getattr(sys, "_fgrid", None)
+        if fgrid is not None and getattr(sys, "_fgrid_sent", None) is not fgrid:
+            e.set_force_grid(fgrid)
+            sys._fgrid_sent = fgrid
+        e.compute(stats=st is not None, tail=self.tail, fcorr=fgrid is not None)
         e.add_forces_f32_soa(sys.forcx, sys.forcy, sys.forcz)
+        e.synchronize()          # sys.forc* are read on torch's stream afterwards
         if st is not None:
             st._add(e.stats()[0])
 

@@ -15,7 +15,11 @@ positions and boxes as .npz files next to this script:
                   pubilsh.pre/resubmit/fig/t0.85-n16000-adapt-e0.0045/): the
                   run-averaged density along x (a.density.x.out), the rc averaged
                   over y, z (a.rcut.x.out), and the estimated error and rc of the
-                  y = z = 0 column at the last update (error.x.out: x, error, rc)
+                  y = z = 0 column at the last update (error.x.out: x, error, rc);
+                  and of its force-corrected run at uniform rc 7.5
+                  (…/t0.85-n16000-fcorr-rc07.5-feq0200/): that run's averaged
+                  density (a.density.x.out) and force correction of the
+                  y = z = 0 column (afc.x.out: x, fcx, fcy, fcz)
 GROMACS .gro layout: title line, atom count, one fixed-column line per atom
 (positions in columns 21-44, 8 characters each), box line.
 """
@@ -45,15 +49,19 @@ def read_gro(path):
     return xyz, box
 
 
-ARC_DIR = ("/root/reference/shortRange.inhomo/alltext/pubilsh.pre/resubmit/fig/"
-           "t0.85-n16000-adapt-e0.0045")
+FIG = "/root/reference/shortRange.inhomo/alltext/pubilsh.pre/resubmit/fig/"
+ARC_DIR = FIG + "t0.85-n16000-adapt-e0.0045"
+FC_DIR = FIG + "t0.85-n16000-fcorr-rc07.5-feq0200"
 
 
 def arc_fixture():
     tables = {}
-    for key, name in (("a_density", "a.density.x.out"), ("a_rcut", "a.rcut.x.out"),
-                      ("error_x", "error.x.out")):
-        src = os.path.join(ARC_DIR, name)
+    for key, d, name in (("a_density", ARC_DIR, "a.density.x.out"),
+                         ("a_rcut", ARC_DIR, "a.rcut.x.out"),
+                         ("error_x", ARC_DIR, "error.x.out"),
+                         ("fc_density", FC_DIR, "a.density.x.out"),
+                         ("afc", FC_DIR, "afc.x.out")):
+        src = os.path.join(d, name)
         if not os.path.exists(src):
             print(f"missing {src}", file=sys.stderr)
             return

@@ -206,3 +206,100 @@ def test_reference_pressure_correction_class(torch_cuda):
     got = np.array([pc.pxx, pc.pyy, pc.pzz])
     assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
     assert got[0] != got[1]                      # the configuration is not planar-symmetric
+
+
+def test_force_grid_and_correction(torch_cuda):
+    """One candidate (uniform rc): the device's mean error force grid equals
+    the oracle's, and SRE_FCORR adds each atom's cell value (reference FP32
+    cell rule) to the pair forces, nothing else changing."""
+    from sre import configs
+    w = configs.slab(seed=31, n=12000, box=(20.0, 20.0, 60.0), nz=60, name="slab_12k")
+    box = tuple(float(v) for v in w.boxes[0])
+    arc = Arc(box, 1.0)
+    arc.set_candidates(4.0, 4.0, 1.0)
+    pos = torch_cuda.from_numpy(w.xyz.copy()).cuda()
+    for _ in range(2):
+        arc.deposit(pos)
+    arc.cal_error(keep_parts=True)
+    arc.cal_rcut(1.0)
+    grid = arc.force_grid()
+    rho = arc.density()
+    S1, G, kv = A.kernels(box, arc.dims, np.array([4.0]))
+    _, _, e2 = A.cal_error(rho, box, S1, G, kv, parts=True)
+    want = np.moveaxis(e2[0], 0, -1).astype(np.float32)
+    assert np.abs(grid - want).max() < 1e-6 * np.abs(want).max()
+    # a transverse component has no kx = 0 (ky = 0) Fourier part, so its
+    # average along its own axis vanishes; the z profile carries the slab
+    zmax = np.abs(grid[..., 2].mean(axis=(0, 1))).max()
+    assert np.abs(grid[..., 0].mean(axis=0)).max() < 1e-5 * zmax
+    assert np.abs(grid[..., 1].mean(axis=1)).max() < 1e-5 * zmax
+    assert zmax > 0.01
+
+    e = Engine(n_types=1, skin=w.skin)
+    w.setup(e, tail=False)
+    torch_cuda.cuda.synchronize()
+    e.bind_positions(pos)
+    e.rebuild()
+    e.compute(stats=True)
+    f0, s0 = e.forces_fixed(), e.stats()
+    arc.apply_force_correction(e)
+    e.compute(stats=True, fcorr=True)
+    f1, s1 = e.forces_fixed(), e.stats()
+    assert np.array_equal(s0, s1)
+    # oracle cell rule (AssignForceCorr::assign)
+    xyz = w.xyz[:, :3]
+    L = np.asarray(box, np.float32)
+    cells = []
+    for d in range(3):
+        x = xyz[:, d].astype(np.float32)
+        x = np.where(x < 0, (x + L[d]).astype(np.float32),
+                     np.where(x >= L[d], (x - L[d]).astype(np.float32), x))
+        c = ((x * np.float32(arc.dims[d])).astype(np.float32) / L[d]).astype(np.int64)
+        cells.append(np.clip(c, 0, arc.dims[d] - 1))
+    add = grid[cells[0], cells[1], cells[2]].astype(np.float64)
+    fix = np.round(add * 2.0 ** 32).astype(np.int64)
+    assert np.array_equal(f1 - f0, fix)
+    e.compute(stats=True)                    # without the flag: the pair forces again
+    assert np.array_equal(e.forces_fixed(), f0)
+
+
+def test_reference_force_corr_classes(torch_cuda):
+    from sre import reference_api as R
+    g = np.load(os.path.join(GOLDEN, "conf16000.npz"))
+    xyz, box = g["xyz"], g["box"]
+    sys = R.MDSystem()
+    sys.initConfig(xyz, box)
+    nb = R.SystemNonBondedInteraction()
+    nb.add(0, 0, R.LennardJones6_12Parameter(1.0, 1.0, 0.0, 5.0))
+    sys.initTopology()
+    sys.initDeviceData()
+    dp = R.DensityProfile_PiecewiseConst()
+    dp.reinit(*box, 1.0)
+    fc = R.ForceCorr()
+    fc.reinit(5.0, dp)
+    afc = R.AssignForceCorr()
+    afc.reinit(sys, fc)
+    afc.zero()
+    afc.assign(sys)
+    assign = R.AssignRCut()
+    assign.reinit(sys, (1, 1, 1))
+    assign.uniform(5.0)
+    assign.assign(sys)
+    nlist = R.NeighborList(nb, sys, 5.3, 0.3)
+    nlist.rebuild(sys)
+    inter = R.InteractionEngine(sys)
+    inter.registNonBondedInteraction(nb)
+    inter.clearInteraction(sys)
+    inter.applyNonBondedInteraction(sys, nlist)
+    f_zero = sys.forcx.cpu().numpy().copy()
+    dp.deposite(sys)
+    dp.calculate()
+    fc.calError(dp)
+    afc.getForceCorr(fc)
+    afc.assign(sys)
+    inter.clearInteraction(sys)
+    inter.applyNonBondedInteraction(sys, nlist)
+    f_corr = sys.forcx.cpu().numpy()
+    diff = f_corr - f_zero
+    assert np.abs(fc.getForceCorrX()).max() > 0
+    assert 0 < np.abs(diff).max() <= 1.01 * np.abs(fc.getForceCorrX()).max() + 1e-6

@@ -220,3 +220,26 @@ def test_planar_pressure_correction_matches_direct_integral():
     assert abs(p[0] / direct["xx"] - 1) < 2e-3 and p[0] == p[1]
     assert abs(p[2] / direct["zz"] - 1) < 2e-3
     assert abs(p[2] - p[0]) > 1e-3 * abs(p[0])       # anisotropic at the interfaces
+
+
+def test_reference_force_correction_profile():
+    """The reference's force-corrected run (uniform rc 7.5): from its
+    run-averaged density, taken as uniform in y and z, the mean error force
+    along x reproduces its printed correction where it matters (|F| > 1e-3,
+    the interfaces): median within 3 %, every bin within 20 %, peaks within 3 %."""
+    fx = np.load(os.path.join(GOLDEN, "arc_t085.npz"))
+    d, afc = fx["fc_density"], fx["afc"]
+    box = (150.0, 21.0, 21.0)
+    nx = A.grid_dims(box, 1.0)[0]
+    xc = (np.arange(nx) + 0.5) * 150.0 / nx
+    rho = np.interp(xc, d[:, 0], d[:, 1], period=150.0)
+    dims = (nx, 1, 1)
+    S1, G, kv = A.kernels(box, dims, np.array([7.5]))
+    _, _, e2 = A.cal_error(rho.reshape(dims), box, S1, G, kv, parts=True)
+    fx_ = e2[0, 0, :, 0, 0]
+    big = np.abs(afc[:, 1]) > 1e-3
+    r = fx_[big] / afc[big, 1]
+    assert big.sum() > 20
+    assert np.median(np.abs(r - 1)) < 0.03 and np.abs(r - 1).max() < 0.2
+    assert abs(fx_.max() / afc[:, 1].max() - 1) < 0.03
+    assert abs(fx_.min() / afc[:, 1].min() - 1) < 0.03
