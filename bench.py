@@ -370,6 +370,8 @@ def main():
                     help="pick tile/block/cell sizes with sre_autotune before timing")
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the adjoint (parameter gradients) of the mixed form")
+    ap.add_argument("--no-arc", action="store_true",
+                    help="skip timing the adaptive-cutoff producer")
     ap.add_argument("--no-md", action="store_true",
                     help="skip timing the device MD loop (VV + NHC2 + COM removal)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -463,7 +465,7 @@ def main():
         result["autotune_s"] = round(mixed["autotune_s"], 2)
     if args.adjoint:
         result["adjoint"] = time_adjoint(mixed["engine"], w, torch)
-    if len(w.n_atoms) == 1:
+    if not args.no_arc and len(w.n_atoms) == 1:
         try:
             result["arc"] = time_arc(w, torch, cpu=not args.no_cpu)
         except Exception as ex:          # reported in the line, never silent
