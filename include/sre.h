@@ -51,7 +51,7 @@ typedef enum {
   SRE_ERR_INVALID = 4,         /* bad argument or call order */
   SRE_ERR_HIP = 5,             /* a HIP runtime call failed (cf. checkCUDAError, H/include/error.h:8-16) */
   SRE_ERR_BOX_TOO_SMALL = 6,   /* rc_max + skin >= L/2 in some direction: minimum image is ambiguous */
-  SRE_ERR_FORCE_RANGE = 7,     /* a pair force component exceeded 2^19: fixed-point range */
+  SRE_ERR_FORCE_RANGE = 7,     /* pair forces beyond the exact fixed-point range (overlap) */
   SRE_ERR_SKIN = 8,            /* an atom moved more than skin/2 since the last rebuild (fixed-period mode) */
   SRE_ERR_NOT_READY = 9        /* compute requested before the first rebuild */
 } sre_status;

@@ -1091,7 +1091,7 @@ static sre_status check_errors(sre_engine *e) {
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (f & 1u)
     return fail(SRE_ERR_FORCE_RANGE,
-                "a pair force component exceeded 2^19 (overlapping atoms?)");
+                "pair forces beyond the exact fixed-point range (overlapping atoms?)");
   return SRE_OK;
 }
 
@@ -1370,11 +1370,11 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
                       t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
+  if (memcmp(&e->tune, t, sizeof *t) != 0) e->graph_epoch++;  // captured launches
   e->tune = *t;
   if (layout) {
     e->built = false;
     e->cap = 0;
-    e->graph_epoch++;
   }
   return SRE_OK;
 }
@@ -1917,7 +1917,7 @@ static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
   if (disp2) *disp2 = d2;
   if (*ok && (bits[1] & 1u))
     return fail(SRE_ERR_FORCE_RANGE,
-                "a pair force component exceeded 2^19 (overlapping atoms?)");
+                "pair forces beyond the exact fixed-point range (overlapping atoms?)");
   return SRE_OK;
 }
 

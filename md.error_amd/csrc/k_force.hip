@@ -263,7 +263,11 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       wave_max_int((nn + T * U - 1) / (T * U)));
   PairParam P = a.params[0];
 
-  uint64_t ax = 0, ay = 0, az = 0;
+  // fixed-point sums: each pair component (already in units of 2^-32) is
+  // rounded to the nearest integer (ties to even) in FP32 and the integers
+  // are summed in FP64, exact while the lane's sum of magnitudes stays below
+  // 2^53 (checked at the end)
+  double ax = 0.0, ay = 0.0, az = 0.0;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
   const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
   const __amdgpu_buffer_rsrc_t prs = pos_rsrc(a.pos4, a.n);
@@ -286,9 +290,9 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       fs = in ? fs : 0.f;
       fsmax = fmaxf(fsmax, fabsf(fs));
       const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
-      ax += (uint64_t)__double_as_longlong((double)fpx + kMagic);
-      ay += (uint64_t)__double_as_longlong((double)fpy + kMagic);
-      az += (uint64_t)__double_as_longlong((double)fpz + kMagic);
+      ax += (double)__builtin_rintf(fpx);
+      ay += (double)__builtin_rintf(fpy);
+      az += (double)__builtin_rintf(fpz);
       if (STATS) {
         const float ep = __builtin_fmaf(ri6, __builtin_fmaf(P.A, ri6, -P.B), P.shift);
         e += in ? ep : 0.f;
@@ -325,13 +329,14 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
     }
   }
   const int iters = groups * U;
-  const uint64_t mb = (uint64_t)iters * magic_bits();
-  int64_t Fx = (int64_t)(ax - mb), Fy = (int64_t)(ay - mb), Fz = (int64_t)(az - mb);
+  int64_t Fx = (int64_t)ax, Fy = (int64_t)ay, Fz = (int64_t)az;
   Fx = group_sum(Fx, T);
   Fy = group_sum(Fy, T);
   Fz = group_sum(Fz, T);
-  // |pair component| * 2^32 must stay below 2^51 (exact rounding by the magic add)
-  if (__ballot(act && fsmax * rc >= 2251799813685248.0f) && lane == 0)
+  // every lane's sum of |pair component| * 2^32 (at most iters terms of at
+  // most fsmax * rc each) must stay below 2^53 for the FP64 sums to be exact
+  if (__ballot(act && fsmax * rc * (float)max(iters, 4) >= 9007199254740992.0f) &&
+      lane == 0)
     atomicOr(a.err_flag, 1u);
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {

@@ -293,3 +293,23 @@ def test_adjoint_matches_oracle_gradient(torch_cuda, binary, prec):
     np.testing.assert_allclose(ge, pe + te, rtol=tol, atol=tol * np.abs(pe).max())
     np.testing.assert_allclose(gs, ps + ts, rtol=tol, atol=tol * np.abs(ps).max())
     np.testing.assert_allclose(grc, trc, rtol=1e-6, atol=1e-6 * np.abs(trc).max())
+
+
+def test_overlapping_atoms_raise_force_range(torch_cuda):
+    """Two atoms 0.3 sigma apart: their pair force leaves the range where the
+    fixed-point sums are exact; the compute finishes and the read raises."""
+    from sre.engine import ForceRange
+    from sre import gen_fcc
+    xyz = gen_fcc(5, 1.6796, 0.02, 3)
+    xyz[1, :3] = xyz[0, :3] + np.array([0.3, 0.0, 0.0], np.float32)
+    L = np.float32(5 * 1.6796)
+    e = Engine(n_types=1, skin=0.3)
+    e.set_system([len(xyz)], np.array([[L, L, L]], np.float32))
+    e.set_types(np.zeros(len(xyz), np.int32))
+    e.set_pair_params(np.ones((1, 1)), np.ones((1, 1)))
+    e.set_rc_table(np.full((1, 1, 1), 2.5, np.float32))
+    e.upload_positions(xyz)
+    e.rebuild()
+    e.compute(stats=True)
+    with pytest.raises(ForceRange):
+        e.stats()
