@@ -1,0 +1,116 @@
+"""On-disk formats of the reference's MD driver (H = shortRange.inhomo/template/
+tools/simulation.hetero.cutoff), written and read byte for byte alike:
+
+  .gro        GromacsFileManager::writeGroFile (H/src/GromacsFileManager.cpp:95-121)
+              via MDSystem::writeHostDataGro (H/src/MDSystem_interface.cu:174-214):
+              a title line
+              "# at time = %f, step = %d", the atom count, one
+              "%5d%5s%5s%5d%8.3f%8.3f%8.3f%8.4f%8.4f%8.4f" line per atom
+              (positions then velocities), the box "%f %f %f".
+  rcut.rtj    AssignRCut::init_write / write (H/src/AssignRCut.cu:145-177):
+              box (3 float64), grid (3 int32), then per frame the time
+              (float32) and the rc lattice (float32 x nx ny nz).
+  density.dtj DensityProfile_PiecewiseConst::init_write / write
+              (H/src/DensityProfile.cu:273-306): the same header, frames of
+              time (float32) + density (float64 x nx ny nz).
+  density.save DensityProfile_PiecewiseConst::save / load (:225-258): text
+              "%f %f %f  %d %d %d\\n" then "%e " per cell.
+Native byte order, as the reference's fwrite.  (xtc, compressed by
+libxdrfile, is not provided.)
+"""
+import numpy as np
+
+
+def read_gro(path, velocities=False):
+    """GROMACS .gro: positions (columns 21-44), optional velocities (45-68),
+    the box (last line)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].split()[0])
+    xyz = np.empty((n, 3), np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    for i in range(n):
+        ln = lines[2 + i]
+        xyz[i] = [float(ln[20:28]), float(ln[28:36]), float(ln[36:44])]
+        if velocities and len(ln) >= 68:
+            vel[i] = [float(ln[44:52]), float(ln[52:60]), float(ln[60:68])]
+    box = np.array([float(v) for v in lines[2 + n].split()[:3]], np.float32)
+    return (xyz, box, vel) if velocities else (xyz, box)
+
+
+def write_gro(path, xyz, box, vel=None, step=0, time=0.0, resname="LJ", atomname="A"):
+    """MDSystem::writeHostDataGro; residue and atom indices count from 1 and
+    wrap at 100000 as the fixed columns do."""
+    xyz = np.asarray(xyz, np.float64)
+    n = len(xyz)
+    vel = np.zeros((n, 3)) if vel is None else np.asarray(vel, np.float64)
+    out = ["# at time = %f, step = %d" % (time, step), "\n%d\n" % n]
+    for i in range(n):
+        k = (i + 1) % 100000
+        out.append("%5d%5s%5s%5d%8.3f%8.3f%8.3f%8.4f%8.4f%8.4f\n" % (
+            k, resname, atomname, k, xyz[i, 0], xyz[i, 1], xyz[i, 2],
+            vel[i, 0], vel[i, 1], vel[i, 2]))
+    out.append("%f %f %f\n" % tuple(float(v) for v in box[:3]))
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+class GridTrajectory:
+    """Writer of rcut.rtj (frame dtype float32) or density.dtj (float64)."""
+
+    def __init__(self, path, box, dims, dtype):
+        self.dims = tuple(int(v) for v in dims)
+        self.dtype = np.dtype(dtype)
+        self.f = open(path, "wb")
+        self.f.write(np.asarray(box, np.float64).reshape(3).tobytes())
+        self.f.write(np.asarray(self.dims, np.int32).tobytes())
+
+    def write(self, time, grid):
+        g = np.ascontiguousarray(np.asarray(grid, self.dtype).reshape(self.dims))
+        self.f.write(np.float32(time).tobytes())
+        self.f.write(g.tobytes())
+
+    def close(self):
+        self.f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def read_grid_trajectory(path, dtype):
+    """(box, dims, times, frames) of an .rtj (float32) / .dtj (float64) file."""
+    dt = np.dtype(dtype)
+    raw = open(path, "rb").read()
+    box = np.frombuffer(raw, np.float64, 3, 0)
+    dims = tuple(int(v) for v in np.frombuffer(raw, np.int32, 3, 24))
+    nele = int(np.prod(dims))
+    frame = 4 + nele * dt.itemsize
+    body = raw[36:]
+    if len(body) % frame:
+        raise ValueError(f"{path}: truncated frame")
+    times, frames = [], []
+    for o in range(0, len(body), frame):
+        times.append(float(np.frombuffer(body, np.float32, 1, o)[0]))
+        frames.append(np.frombuffer(body, dt, nele, o + 4).reshape(dims))
+    return box.copy(), dims, np.array(times, np.float32), np.array(frames)
+
+
+def save_density(path, box, rho):
+    """DensityProfile_PiecewiseConst::save."""
+    rho = np.asarray(rho, np.float64)
+    with open(path, "w") as f:
+        f.write("%f %f %f  " % tuple(float(v) for v in box[:3]))
+        f.write("%d %d %d\n" % rho.shape)
+        f.write("".join("%e " % v for v in rho.ravel()))
+
+
+def load_density(path):
+    """DensityProfile_PiecewiseConst::load -> (box, rho[nx, ny, nz])."""
+    tok = open(path).read().split()
+    box = np.array([float(v) for v in tok[:3]])
+    dims = tuple(int(v) for v in tok[3:6])
+    rho = np.array([float(v) for v in tok[6:6 + int(np.prod(dims))]]).reshape(dims)
+    return box, rho

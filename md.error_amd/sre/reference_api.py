@@ -26,21 +26,14 @@ exact), and rc assignment happens at every list rebuild.
 """
 import numpy as np
 
+from . import io as _io
 from .arc import Arc
-from .engine import Engine
+from .engine import Engine, NotReady
 
 
 def read_gro(path):
     """GROMACS .gro: positions (columns 21-44) and the box (last line)."""
-    with open(path) as f:
-        lines = f.read().splitlines()
-    n = int(lines[1].split()[0])
-    xyz = np.empty((n, 3), np.float32)
-    for i in range(n):
-        ln = lines[2 + i]
-        xyz[i] = [float(ln[20:28]), float(ln[28:36]), float(ln[36:44])]
-    box = np.array([float(v) for v in lines[2 + n].split()[:3]], np.float32)
-    return xyz, box
+    return _io.read_gro(path)
 
 
 class LennardJones6_12Parameter:
@@ -102,7 +95,7 @@ class MDSystem:
 
     def initConfig(self, filename_or_xyz, box=None):
         if isinstance(filename_or_xyz, str):
-            self.xyz, self.box = read_gro(filename_or_xyz)
+            self.xyz, self.box, self.velo = _io.read_gro(filename_or_xyz, velocities=True)
         else:
             self.xyz = np.ascontiguousarray(filename_or_xyz, np.float32)[:, :3]
             self.box = np.asarray(box, np.float32)
@@ -127,6 +120,18 @@ class MDSystem:
 
     def normalizeDeviceData(self):
         pass
+
+    def writeHostDataGro(self, filename, step=0, time=0.0, timer=None):
+        """Positions from the device (and velocities if set) as .gro
+        (H/src/MDSystem_interface.cu:174-214)."""
+        xyz = self._pos[:, :3].cpu().numpy()
+        vel = getattr(self, "velo", None)
+        if self.engine is not None:
+            try:                       # the engine's MD velocities, if it has any
+                vel = self.engine.velocities()
+            except NotReady:
+                pass
+        _io.write_gro(filename, xyz, self.box, vel, step, time)
 
     def _engine(self, n_types, skin):
         if self.engine is None:
@@ -184,6 +189,25 @@ class DensityProfile_PiecewiseConst:
         with open(path, "w") as f:
             for i in range(self.nx):
                 f.write("%f %f\n" % ((i + 0.5) * hx, self.profile[i, 0, 0]))
+
+    def save(self, path):
+        _io.save_density(path, self.arc.box, self.profile)
+
+    def load(self, path):
+        box, rho = _io.load_density(path)
+        if tuple(rho.shape) != tuple(self.arc.dims):
+            raise ValueError(f"{path}: grid {rho.shape} differs from {self.arc.dims}")
+        self.arc.set_density(rho)
+        self.profile = rho
+
+    def init_write(self, path):
+        self._traj = _io.GridTrajectory(path, self.arc.box, self.arc.dims, np.float64)
+
+    def write(self, time):
+        self._traj.write(time, self.profile)
+
+    def end_write(self):
+        self._traj.close()
 
 
 class AdaptRCut:
@@ -391,6 +415,15 @@ class AssignRCut:
 
     def uniform(self, rc):
         self.getRCut(np.full(self.dims, rc, np.float32))
+
+    def init_write(self, path):
+        self._traj = _io.GridTrajectory(path, self.sys.box, self.dims, np.float32)
+
+    def write(self, time):
+        self._traj.write(time, self.table)
+
+    def end_write(self):
+        self._traj.close()
 
     def assign(self, sys):
         sys._rc = self.table

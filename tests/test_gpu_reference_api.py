@@ -163,3 +163,20 @@ def test_reference_adaptive_cutoff_sequence(torch_cuda, tmp_path):
     arc2.load_rc(path)
     assert np.array_equal(arc2.getProfileIndex(), arc.getProfileIndex())
     assert np.array_equal(arc2.getRCut(), table)
+    # trajectories and configuration as the driver writes them
+    from sre import io
+    assign.init_write(str(tmp_path / "rcut.rtj"))
+    dp.init_write(str(tmp_path / "density.dtj"))
+    for t in (1.0, 2.0):
+        assign.write(t)
+        dp.write(t)
+    assign.end_write()
+    dp.end_write()
+    _, d1, times, fr = io.read_grid_trajectory(str(tmp_path / "rcut.rtj"), np.float32)
+    assert d1 == dims and np.array_equal(times, [1.0, 2.0])
+    assert np.array_equal(fr[1], table.astype(np.float32))
+    _, _, _, fr = io.read_grid_trajectory(str(tmp_path / "density.dtj"), np.float64)
+    assert np.array_equal(fr[0], rho)
+    sys.writeHostDataGro(str(tmp_path / "confout.gro"), 10, 0.05)
+    x2, b2 = io.read_gro(str(tmp_path / "confout.gro"))
+    assert np.abs(x2 - xyz).max() < 5.1e-4 and np.array_equal(b2, box)
