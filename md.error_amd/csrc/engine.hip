@@ -1944,9 +1944,16 @@ static sre_status md_snapshot(sre_engine *e, bool restore) {
       {e->d_fcall, e->d_snap_fcall, sizeof(float) * 3 * N},
       {e->d_md_part, e->d_snap_part, sizeof(double) * 4 * md_part_count(N)},
       {e->d_md, e->d_snap_md, sizeof(MDState)}};
-  for (const Pair &p : pairs)
-    HIP_TRY(hipMemcpyAsync(restore ? p.live : p.snap, restore ? p.snap : p.live,
-                           p.bytes, hipMemcpyDeviceToDevice, e->stream));
+  static_assert(sizeof(MDState) % 4 == 0, "MDState copies as 4-byte words");
+  CopySegs c{};
+  for (const Pair &p : pairs) {
+    c.src[c.count] = static_cast<const uint32_t *>(restore ? p.snap : p.live);
+    c.dst[c.count] = static_cast<uint32_t *>(restore ? p.live : p.snap);
+    c.words[c.count] = p.bytes / 4;
+    ++c.count;
+  }
+  launch_md_copy(c, e->stream);     // one launch for all five buffers
+  HIP_TRY(hipGetLastError());
   return SRE_OK;
 }
 

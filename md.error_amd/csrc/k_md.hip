@@ -183,6 +183,27 @@ __global__ __launch_bounds__(kMDBlock) void k_md_scale(int n, const MDState *st,
   vel[i] = (float)(st->a1 * (double)vel[i]);
 }
 
+// Copies up to kCopySegs buffers (whole 4-byte words) in one launch: the
+// snapshot / restore of an adaptive MD stretch.
+__global__ __launch_bounds__(kMDBlock) void k_md_copy(CopySegs c) {
+  size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = 0; k < c.count; ++k) {
+    if (w < c.words[k]) {
+      c.dst[k][w] = c.src[k][w];
+      return;
+    }
+    w -= c.words[k];
+  }
+}
+
+void launch_md_copy(const CopySegs &c, hipStream_t st) {
+  size_t total = 0;
+  for (int k = 0; k < c.count; ++k) total += c.words[k];
+  if (total == 0) return;
+  hipLaunchKernelGGL(k_md_copy, dim3((unsigned)((total + kMDBlock - 1) / kMDBlock)),
+                     dim3(kMDBlock), 0, st, c);
+}
+
 int md_part_count(int n) { return (n + kMDBlock - 1) / kMDBlock; }
 
 void launch_md_kick(int n, const uint32_t *inv_perm, const int *types,

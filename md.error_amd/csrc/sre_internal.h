@@ -251,6 +251,14 @@ void launch_add_force_grid(int n, const float *pos, int stride, const uint4 *bac
                            const uint32_t *inv_perm, const RepInfo *reps,
                            const float4 *grid, int64_t *fx, int64_t *fy, int64_t *fz,
                            double *fxd, double *fyd, double *fzd, hipStream_t st);
+constexpr int kCopySegs = 6;
+struct CopySegs {
+  const uint32_t *src[kCopySegs];
+  uint32_t *dst[kCopySegs];
+  size_t words[kCopySegs];
+  int count;
+};
+void launch_md_copy(const CopySegs &c, hipStream_t st);
 int md_part_count(int n);
 void launch_md_kick(int n, const uint32_t *inv_perm, const int *types,
                     const int64_t *fx, const int64_t *fy, const int64_t *fz,
