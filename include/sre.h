@@ -345,7 +345,10 @@ sre_status sre_arc_set_candidates(sre_arc *a, double rcmin, double rcmax,
 sre_status sre_arc_rc_list(sre_arc *a, double *rcs /* [n_rc] */);
 /* Density samples: clearData, deposite (device positions, n atoms, stride >= 3
  * floats), the same from an engine replica's bound positions, a host density
- * in place of samples, and calculate() read back. */
+ * in place of samples, and calculate() read back.  sre_arc_deposit reads
+ * d_pos on the arc's stream (the caller orders it after the writes);
+ * sre_arc_deposit_engine orders itself after the engine's queued work and
+ * the engine's later work after it. */
 sre_status sre_arc_clear(sre_arc *a);
 sre_status sre_arc_deposit(sre_arc *a, const float *d_pos, int n, int stride);
 sre_status sre_arc_deposit_engine(sre_arc *a, sre_engine *e, int replica);

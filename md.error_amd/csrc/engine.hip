@@ -2127,6 +2127,8 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
   return SRE_OK;
 }
 
+hipStream_t sre::engine_stream(sre_engine *e) { return e->stream; }
+
 sre_status sre::engine_replica_positions(sre_engine *e, int replica,
                                          const float **pos, int *n, int *stride,
                                          float box[3]) {

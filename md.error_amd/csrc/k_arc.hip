@@ -436,6 +436,9 @@ struct sre_arc {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipEvent_t engine_done = nullptr;         // engine work before a deposit from its
+  hipEvent_t deposit_done = nullptr;        // positions, and that deposit before later
+                                            // engine work
   ArcGrid g{};
   std::vector<double> rcs;
   double *d_rcs = nullptr;
@@ -465,6 +468,8 @@ sre_arc::~sre_arc() {
   dfree(d_counts); dfree(d_rho); dfree(d_rhok); dfree(d_zr); dfree(d_za);
   dfree(d_zb); dfree(d_err); dfree(d_parts); dfree(d_idx); dfree(d_idx2);
   dfree(d_rc); dfree(d_res_err); dfree(d_fgrid);
+  if (engine_done) (void)hipEventDestroy(engine_done);
+  if (deposit_done) (void)hipEventDestroy(deposit_done);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -523,6 +528,8 @@ extern "C" sre_status sre_arc_create(sre_arc **out, const double box[3],
       HIP_TRY(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
       a->own_stream = true;
     }
+    HIP_TRY(hipEventCreateWithFlags(&a->engine_done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&a->deposit_done, hipEventDisableTiming));
     HIP_TRY(dalloc(&a->d_counts, nele));
     HIP_TRY(dalloc(&a->d_rho, nele));
     HIP_TRY(dalloc(&a->d_rhok, nele));
@@ -644,7 +651,17 @@ extern "C" sre_status sre_arc_deposit_engine(sre_arc *a, sre_engine *e, int repl
     if ((double)box[d] != a->g.L[d])
       return fail(SRE_ERR_INVALID, "replica %d box differs from the density grid's",
                   replica);
-  return sre_arc_deposit(a, pos, n, stride);
+  hipStream_t es = engine_stream(e);
+  if (es != a->stream) {
+    HIP_TRY(hipSetDevice(a->device));
+    HIP_TRY(hipEventRecord(a->engine_done, es));
+    HIP_TRY(hipStreamWaitEvent(a->stream, a->engine_done, 0));
+  }
+  s = sre_arc_deposit(a, pos, n, stride);
+  if (s != SRE_OK || es == a->stream) return s;
+  HIP_TRY(hipEventRecord(a->deposit_done, a->stream));
+  HIP_TRY(hipStreamWaitEvent(es, a->deposit_done, 0));
+  return SRE_OK;
 }
 
 // DensityProfile_PiecewiseConst::calculate

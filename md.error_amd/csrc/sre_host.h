@@ -16,6 +16,10 @@ sre_status fail(sre_status s, const char *fmt, ...);
 sre_status engine_replica_positions(sre_engine *e, int replica, const float **pos,
                                     int *n, int *stride, float box[3]);
 
+// The stream the engine's kernels run on (work another stream does on the
+// engine's buffers waits for it through an event).
+hipStream_t engine_stream(sre_engine *e);
+
 template <typename T>
 hipError_t dalloc(T **p, size_t count) {
   *p = nullptr;

@@ -138,6 +138,19 @@ SIGNATURES = {
 _LIB = None
 
 
+def _init_torch_first():
+    """libsre.so runs on the system HIP runtime; PyTorch for ROCm carries a
+    runtime of its own, and in a process that holds both, PyTorch finds no
+    device if the system runtime initialised first.  So where PyTorch is
+    importable its runtime starts before the library is loaded."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     """Loads libsre.so; raises if it has not been built."""
     global _LIB
@@ -145,6 +158,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} not found: build it with `python md.error_amd/build.py`")
+        _init_torch_first()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

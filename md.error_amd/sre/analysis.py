@@ -1,0 +1,63 @@
+"""Observables of a liquid-vapour slab, as the reference's analysis tools
+compute them (R = shortRange.inhomo/template/tools):
+
+  BlockAverage   R/cal.density.profile/src/BlockAverage.cpp:4-54 — the mean
+                 over the first nblock * (n // nblock) samples and the error
+                 of that mean from the spread of the nblock block means,
+                 sqrt(sum (b_k - mean)^2 / (nblock (nblock - 1))).
+  slab_densities R/cal.density.profile/app/density.cpp:84-112 — per frame, the
+                 atoms whose x (wrapped once into [0, Lx)) lies in
+                 [l1, l2] (liquid) or in [0, g1] or [g2, Lx) (gas), divided by
+                 the region volumes.
+  tension        column 11 of the MD driver's thermo line,
+                 (P_xx - (P_yy + P_zz) / 2) * Lx / 2
+                 (R/simulation.uniform.cutoff/app/lj.nhc.cu:158-171), averaged
+                 from a start time on (R/sh.scripts/tension.uniform.sh:5-22,
+                 which quotes twice the error of the mean).
+"""
+import numpy as np
+
+
+class BlockAverage:
+    def __init__(self, data=None, nblock=16):
+        if data is not None:
+            self.process(data, nblock)
+
+    def process(self, data, nblock=16):
+        v = np.asarray(data, np.float64).ravel()
+        per = v.size // nblock
+        if per < 1:
+            raise ValueError(f"{v.size} samples cannot fill {nblock} blocks")
+        used = v[:per * nblock]
+        self.n_used = used.size
+        self.avg = used.mean()
+        blocks = used.reshape(nblock, per).mean(1)
+        self.error = float(np.sqrt(((blocks - self.avg) ** 2).sum() / (nblock * (nblock - 1))))
+        return self
+
+
+def slab_densities(x, box, l1, l2, g1, g2):
+    """(gas, liquid) number densities of one frame; x holds the atoms'
+    x coordinates (the slab normal is x)."""
+    lx, ly, lz = (float(v) for v in box[:3])
+    t = np.asarray(x, np.float64).copy()
+    t[t >= lx] -= lx
+    t[t < 0] += lx
+    vol_g = ly * lz * (g1 + lx - g2)
+    vol_l = ly * lz * (l2 - l1)
+    gas = np.count_nonzero((t <= g1) | (t >= g2))
+    liq = np.count_nonzero((t >= l1) & (t <= l2) & ~((t <= g1) | (t >= g2)))
+    return gas / vol_g, liq / vol_l
+
+
+def pressure_diag(stats, kinetic_diag, box):
+    """P_aa = 2 / V (K_aa - W_aa / 2) from the engine's (E, W_xx, W_yy, W_zz)
+    and the kinetic tensor diagonal (R/../simulation.hetero.cutoff/include/
+    Statistic_interface.h:92-117)."""
+    vol = float(box[0]) * float(box[1]) * float(box[2])
+    w = np.asarray(stats, np.float64)[1:4]
+    return 2.0 / vol * (np.asarray(kinetic_diag, np.float64) - 0.5 * w)
+
+
+def tension(p, lx):
+    return (p[0] - 0.5 * (p[1] + p[2])) * 0.5 * float(lx)

@@ -20,6 +20,13 @@ positions and boxes as .npz files next to this script:
                   (…/t0.85-n16000-fcorr-rc07.5-feq0200/): that run's averaged
                   density (a.density.x.out) and force correction of the
                   y = z = 0 column (afc.x.out: x, fcx, fcy, fcz)
+  slab_t085.npz   the last configuration, with velocities, of the reference's
+                  16000-atom slab run at T* = 0.85 and uniform rc 7.5
+                  (…/fig/t0.85-n16000-rc07.5uni/confout.gro, time 15000), and the
+                  run-averaged observables of that system
+                  (…/fig/converge/t0.85.{liquid,gas,tension}.{uni,adapt,fcorr}.out:
+                  rc, value, error, large-rc reference value, its error; the
+                  tension tables carry the tail correction in column 4)
 GROMACS .gro layout: title line, atom count, one fixed-column line per atom
 (positions in columns 21-44, 8 characters each), box line.
 """
@@ -70,8 +77,40 @@ def arc_fixture():
     print("arc_t085.npz", {k: v.shape for k, v in tables.items()})
 
 
+def read_gro_v(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].split()[0])
+    xyz = np.empty((n, 3), np.float32)
+    vel = np.empty((n, 3), np.float32)
+    for i in range(n):
+        ln = lines[2 + i]
+        xyz[i] = [float(ln[20:28]), float(ln[28:36]), float(ln[36:44])]
+        vel[i] = [float(ln[44:52]), float(ln[52:60]), float(ln[60:68])]
+    box = np.array([float(v) for v in lines[2 + n].split()[:3]], np.float32)
+    return xyz, vel, box
+
+
+def slab_fixture():
+    src = FIG + "t0.85-n16000-rc07.5uni/confout.gro"
+    if not os.path.exists(src):
+        print(f"missing {src}", file=sys.stderr)
+        return
+    xyz, vel, box = read_gro_v(src)
+    tables = {}
+    for obs in ("liquid", "gas", "tension"):
+        for kind in ("uni", "adapt", "fcorr"):
+            f = FIG + f"converge/t0.85.{obs}.{kind}.out"
+            # commented lines are runs left out of the published figure
+            tables[f"{obs}_{kind}"] = np.loadtxt(f, comments="#", ndmin=2)
+    np.savez_compressed(os.path.join(HERE, "slab_t085.npz"), xyz=xyz, vel=vel, box=box,
+                        **tables)
+    print("slab_t085.npz", xyz.shape, box, {k: v.shape for k, v in tables.items()})
+
+
 def main():
     arc_fixture()
+    slab_fixture()
     for out, rel in FILES.items():
         src = os.path.join(REF, rel)
         if not os.path.exists(src):

@@ -1,0 +1,139 @@
+"""Slab MD on the GPU against the reference's published observables.
+
+Not collected by pytest (it runs for minutes); run by hand on a GPU:
+
+    python tests/slab_observables.py --rc 5.0 [--fcorr] --equil 100000 \
+        --steps 1000000 --out gpurun_out/slab_obs_rc5.0.json
+
+Starts from the reference's equilibrated 16000-atom slab (T* = 0.85, box
+150 x 21 x 21, slab normal x; tests/golden/slab_t085.npz), runs the uniform-
+cutoff driver (sre.apps.lj_nhc: velocity Verlet + NHC, dt 0.005, skin 0.49;
+with --fcorr the long-range force correction updated every 20000 steps) at
+the chosen cutoff, drops `--equil` steps, and measures as the reference's
+tools do: liquid density in 70 <= x <= 80, gas density in x <= 20 or
+x >= 130, one frame per 1000 steps (R/cal.density.profile/app/density.cpp,
+limits of R/sh.scripts/rho.sh; R = shortRange.inhomo/template/tools); the
+surface tension as the mean of the thermo line's column 11; errors as twice
+the 16-block error of the mean (R/sh.scripts/tension.uniform.sh).  The
+tension's long-range pressure correction is computed as
+R/analyze.simulation.suit/app/analyze.uniform.cpp does, from the density
+profile of the same frames on the refh = 1 grid with the uniform cutoff
+(sre_arc_pressure).
+The reference's values for the same system and cutoff come from
+shortRange.inhomo/alltext/pubilsh.pre/resubmit/fig/converge/
+t0.85.{liquid,gas,tension}.{uni,fcorr}.out (3e6-step runs, the first 1e6
+steps dropped); their tension tables hold the corrected tension, its error,
+the correction, so the simulated tension is column 2 minus column 4.  A
+quantity agrees when |ours - ref| <= 1.5 sqrt(err_ours^2 + err_ref^2) (both
+errors are 2 sigma); the correction, which the tables give without an
+error, when it is within 10 %.  Exits non-zero when one does not.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "md.error_amd"))
+
+from sre import analysis, apps, io  # noqa: E402
+from sre.arc import Arc  # noqa: E402
+
+L1, L2, G1, G2 = 70.0, 80.0, 20.0, 130.0
+FRAME_EVERY = 1000
+NBLOCK = 16
+
+
+def reference_row(fx, name, rc):
+    t = fx[name]
+    hit = np.nonzero(np.isclose(t[:, 0], rc))[0]
+    if hit.size == 0:
+        raise SystemExit(f"no reference row for rc {rc} in {name}")
+    return t[hit[0]]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rc", type=float, default=5.0)
+    ap.add_argument("--fcorr", action="store_true")
+    ap.add_argument("--update-fc", type=int, default=20000)
+    ap.add_argument("--equil", type=int, default=60000)
+    ap.add_argument("--steps", type=int, default=300000)
+    ap.add_argument("--out", default="gpurun_out/slab_obs.json")
+    args = ap.parse_args()
+
+    fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
+    box = fx["box"]
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    thermo_path = os.path.splitext(args.out)[0] + ".thermo"
+    gas, liq = [], []
+    prof = Arc(box, 1.0)
+    prof.set_candidates(args.rc, args.rc, 1.0)
+
+    def on_frame(step, t, xyz, engine):
+        if step > args.equil:
+            g, l = analysis.slab_densities(xyz[:, 0], box, L1, L2, G1, G2)
+            gas.append(g)
+            liq.append(l)
+            prof.deposit(engine)
+        if step % 20000 == 0:
+            print(f"step {step}: {time.time() - t0:.1f} s", flush=True)
+
+    with tempfile.TemporaryDirectory() as tmp:
+        conf = os.path.join(tmp, "conf.gro")
+        io.write_gro(conf, fx["xyz"], box, fx["vel"], resname="lj", atomname="lj")
+        t0 = time.time()
+        kind = "fcorr" if args.fcorr else "uni"
+        print(f"slab T*=0.85 {kind} rc {args.rc}: {args.equil} + {args.steps} steps",
+              flush=True)
+        with open(thermo_path, "w") as out:
+            rows = apps.lj_nhc(conf, args.equil + args.steps, rcut=args.rc, ref_T=0.85,
+                               out=out, on_frame=on_frame, conf_feq=FRAME_EVERY,
+                               confstart=None, confout=None, fcorr=args.fcorr,
+                               update_fc_feq=args.update_fc)
+        wall = time.time() - t0
+    keep = rows[:, 0] > args.equil
+    res = {"kind": kind, "rc": args.rc, "equil": args.equil, "steps": args.steps,
+           "wall_s": wall, "ms_per_step": 1e3 * wall / (args.equil + args.steps),
+           "temperature": float(rows[keep, 4].mean())}
+    ok = True
+    for name, data in (("liquid", liq), ("gas", gas), ("tension", rows[keep, 10])):
+        b = analysis.BlockAverage(data, NBLOCK)
+        ref = reference_row(fx, f"{name}_{kind}", args.rc)
+        val = ref[1] - ref[3] if name == "tension" else ref[1]
+        err = 2.0 * b.error
+        bound = 1.5 * np.hypot(err, ref[2])
+        agree = bool(abs(b.avg - val) <= bound)
+        ok &= agree
+        res[name] = {"ours": float(b.avg), "ours_err2s": err, "ref": float(val),
+                     "ref_err2s": float(ref[2]), "bound": float(bound), "agree": agree}
+        print(f"{name:8s} ours {b.avg:.4f} ({err:.4f})  ref {val:.4f} ({ref[2]:.4f})"
+              f"  {'ok' if agree else 'DIFFERS'}", flush=True)
+    # long-range pressure correction of the tension from the sampled profile
+    prof.cal_error()
+    prof.cal_rcut(1.0)
+    pc = prof.pressure()
+    corr = analysis.tension(pc, box[0])
+    ref = reference_row(fx, f"tension_{kind}", args.rc)
+    agree = bool(abs(corr - ref[3]) <= 0.1 * abs(ref[3]))
+    ok &= agree
+    res["tension_correction"] = {"ours": float(corr), "ref": float(ref[3]),
+                                 "pressure_correction": [float(v) for v in pc],
+                                 "agree": agree}
+    res["tension_corrected"] = {"ours": float(res["tension"]["ours"] + corr),
+                                "ref": float(ref[1]), "ref_err2s": float(ref[2])}
+    print(f"tension correction ours {corr:.4f}  ref {ref[3]:.4f}"
+          f"  {'ok' if agree else 'DIFFERS'}", flush=True)
+    prof.close()
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res), flush=True)
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

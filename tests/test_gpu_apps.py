@@ -1,0 +1,65 @@
+"""The uniform-cutoff MD driver (sre.apps.lj_nhc) on the reference's
+equilibrated 16000-atom slab (T* = 0.85, tests/golden/slab_t085.npz)."""
+import io as _io
+import os
+
+import numpy as np
+import pytest
+
+from sre import apps, io
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_lj_nhc_driver_short_run(torch_cuda, tmp_path):
+    fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
+    conf = tmp_path / "conf.gro"
+    io.write_gro(conf, fx["xyz"], fx["box"], fx["vel"], resname="lj", atomname="lj")
+    frames = []
+    out = _io.StringIO()
+    rows = apps.lj_nhc(str(conf), 600, rcut=5.0, ref_T=0.85, out=out, conf_feq=200,
+                       on_frame=lambda s, t, x, e: frames.append((s, x)),
+                       confstart=str(tmp_path / "confstart.gro"),
+                       confout=str(tmp_path / "confout.gro"))
+    text = out.getvalue().splitlines()
+    assert text[1].startswith("#*     1") and text[2].startswith("#* nstep")
+    assert rows.shape == (6, 12)
+    assert np.array_equal(rows[:, 0], [100, 200, 300, 400, 500, 600])
+    for ln, r in zip(text[3:], rows):
+        assert int(ln.split()[0]) == r[0]
+    # the equilibrated slab stays at its temperature and density
+    assert np.all(np.abs(rows[:, 4] - 0.85) < 0.03)
+    assert np.all(rows[:, 2] / 16000 < -4.0)
+    assert [s for s, _ in frames] == [0, 200, 400, 600]
+    x0, box = fx["xyz"].astype(np.float64), fx["box"]
+    d = frames[-1][1] - np.mod(x0, box)
+    d -= np.round(d / box) * box
+    assert 0.05 < np.sqrt((d ** 2).sum(1).mean()) < 3.0
+    # confout holds the final frame at the .gro precision
+    xo, bo, vo = io.read_gro(tmp_path / "confout.gro", velocities=True)
+    assert np.array_equal(bo, box)
+    assert np.abs(xo - frames[-1][1]).max() < 6e-4
+    xs, _ = io.read_gro(tmp_path / "confstart.gro")
+    assert np.abs(xs - np.mod(x0, box)).max() < 6e-4
+
+
+def test_lj_nhc_force_corrected_driver(torch_cuda, tmp_path):
+    """The force-corrected driver adds no force before its first correction
+    update: a run whose first update comes at step 200 matches, bit for bit up
+    to step 200, one whose first update comes at its end (same density
+    sampling, so the same stretches), and departs from it afterwards."""
+    fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
+    conf = tmp_path / "conf.gro"
+    io.write_gro(conf, fx["xyz"], fx["box"], fx["vel"], resname="lj", atomname="lj")
+    runs = {}
+    for upd in (200, 400):
+        frames = {}
+        apps.lj_nhc(str(conf), 400, rcut=5.0, ref_T=0.85, out=_io.StringIO(), conf_feq=200,
+                    on_frame=lambda s, t, x, e: frames.__setitem__(s, x), confstart=None,
+                    confout=None, fcorr=True, density_feq=40, update_fc_feq=upd)
+        runs[upd] = frames
+    assert np.array_equal(runs[200][200], runs[400][200])
+    d = np.abs(runs[200][400] - runs[400][400])
+    assert d.max() > 1e-4
