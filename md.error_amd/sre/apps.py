@@ -1,31 +1,41 @@
-"""The reference's uniform-cutoff MD drivers over the engine.
+"""The reference's MD driver programs over the engine (R =
+shortRange.inhomo/template/tools, parameters as its run scripts inject them,
+shortRange.inhomo/template/parameters.sh: dt 0.005, skin 0.49, tau_T 1,
+refh 1, density sampling every 40 steps):
 
-`lj_nhc` follows R/simulation.uniform.cutoff/app/lj.nhc.cu and, with
-`fcorr=True`, R/simulation.fcorr/app/lj.nhc-xtc.cu (R =
-shortRange.inhomo/template/tools), with the parameters their run scripts
-inject (shortRange.inhomo/template/parameters.sh: dt 0.005, skin 0.49,
-tau_T 1, refh 1, density sampling every 40 steps, force-correction update
-every 20000): one LJ species (eps = sigma = 1, no energy shift) with a
-uniform cutoff, velocity Verlet + two-link Nose-Hoover chain +
-centre-of-mass removal, a thermo line every `thermo_feq` steps in the
-drivers' column layout, and confstart.gro / confout.gro.  The force-corrected
-driver samples the density on the refh grid, and every `update_fc_feq` steps
-turns the profile sampled since the last update into the mean error force of
-the cutoff (the long-range force correction), which every later force
-evaluation adds at each atom's grid cell; before the first update the
-correction is zero (R/simulation.fcorr/app/lj.nhc-xtc.cu:147-222).
+  lj_nhc(fcorr=False)  R/simulation.uniform.cutoff/app/lj.nhc.cu — one LJ
+                       species (eps = sigma = 1, no energy shift), a uniform
+                       cutoff, velocity Verlet + two-link Nose-Hoover chain +
+                       centre-of-mass removal.
+  lj_nhc(fcorr=True)   R/simulation.fcorr/app/lj.nhc-xtc.cu — the same plus
+                       the long-range force correction: the density sampled
+                       on the refh grid since the last update becomes, every
+                       `update_fc_feq` steps, the mean error force of the
+                       cutoff, which later force evaluations add at each
+                       atom's grid cell (zero before the first update).
+  lj_nhc_adapt         R/simulation.hetero.cutoff/app/lj.nhc-xtc.cu — the
+                       adaptive cutoff: a uniform start, then every
+                       `update_rc_feq` steps the sampled density gives every
+                       candidate's error, each cell takes the smallest
+                       candidate within `target_prec`, the choice is widened
+                       `n_refine` times, installed, and the pressure
+                       correction of the new cutoffs is printed with the
+                       thermo line.
 
-The stretches between thermo lines, samples and updates run on the device
-(`sre_md_run`, adaptive rebuild stretches); the drivers' trajectory file
-(xtc) is replaced by a callback, on_frame(step, time, xyz, engine), called
-every `conf_feq` steps with the host positions and the engine (whose bound
-positions an Arc can sample directly).  Differences that leave the ensemble unchanged: the centre-of-mass
-velocity is removed every step (the uniform driver: every 10th), the
-correction force is looked up at every force evaluation (the driver assigns
-it every 20 steps), and the kinetic tensor of the thermo line is taken after
-the step's second chain half-step.
+Each prints a thermo line every `thermo_feq` steps in its program's column
+layout and writes confstart.gro / confout.gro.  The stretches between thermo
+lines, samples and updates run on the device (`sre_md_run`, adaptive rebuild
+stretches); the programs' xtc trajectory is replaced by a callback,
+on_frame(step, time, xyz, engine), called every `conf_feq` steps with the
+host positions and the engine (whose bound positions an Arc can sample
+directly).  Differences that leave the ensemble unchanged: the centre-of-mass
+velocity is removed every step (the uniform program: every 10th), the
+correction force and the per-atom cutoff are looked up from the grids at
+every force evaluation / list rebuild (the programs assign them every 20-40
+steps), and the kinetic tensor of the thermo line is taken after the step's
+second chain half-step.
 
-Command line, as the driver's:  python -m sre.apps conf.gro nstep [rcut refT]
+Command line:  python -m sre.apps conf.gro nstep [rcut refT]
 """
 import sys
 
@@ -42,101 +52,181 @@ HEADER = (
     " pressureXX pressureYY pressureZZ s_tension\n")
 
 
-def thermo_line(step, dt, n, box, stats, kin_diag, nhc_energy, n_list):
-    """One line of the driver's output (lj.nhc.cu:158-171) and its numbers."""
+def thermo_line(step, dt, n, box, stats, kin_diag, nhc_energy, n_list, pc=None):
+    """One line of the programs' output and its numbers: lj.nhc.cu:158-171
+    (twelve columns, 7 digits) or, given the pressure correction pc, the
+    adaptive program's fifteen (hetero.cutoff lj.nhc-xtc.cu:198-222: 5 digits,
+    pc_xx, pc_yy and the correction of the tension before the count)."""
     p = analysis.pressure_diag(stats, kin_diag, box)
     k = float(np.sum(kin_diag))
     e = float(stats[0])
     row = (step, step * dt, e, k, k * 2.0 / 3.0 / (n - 3.0), e + k, e + k + nhc_energy,
-           p[0], p[1], p[2], analysis.tension(p, box[0]), float(n_list))
-    text = "%09d %07e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.2e\n" % row
+           p[0], p[1], p[2], analysis.tension(p, box[0]))
+    if pc is None:
+        row = row + (float(n_list),)
+        text = "%09d %07e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.2e\n" % row
+    else:
+        row = row + (float(pc[0]), float(pc[1]), analysis.tension(pc, box[0]),
+                     float(n_list))
+        text = ("%09d %05e %.5e %.5e %.5e %.5e %.5e %.5e %.5e %.5e %.5e %.5e %.5e %.5e"
+                " %.2e\n") % row
     return text, row
+
+
+class _Run:
+    """Engine, bound positions and the loop shared by the drivers."""
+
+    def __init__(self, conf, rcut, ref_T, tau_T, dt, skin, autotune):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("the MD drivers run on the GPU")
+        xyz, box, vel = io.read_gro(conf, velocities=True)
+        self.n = len(xyz)
+        self.box = np.asarray(box, np.float32)
+        self.dt = dt
+        e = Engine(n_types=1, skin=skin, precision="mixed")
+        e.set_system([self.n], [self.box])
+        e.set_types(np.zeros(self.n, np.int32))
+        one = np.ones((1, 1), np.float32)
+        e.set_pair_params(one, one, np.zeros((1, 1), np.float32))
+        e.set_rc_table(np.full((1, 1, 1), rcut, np.float32))
+        pos4 = np.zeros((self.n, 4), np.float32)
+        pos4[:, :3] = np.mod(xyz, self.box)            # normalizeDeviceData
+        self.pos = torch.from_numpy(pos4).cuda()
+        torch.cuda.synchronize()
+        e.bind_positions(self.pos)
+        e.set_masses([1.0])
+        e.set_velocities(vel)
+        e.set_integrator(dt, "nhc2", ref_T, tau_T, remove_com=True)
+        e.rebuild()
+        if autotune:
+            e.autotune(stats=True)
+        self.e = e
+
+    def positions(self):
+        return self.pos.cpu().numpy()[:, :3].astype(np.float64)
+
+    def write_conf(self, path, step):
+        io.write_gro(path, self.positions(), self.box, self.e.velocities(), step=step,
+                     time=step * self.dt, resname="lj", atomname="lj")
+
+    def loop(self, nstep, thermo_feq, conf_feq, out, on_frame, hooks, pc=lambda: None,
+             fcorr=lambda: False):
+        """hooks: (period, fn(step)) called after every period-th step, after
+        that step's thermo line (the programs' order)."""
+        e = self.e
+        if on_frame is not None:
+            on_frame(0, 0.0, self.positions(), e)
+        periods = [thermo_feq] + [p for p, _ in hooks]
+        if on_frame is not None:
+            periods.append(conf_feq)
+        rows = []
+        done = 0
+        while done < nstep:
+            todo = min([nstep - done] + [p - done % p for p in periods])
+            e.md_run(todo, rebuild_every="adaptive", stats=True, fcorr=fcorr())
+            done += todo
+            if done % thermo_feq == 0:
+                v = e.velocities().astype(np.float64)
+                kin = 0.5 * (v * v).sum(0)
+                nl, _, _ = e.neighbor_count(in_rc=False)
+                text, row = thermo_line(done, self.dt, self.n, self.box, e.stats()[0], kin,
+                                        e.md_state()["nhc_energy"], nl, pc())
+                out.write(text)
+                out.flush()
+                rows.append(row)
+            for p, fn in hooks:
+                if done % p == 0:
+                    fn(done)
+            if on_frame is not None and done % conf_feq == 0:
+                on_frame(done, done * self.dt, self.positions(), e)
+        return np.array(rows, np.float64)
 
 
 def lj_nhc(conf, nstep, rcut=10.0, ref_T=0.70, tau_T=1.0, dt=0.005, skin=0.49,
            thermo_feq=100, conf_feq=2000, out=sys.stdout, on_frame=None,
            confstart="confstart.gro", confout="confout.gro", autotune=True,
            fcorr=False, refh=1.0, density_feq=40, update_fc_feq=20000):
-    """Run the driver; returns the thermo rows (float64 array, one row per
-    thermo line, columns as printed, the neighbour count last)."""
-    import torch
-
-    if not torch.cuda.is_available():
-        raise RuntimeError("lj_nhc runs on the GPU")
-    xyz, box, vel = io.read_gro(conf, velocities=True)
-    n = len(xyz)
-    box = np.asarray(box, np.float32)
-    e = Engine(n_types=1, skin=skin, precision="mixed")
-    e.set_system([n], [box])
-    e.set_types(np.zeros(n, np.int32))
-    one = np.ones((1, 1), np.float32)
-    e.set_pair_params(one, one, np.zeros((1, 1), np.float32))
-    e.set_rc_table(np.full((1, 1, 1), rcut, np.float32))
-    pos4 = np.zeros((n, 4), np.float32)
-    pos4[:, :3] = np.mod(xyz, box)            # normalizeDeviceData
-    pos = torch.from_numpy(pos4).cuda()
-    torch.cuda.synchronize()
-    e.bind_positions(pos)
-    e.set_masses([1.0])
-    e.set_velocities(vel)
-    e.set_integrator(dt, "nhc2", ref_T, tau_T, remove_com=True)
-    e.rebuild()
-    if autotune:
-        e.autotune(stats=True)
-
-    def positions():
-        return pos.cpu().numpy()[:, :3].astype(np.float64)
-
+    """Run the uniform-cutoff (or force-corrected) program; returns the thermo
+    rows (float64, one row per line, columns as printed)."""
+    r = _Run(conf, rcut, ref_T, tau_T, dt, skin, autotune)
     if confstart:
-        io.write_gro(confstart, positions(), box, e.velocities(), step=0, time=0.0,
-                     resname="lj", atomname="lj")
+        r.write_conf(confstart, 0)
     out.write("# prepare ok, start to run\n")
     out.write(HEADER)
-    if on_frame is not None:
-        on_frame(0, 0.0, positions(), e)
-    fc = None
+    hooks, fc, on = [], None, [False]
     if fcorr:
-        fc = Arc(box, refh)
+        fc = Arc(r.box, refh)
         fc.set_candidates(rcut, rcut, 1.0)
-    fc_on = False
-    rows = []
-    done = 0
-    while done < nstep:
-        todo = min(thermo_feq - done % thermo_feq, nstep - done)
-        if on_frame is not None:
-            todo = min(todo, conf_feq - done % conf_feq)
-        if fc is not None:
-            todo = min(todo, density_feq - done % density_feq,
-                       update_fc_feq - done % update_fc_feq)
-        e.md_run(todo, rebuild_every="adaptive", stats=True, fcorr=fc_on)
-        done += todo
-        if fc is not None and done % density_feq == 0:
-            fc.deposit(e)
-        if fc is not None and done % update_fc_feq == 0:
+
+        def update(step):
             fc.cal_error(keep_parts=True)
             fc.cal_rcut(1.0)
-            fc.apply_force_correction(e)
-            fc_on = True
-            if done != nstep:
+            fc.apply_force_correction(r.e)
+            on[0] = True
+            if step != nstep:
                 fc.clear()
-        if done % thermo_feq == 0:
-            v = e.velocities().astype(np.float64)
-            kin = 0.5 * (v * v).sum(0)
-            s = e.md_state()
-            nl, _, _ = e.neighbor_count(in_rc=False)
-            text, row = thermo_line(done, dt, n, box, e.stats()[0], kin, s["nhc_energy"], nl)
-            out.write(text)
-            out.flush()
-            rows.append(row)
-        if on_frame is not None and done % conf_feq == 0:
-            on_frame(done, done * dt, positions(), e)
+
+        hooks = [(density_feq, lambda step: fc.deposit(r.e)), (update_fc_feq, update)]
+    rows = r.loop(nstep, thermo_feq, conf_feq, out, on_frame, hooks,
+                  fcorr=lambda: on[0])
     if confout:
-        io.write_gro(confout, positions(), box, e.velocities(), step=nstep,
-                     time=nstep * dt, resname="lj", atomname="lj")
+        r.write_conf(confout, nstep)
     if fc is not None:
         fc.close()
-    e.close()
-    return np.array(rows, np.float64).reshape(-1, 12)
+    r.e.close()
+    return rows.reshape(-1, 12)
+
+
+def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.49,
+                 rcmin=3.0, rcmax=10.0, rcstep=0.5, target_prec=0.020, refh=1.0,
+                 density_feq=40, update_rc_feq=2000, n_refine=0, thermo_feq=100,
+                 conf_feq=2000, out=sys.stdout, on_frame=None, confstart="confstart.gro",
+                 confout="confout.gro", autotune=True, rc_trajectory=None,
+                 on_update=None):
+    """Run the adaptive-cutoff program (defaults as its source; the run
+    scripts set rcutUpdateFeq 20000 and rcutNumRefine 2).  Returns the thermo
+    rows (15 columns as printed) and the last installed rc lattice and its
+    estimated error.  rc_trajectory: a path for the rc lattice trajectory
+    (rcut.rtj layout), one frame per `conf_feq` steps; on_update(step, rc,
+    err) is called after each cutoff update."""
+    r = _Run(conf, rcut, ref_T, tau_T, dt, skin, autotune)
+    arc = Arc(r.box, refh)
+    arc.set_candidates(rcmin, rcmax, rcstep)
+    if confstart:
+        r.write_conf(confstart, 0)
+    out.write("# prepare ok, start to run\n")
+    pc = [np.zeros(3)]
+    last = [np.full(arc.dims, rcut, np.float32), None]
+    rtj = None
+    if rc_trajectory:
+        rtj = io.GridTrajectory(rc_trajectory, r.box, arc.dims, np.float32)
+
+    def update(step):
+        arc.cal_error()
+        arc.cal_rcut(target_prec)
+        arc.refine(n_refine)
+        arc.apply(r.e)
+        pc[0] = arc.pressure()
+        rc, err, _ = arc.result()
+        last[0], last[1] = rc, err
+        if on_update is not None:
+            on_update(step, rc, err)
+        if step != nstep:
+            arc.clear()
+
+    hooks = [(density_feq, lambda step: arc.deposit(r.e)), (update_rc_feq, update)]
+    if rtj is not None:
+        hooks.append((conf_feq, lambda step: rtj.write(step * dt, last[0])))
+    rows = r.loop(nstep, thermo_feq, conf_feq, out, on_frame, hooks, pc=lambda: pc[0])
+    if rtj is not None:
+        rtj.close()
+    if confout:
+        r.write_conf(confout, nstep)
+    arc.close()
+    r.e.close()
+    return rows.reshape(-1, 15), last[0], last[1]
 
 
 def main(argv):

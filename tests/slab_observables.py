@@ -4,12 +4,16 @@ Not collected by pytest (it runs for minutes); run by hand on a GPU:
 
     python tests/slab_observables.py --rc 5.0 [--fcorr] --equil 100000 \
         --steps 1000000 --out gpurun_out/slab_obs_rc5.0.json
+    python tests/slab_observables.py --adapt 0.0045 --ref-rc 7.5 ...
 
 Starts from the reference's equilibrated 16000-atom slab (T* = 0.85, box
 150 x 21 x 21, slab normal x; tests/golden/slab_t085.npz), runs the uniform-
 cutoff driver (sre.apps.lj_nhc: velocity Verlet + NHC, dt 0.005, skin 0.49;
-with --fcorr the long-range force correction updated every 20000 steps) at
-the chosen cutoff, drops `--equil` steps, and measures as the reference's
+with --fcorr the long-range force correction updated every 20000 steps;
+with --adapt PREC the adaptive-cutoff driver sre.apps.lj_nhc_adapt, which
+starts from a uniform rc 5 and every 20000 steps picks per cell the
+smallest of rc = 3, 3.25, ..., 10 within PREC, widened twice) at the chosen
+cutoff, drops `--equil` steps, and measures as the reference's
 tools do: liquid density in 70 <= x <= 80, gas density in x <= 20 or
 x >= 130, one frame per 1000 steps (R/cal.density.profile/app/density.cpp,
 limits of R/sh.scripts/rho.sh; R = shortRange.inhomo/template/tools); the
@@ -18,10 +22,17 @@ the 16-block error of the mean (R/sh.scripts/tension.uniform.sh).  The
 tension's long-range pressure correction is computed as
 R/analyze.simulation.suit/app/analyze.uniform.cpp does, from the density
 profile of the same frames on the refh = 1 grid with the uniform cutoff
-(sre_arc_pressure).
+(sre_arc_pressure); for the adaptive driver, as
+R/sh.scripts/tension.adapt.sh does, it is the mean of the correction the
+driver prints with each thermo line (that of its current cutoffs).  The
+mean neighbour-list size is reported beside the cost column of the paper's
+Table I (shortRange.inhomo/alltext/pubilsh.pre/resubmit/main.tex:995-1020),
+which counts the same pairs.
 The reference's values for the same system and cutoff come from
 shortRange.inhomo/alltext/pubilsh.pre/resubmit/fig/converge/
-t0.85.{liquid,gas,tension}.{uni,fcorr}.out (3e6-step runs, the first 1e6
+t0.85.{liquid,gas,tension}.{uni,fcorr,adapt}.out (3e6-step runs; the
+adaptive runs are listed at the uniform cutoff whose maximum error their
+PREC matches, Table I, the first 1e6
 steps dropped); their tension tables hold the corrected tension, its error,
 the correction, so the simulated tension is column 2 minus column 4.  A
 quantity agrees when |ours - ref| <= 1.5 sqrt(err_ours^2 + err_ref^2) (both
@@ -44,6 +55,12 @@ from sre import analysis, apps, io  # noqa: E402
 from sre.arc import Arc  # noqa: E402
 
 L1, L2, G1, G2 = 70.0, 80.0, 20.0, 130.0
+# main.tex Table I: mean pairwise interactions per step (1e6), by rc and method
+PAPER_COST = {("uni", 3.5): 2.9, ("adapt", 3.5): 1.9, ("fcorr", 3.5): 3.1,
+              ("uni", 5.0): 7.8, ("adapt", 5.0): 4.2, ("fcorr", 5.0): 7.9,
+              ("uni", 6.0): 13.0, ("uni", 7.5): 24.0, ("adapt", 7.5): 12.0,
+              ("fcorr", 7.5): 24.0, ("uni", 10.0): 53.0, ("adapt", 10.0): 26.0,
+              ("fcorr", 10.0): 53.0}
 FRAME_EVERY = 1000
 NBLOCK = 16
 
@@ -61,11 +78,18 @@ def main():
     ap.add_argument("--rc", type=float, default=5.0)
     ap.add_argument("--fcorr", action="store_true")
     ap.add_argument("--update-fc", type=int, default=20000)
+    ap.add_argument("--adapt", type=float, default=None, metavar="PREC")
+    # the reference's adaptive run chose rc 3.25, 4.75, 5.25 (error.x.out of
+    # fig/t0.85-n16000-adapt-e0.0045): candidates 0.25 apart
+    ap.add_argument("--rcstep", type=float, default=0.25)
+    ap.add_argument("--ref-rc", type=float, default=None,
+                    help="reference row (default: --rc)")
     ap.add_argument("--equil", type=int, default=60000)
     ap.add_argument("--steps", type=int, default=300000)
     ap.add_argument("--out", default="gpurun_out/slab_obs.json")
     args = ap.parse_args()
 
+    ref_rc = args.rc if args.ref_rc is None else args.ref_rc
     fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
     box = fx["box"]
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
@@ -87,23 +111,57 @@ def main():
         conf = os.path.join(tmp, "conf.gro")
         io.write_gro(conf, fx["xyz"], box, fx["vel"], resname="lj", atomname="lj")
         t0 = time.time()
-        kind = "fcorr" if args.fcorr else "uni"
-        print(f"slab T*=0.85 {kind} rc {args.rc}: {args.equil} + {args.steps} steps",
-              flush=True)
+        kind = "adapt" if args.adapt else "fcorr" if args.fcorr else "uni"
+        print(f"slab T*=0.85 {kind} rc {args.rc} prec {args.adapt}: "
+              f"{args.equil} + {args.steps} steps", flush=True)
+        nstep = args.equil + args.steps
         with open(thermo_path, "w") as out:
-            rows = apps.lj_nhc(conf, args.equil + args.steps, rcut=args.rc, ref_T=0.85,
-                               out=out, on_frame=on_frame, conf_feq=FRAME_EVERY,
-                               confstart=None, confout=None, fcorr=args.fcorr,
-                               update_fc_feq=args.update_fc)
+            if args.adapt:
+                # rc along x of each update in the measured range (each is in
+                # force for the same number of steps)
+                rc_sum = [None, 0]
+
+                def on_update(step, rc, err):
+                    if step >= args.equil and step < nstep:
+                        x = rc.mean((1, 2)).astype(np.float64)
+                        rc_sum[0] = x if rc_sum[0] is None else rc_sum[0] + x
+                        rc_sum[1] += 1
+
+                rows, rc_last, _ = apps.lj_nhc_adapt(
+                    conf, nstep, rcut=args.rc, ref_T=0.85, tau_T=1.0, rcmin=3.0, rcmax=10.0,
+                    rcstep=args.rcstep, target_prec=args.adapt, update_rc_feq=20000,
+                    n_refine=2,
+                    out=out, on_frame=on_frame, conf_feq=FRAME_EVERY, confstart=None,
+                    confout=None, on_update=on_update)
+                rc_mean_x = rc_sum[0] / rc_sum[1]
+            else:
+                rows = apps.lj_nhc(conf, nstep, rcut=args.rc, ref_T=0.85, out=out,
+                                   on_frame=on_frame, conf_feq=FRAME_EVERY, confstart=None,
+                                   confout=None, fcorr=args.fcorr,
+                                   update_fc_feq=args.update_fc)
         wall = time.time() - t0
     keep = rows[:, 0] > args.equil
-    res = {"kind": kind, "rc": args.rc, "equil": args.equil, "steps": args.steps,
-           "wall_s": wall, "ms_per_step": 1e3 * wall / (args.equil + args.steps),
-           "temperature": float(rows[keep, 4].mean())}
+    res = {"kind": kind, "rc": args.rc, "prec": args.adapt, "ref_rc": ref_rc,
+           "rcstep": args.rcstep if args.adapt else None,
+           "equil": args.equil, "steps": args.steps,
+           "wall_s": wall, "ms_per_step": 1e3 * wall / nstep,
+           "temperature": float(rows[keep, 4].mean()),
+           "pairs_per_step": float(rows[keep, -1].mean()),
+           "paper_cost": 1e6 * PAPER_COST.get((kind, ref_rc), float("nan"))}
+    if args.adapt:
+        # the run-averaged rc along x against the reference's a.rcut.x.out
+        # (its run at PREC 0.0045; tests/golden/arc_t085.npz)
+        res["rc_mean_x"] = [round(float(v), 4) for v in rc_mean_x]
+        if args.adapt == 0.0045:
+            a = np.load(os.path.join(HERE, "golden", "arc_t085.npz"))["a_rcut"][:, 1]
+            res["rc_mean_x_vs_reference"] = {
+                "mean_abs_diff": float(np.abs(rc_mean_x - a).mean()),
+                "max_abs_diff": float(np.abs(rc_mean_x - a).max()),
+                "mean_ours": float(rc_mean_x.mean()), "mean_ref": float(a.mean())}
     ok = True
     for name, data in (("liquid", liq), ("gas", gas), ("tension", rows[keep, 10])):
         b = analysis.BlockAverage(data, NBLOCK)
-        ref = reference_row(fx, f"{name}_{kind}", args.rc)
+        ref = reference_row(fx, f"{name}_{kind}", ref_rc)
         val = ref[1] - ref[3] if name == "tension" else ref[1]
         err = 2.0 * b.error
         bound = 1.5 * np.hypot(err, ref[2])
@@ -113,12 +171,17 @@ def main():
                      "ref_err2s": float(ref[2]), "bound": float(bound), "agree": agree}
         print(f"{name:8s} ours {b.avg:.4f} ({err:.4f})  ref {val:.4f} ({ref[2]:.4f})"
               f"  {'ok' if agree else 'DIFFERS'}", flush=True)
-    # long-range pressure correction of the tension from the sampled profile
-    prof.cal_error()
-    prof.cal_rcut(1.0)
-    pc = prof.pressure()
-    corr = analysis.tension(pc, box[0])
-    ref = reference_row(fx, f"tension_{kind}", args.rc)
+    # long-range pressure correction of the tension: from the sampled profile
+    # at the uniform cutoff, or the adaptive driver's own (column 14)
+    if args.adapt:
+        pc = [float("nan")] * 3
+        corr = float(rows[keep, 13].mean())
+    else:
+        prof.cal_error()
+        prof.cal_rcut(1.0)
+        pc = prof.pressure()
+        corr = analysis.tension(pc, box[0])
+    ref = reference_row(fx, f"tension_{kind}", ref_rc)
     agree = bool(abs(corr - ref[3]) <= 0.1 * abs(ref[3]))
     ok &= agree
     res["tension_correction"] = {"ours": float(corr), "ref": float(ref[3]),

@@ -63,3 +63,32 @@ def test_lj_nhc_force_corrected_driver(torch_cuda, tmp_path):
     assert np.array_equal(runs[200][200], runs[400][200])
     d = np.abs(runs[200][400] - runs[400][400])
     assert d.max() > 1e-4
+
+
+def test_lj_nhc_adapt_driver(torch_cuda, tmp_path):
+    """The adaptive-cutoff driver: fifteen thermo columns, the pressure
+    correction of the installed cutoffs from the first update on, a cutoff
+    lattice made of candidates that is wider at the interfaces than in the
+    bulk liquid, and its rc trajectory file."""
+    fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
+    conf = tmp_path / "conf.gro"
+    io.write_gro(conf, fx["xyz"], fx["box"], fx["vel"], resname="lj", atomname="lj")
+    rtj = tmp_path / "rcut.rtj"
+    rows, rc, err = apps.lj_nhc_adapt(
+        str(conf), 800, rcut=5.0, ref_T=0.85, target_prec=0.0045, update_rc_feq=400,
+        n_refine=2, out=_io.StringIO(), conf_feq=400, confstart=None, confout=None,
+        rc_trajectory=str(rtj))
+    assert rows.shape == (8, 15)
+    assert np.all(rows[:4, 11:14] == 0.0)
+    assert np.all(rows[4:, 13] > 0.0)         # the correction raises the tension
+    cands = np.arange(3.0, 10.01, 0.5)
+    assert rc.shape == (151, 21, 21)
+    assert np.all(np.isin(rc, cands.astype(np.float32)))
+    prof = rc.mean((1, 2))
+    assert prof[70:80].max() < prof[45:60].max()   # bulk liquid vs the left interface
+    assert np.all(err[70:80] <= 0.0045 * 1.0001)
+    box, dims, times, frames = io.read_grid_trajectory(rtj, np.float32)
+    assert dims == (151, 21, 21) and np.allclose(times, [2.0, 4.0])
+    # a frame holds the lattice installed at or before its step
+    assert np.all(np.isin(frames[0], cands.astype(np.float32)))
+    assert np.array_equal(frames[1], rc)
