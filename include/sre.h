@@ -113,7 +113,8 @@ sre_status sre_set_pair_params(sre_engine *e, const float *eps,
 /* Cutoff lattice for replica r: rc[iz + nz*(iy + ny*ix)], FP32.  Replaces
  * AssignRCut::reinit/getRCut/uniform (H/src/AssignRCut.cu:17-63).  The lattice
  * is assigned to atoms at every rebuild with the reference's bin rule
- * (H/src/AssignRCut.cu:77-104). */
+ * (H/src/AssignRCut.cu:77-104), or at sre_assign_rcut calls once that is
+ * used. */
 sre_status sre_set_rc_table(sre_engine *e, int replica, int nx, int ny, int nz,
                             const float *rc);
 
@@ -191,6 +192,17 @@ sre_status sre_neighbor_count(sre_engine *e, int64_t *n_list, int64_t *n_in_rc,
 sre_status sre_export_pairs(sre_engine *e, int64_t *offsets, int32_t *j);
 /* Per-atom cutoff rc_i (caller order, host float[N]) as last assigned. */
 sre_status sre_get_rcut(sre_engine *e, float *rc);
+/* Replaces AssignRCut::assign (H/src/AssignRCut.cu:107-117; the adaptive
+ * driver calls it every rcutAssignFeq steps, H/app/lj.nhc-xtc.cu:184-188):
+ * every atom's rc_i from the current rc tables at its current position,
+ * without rebuilding the lists.  By default the engine assigns rc_i at each
+ * rebuild instead, so that lists always hold every pair within rc_i + skin;
+ * after the first call it keeps each atom's assigned rc_i across rebuilds
+ * until the next call, as the reference does (a list then misses the pairs
+ * of an atom whose rc_i grew beyond its rc_i + skin of the last rebuild,
+ * until the next rebuild).  rc gradients (sre_adjoint g_rc) are refused in
+ * this mode.  sre_set_system returns to assignment at rebuild. */
+sre_status sre_assign_rcut(sre_engine *e);
 
 /* ---- adjoint (no reference counterpart) ----------------------------------- */
 

@@ -118,6 +118,10 @@ struct sre_engine {
   float4 *d_res4 = nullptr;   // FP64 form only
   float *d_rcut = nullptr;
   int *d_tail_idx = nullptr;
+  // rc per atom in caller order, set by sre_assign_rcut; once it has been
+  // called, rebuilds carry these values instead of reading the table
+  float *d_rc_atom = nullptr;
+  bool rc_explicit = false;
 
   // neighbor list
   int cap = 0;
@@ -211,7 +215,7 @@ sre_engine::~sre_engine() {
   dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
   dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
   dfree(d_inv_perm);
-  dfree(d_rcut); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
+  dfree(d_rcut); dfree(d_rc_atom); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
@@ -352,6 +356,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   const int N = e->n_atoms;
   dfree(e->d_rep_of); dfree(e->d_types); dfree(e->d_reps);
   dfree(e->d_pos4); dfree(e->d_backup); dfree(e->d_rcut); dfree(e->d_tail_idx);
+  dfree(e->d_rc_atom);
+  e->rc_explicit = false;
   dfree(e->d_inv_perm);
   dfree(e->d_res4);
   dfree(e->d_nnb); dfree(e->d_fx); dfree(e->d_fy); dfree(e->d_fz);
@@ -373,6 +379,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->d_backup, N));
   HIP_TRY(dalloc(&e->d_inv_perm, N));
   HIP_TRY(dalloc(&e->d_rcut, N));
+  HIP_TRY(dalloc(&e->d_rc_atom, N));
   HIP_TRY(dalloc(&e->d_tail_idx, N));
   HIP_TRY(dalloc(&e->d_nnb, N));
   if (e->precision == SRE_PREC_MIXED) {
@@ -825,7 +832,8 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
   const int ints[] = {e->n_atoms, e->key_bits, e->n_cells, e->n_cols, e->n_rep,
                       e->cap, e->tune.build_mode, e->tune.build_lanes,
                       e->tune.force_lanes, e->n_types, e->pos_stride, n_parts,
-                      (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) ? 1 : 0};
+                      (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) ? 1 : 0,
+                      e->rc_explicit ? 1 : 0};
   put(ints, sizeof ints);
   put(&e->skin, sizeof e->skin);
   const void *ptrs[] = {e->d_pos, e->d_rep_of, e->d_reps, e->sb.key_a, e->sb.key_b,
@@ -833,7 +841,8 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
                         e->sb.counts, e->sb.offsets, e->sb.scan_tmp, e->d_start,
                         e->d_types, e->d_rc, e->d_pos4, e->d_backup,
                         e->d_inv_perm, e->d_res4,
-                        e->d_rcut, e->d_tail_idx, e->d_tail_atom, e->d_paramsd,
+                        e->d_rcut, e->d_rc_atom, e->d_tail_idx, e->d_tail_atom,
+                        e->d_paramsd,
                         e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
                         e->d_counters, e->d_skin_part};
@@ -865,7 +874,8 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   launch_gather_sorted(N, e->n_cells, e->sb.key_a, e->sb.val_a, e->d_pos,
                        e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
                        e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
-                       e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx, st);
+                       e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
+                       e->rc_explicit ? e->d_rc_atom : nullptr, st);
   if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);
@@ -1248,6 +1258,31 @@ extern "C" sre_status sre_get_rcut(sre_engine *e, float *rc) {
   return SRE_OK;
 }
 
+// AssignRCut::assign (H/src/AssignRCut.cu:107-117), called by the adaptive
+// driver every rcutAssignFeq steps (H/app/lj.nhc-xtc.cu:184-188): cutoffs
+// from the current table at the current positions, without a list rebuild.
+extern "C" sre_status sre_assign_rcut(sre_engine *e) {
+  if (!e) return fail(SRE_ERR_INVALID, "null engine");
+  if (!e->system_set || !e->d_pos)
+    return fail(SRE_ERR_NOT_READY, "sre_assign_rcut: system or positions not set");
+  HIP_TRY(hipSetDevice(e->device));
+  sre_status s = prepare_tables(e);
+  if (s != SRE_OK) return s;
+  if (!e->rc_explicit) {
+    e->rc_explicit = true;
+    e->graph_epoch++;   // captured rebuilds now carry rc_atom
+  }
+  launch_assign_rc(e->n_atoms, e->d_pos, e->pos_stride, e->d_rep_of, e->d_reps,
+                   e->d_rc, e->built ? e->d_inv_perm : nullptr, e->d_rc_atom,
+                   e->d_rcut, e->stream);
+  HIP_TRY(hipGetLastError());
+  if (e->built && e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
+    launch_tail_exact(e->n_atoms, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
+                      e->d_paramsd, e->d_rho, e->d_tail_atom, e->stream);
+  e->forces_valid = false;
+  return SRE_OK;
+}
+
 // ---------------------------------------------------------------- adjoint --
 
 extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
@@ -1257,6 +1292,9 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
   if (!e || !g_eps || !g_sigma)
     return fail(SRE_ERR_INVALID, "sre_adjoint: null argument");
   if (!e->built) return fail(SRE_ERR_NOT_READY, "adjoint before the first rebuild");
+  if (g_rc && e->rc_explicit)
+    return fail(SRE_ERR_INVALID,
+                "rc gradients need cutoffs assigned at rebuild (sre_assign_rcut was used)");
   const bool use_tail = (flags & SRE_TAIL) && e->tail_bins > 0;
   if (use_tail && e->tail_mode != SRE_TAIL_BIN)
     return fail(SRE_ERR_INVALID, "adjoint of the tail term needs SRE_TAIL_BIN");

@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
     const float *pos, int stride, const int *types, const int *rep_of,
     const RepInfo *reps, const float *rc_table, int n_types, int *start,
     uint4 *pos4, uint4 *backup, uint32_t *inv_perm, float4 *res4, float *rcut,
-    int *tail_idx) {
+    int *tail_idx, const float *rc_atom) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > n) return;
   {
@@ -380,8 +380,40 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
   const int ix = rc_bin(x, R.L[0], R.rc_n[0]);
   const int iy = rc_bin(y, R.L[1], R.rc_n[1]);
   const int iz = rc_bin(z, R.L[2], R.rc_n[2]);
-  rcut[s] = rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
+  rcut[s] = rc_atom ? rc_atom[i]
+                    : rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
   tail_idx[s] = (R.tail_off >= 0) ? (R.tail_off + iz) * n_types + ty : -1;
+}
+
+// AssignRCut::assign between rebuilds (H/src/AssignRCut.cu:77-117): every
+// atom's rc from the table at its current position, kept per atom in caller
+// order (rc_atom, which later rebuilds carry into cell order) and written to
+// the atom's slot of the current list order (inv_perm, when a list exists).
+__global__ __launch_bounds__(256) void k_assign_rc(int n, const float *pos, int stride,
+                                                   const int *rep_of,
+                                                   const RepInfo *reps,
+                                                   const float *rc_table,
+                                                   const uint32_t *inv_perm,
+                                                   float *rc_atom, float *rcut) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const RepInfo &R = reps[rep_of[i]];
+  const float *p = pos + (size_t)i * stride;
+  const int ix = rc_bin(p[0], R.L[0], R.rc_n[0]);
+  const int iy = rc_bin(p[1], R.L[1], R.rc_n[1]);
+  const int iz = rc_bin(p[2], R.L[2], R.rc_n[2]);
+  const float v = rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
+  rc_atom[i] = v;
+  if (inv_perm) rcut[inv_perm[i]] = v;
+}
+
+void launch_assign_rc(int n, const float *pos, int stride, const int *rep_of,
+                      const RepInfo *reps, const float *rc_table,
+                      const uint32_t *inv_perm, float *rc_atom, float *rcut,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assign_rc, dim3((n + 255) / 256), dim3(256), 0, st, n, pos,
+                     stride, rep_of, reps, rc_table, inv_perm, rc_atom, rcut);
 }
 
 void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
@@ -390,11 +422,11 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
                           uint32_t *inv_perm, float4 *res4, float *rcut,
-                          int *tail_idx, hipStream_t st) {
+                          int *tail_idx, const float *rc_atom, hipStream_t st) {
   hipLaunchKernelGGL(k_gather_sorted, dim3((n + 1 + 255) / 256), dim3(256), 0,
                      st, n, n_cells, keys, perm, pos, stride, types, rep_of,
                      reps, rc_table, n_types, start, pos4, backup, inv_perm,
-                     res4, rcut, tail_idx);
+                     res4, rcut, tail_idx, rc_atom);
 }
 
 // ------------------------------------------------------------------ fills --

@@ -182,7 +182,13 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
                           uint32_t *inv_perm, float4 *res4, float *rcut,
-                          int *tail_idx, hipStream_t st);
+                          int *tail_idx, const float *rc_atom, hipStream_t st);
+// rc_atom[i] (caller order) from the table at the atom's current position,
+// also into rcut[inv_perm[i]] when inv_perm is given
+void launch_assign_rc(int n, const float *pos, int stride, const int *rep_of,
+                      const RepInfo *reps, const float *rc_table,
+                      const uint32_t *inv_perm, float *rc_atom, float *rcut,
+                      hipStream_t st);
 // Both builders write one (longest list, total length) pair per wavefront to
 // part[] (build_part_count entries, every one written by every launch);
 // launch_rebuild_info reduces them.

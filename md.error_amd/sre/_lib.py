@@ -87,6 +87,7 @@ SIGNATURES = {
     "sre_neighbor_count": (_st, [_vp, _i64p, _i64p, _i32p]),
     "sre_export_pairs": (_st, [_vp, _i64p, _i32p]),
     "sre_get_rcut": (_st, [_vp, _f32p]),
+    "sre_assign_rcut": (_st, [_vp]),
     "sre_adjoint": (_st, [_vp, _f64p, _vp, C.c_uint, _f64p, _f64p, _f64p]),
     "sre_get_tuning": (_st, [_vp, C.POINTER(Tuning)]),
     "sre_set_tuning": (_st, [_vp, C.POINTER(Tuning)]),

@@ -184,13 +184,16 @@ def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.
                  density_feq=40, update_rc_feq=2000, n_refine=0, thermo_feq=100,
                  conf_feq=2000, out=sys.stdout, on_frame=None, confstart="confstart.gro",
                  confout="confout.gro", autotune=True, rc_trajectory=None,
-                 on_update=None):
+                 on_update=None, assign_feq=None):
     """Run the adaptive-cutoff program (defaults as its source; the run
     scripts set rcutUpdateFeq 20000 and rcutNumRefine 2).  Returns the thermo
     rows (15 columns as printed) and the last installed rc lattice and its
     estimated error.  rc_trajectory: a path for the rc lattice trajectory
     (rcut.rtj layout), one frame per `conf_feq` steps; on_update(step, rc,
-    err) is called after each cutoff update."""
+    err) is called after each cutoff update.  assign_feq: None assigns each
+    atom's cutoff at every list rebuild (lists always cover rc_i + skin); a
+    number reassigns every assign_feq steps without a rebuild, as the program
+    does (rcutAssignFeq 40, H/app/lj.nhc-xtc.cu:184-188)."""
     r = _Run(conf, rcut, ref_T, tau_T, dt, skin, autotune)
     arc = Arc(r.box, refh)
     arc.set_candidates(rcmin, rcmax, rcstep)
@@ -217,6 +220,9 @@ def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.
             arc.clear()
 
     hooks = [(density_feq, lambda step: arc.deposit(r.e)), (update_rc_feq, update)]
+    if assign_feq:
+        r.e.assign_rcut()
+        hooks.append((assign_feq, lambda step: r.e.assign_rcut()))
     if rtj is not None:
         hooks.append((conf_feq, lambda step: rtj.write(step * dt, last[0])))
     rows = r.loop(nstep, thermo_feq, conf_feq, out, on_frame, hooks, pc=lambda: pc[0])

@@ -254,6 +254,12 @@ class Engine:
         check(lib().sre_get_rcut(self._h, _ptr(out, C.c_float)))
         return out
 
+    def assign_rcut(self):
+        """AssignRCut::assign: rc_i from the tables at the current positions,
+        without a list rebuild; from then on rc_i changes only at these calls
+        (the reference's timing) instead of at every rebuild."""
+        check(lib().sre_assign_rcut(self._h))
+
     def adjoint(self, dLdE=None, dLdF=None, tail=False, n_rc=None):
         """Gradient of sum_r dLdE[r] E_r + sum_i dLdF_i . F_i.
 

@@ -82,6 +82,9 @@ def main():
     # the reference's adaptive run chose rc 3.25, 4.75, 5.25 (error.x.out of
     # fig/t0.85-n16000-adapt-e0.0045): candidates 0.25 apart
     ap.add_argument("--rcstep", type=float, default=0.25)
+    # reassign cutoffs every N steps without a list rebuild (the reference's
+    # rcutAssignFeq) instead of at every rebuild
+    ap.add_argument("--assign-feq", type=int, default=None)
     ap.add_argument("--ref-rc", type=float, default=None,
                     help="reference row (default: --rc)")
     ap.add_argument("--equil", type=int, default=60000)
@@ -132,7 +135,7 @@ def main():
                     rcstep=args.rcstep, target_prec=args.adapt, update_rc_feq=20000,
                     n_refine=2,
                     out=out, on_frame=on_frame, conf_feq=FRAME_EVERY, confstart=None,
-                    confout=None, on_update=on_update)
+                    confout=None, on_update=on_update, assign_feq=args.assign_feq)
                 rc_mean_x = rc_sum[0] / rc_sum[1]
             else:
                 rows = apps.lj_nhc(conf, nstep, rcut=args.rc, ref_T=0.85, out=out,
@@ -143,6 +146,7 @@ def main():
     keep = rows[:, 0] > args.equil
     res = {"kind": kind, "rc": args.rc, "prec": args.adapt, "ref_rc": ref_rc,
            "rcstep": args.rcstep if args.adapt else None,
+           "assign_feq": args.assign_feq,
            "equil": args.equil, "steps": args.steps,
            "wall_s": wall, "ms_per_step": 1e3 * wall / nstep,
            "temperature": float(rows[keep, 4].mean()),
