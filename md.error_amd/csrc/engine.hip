@@ -68,6 +68,7 @@ struct sre_engine {
 
   // force field
   std::vector<float> eps, sig, shift;
+  bool has_shift = false;
   PairParam *d_params = nullptr;
   PairParamD *d_paramsd = nullptr;
   bool params_set = false;
@@ -483,6 +484,12 @@ extern "C" sre_status sre_set_pair_params(sre_engine *e, const float *eps,
   e->sig.assign(sigma, sigma + nt * nt);
   if (shift) e->shift.assign(shift, shift + nt * nt);
   else e->shift.assign(nt * nt, 0.f);
+  const bool has_shift =
+      std::any_of(e->shift.begin(), e->shift.end(), [](float v) { return v != 0.f; });
+  if (has_shift != e->has_shift) {
+    e->has_shift = has_shift;
+    e->graph_epoch++;   // the captured force kernel variant depends on it
+  }
   sre_status s = upload_params(e);
   if (s != SRE_OK) return s;
   e->params_set = true;
@@ -1016,6 +1023,7 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.rcut = e->d_rcut;
   a.nnb = e->d_nnb;
   a.list = e->d_list;
+  a.list_len = e->list_size;
   a.reps = e->d_reps;
   a.params = e->d_params;
   a.paramsd = e->d_paramsd;
@@ -1028,6 +1036,7 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.err_flag = e->d_flags + 1;
   a.step_ctr = nullptr;
   a.xcd_remap = e->tune.xcd_remap;
+  a.has_shift = e->has_shift ? 1 : 0;
   return a;
 }
 

@@ -194,21 +194,33 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return x * q + min(x, r) + (b >> 3);
 }
 
-// Position gathers through a buffer resource over pos4: the 32-bit byte
-// offset j * 16 replaces a 64-bit address computation per pair (pos4 holds
-// fewer than 2^27 atoms, checked when the system is set).  gfx9 resource
-// word 3 = 0x00020000 (32-bit data format, no swizzle).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc(const uint4 *p, int n) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(p), (short)0,
-                                           n * 16, 0x00020000);
+// Position gathers through a buffer resource over pos4 in index mode: a
+// resource with a 16-byte record stride takes the atom index itself
+// (buffer_load ... idxen), so a gather needs neither a 64-bit address nor a
+// per-pair shift (pos4 holds fewer than 2^27 atoms, checked when the system
+// is set).  gfx9 resource word 3 = 0x00020000 (32-bit data format, no
+// swizzle).  These are the LLVM structured buffer loads, which this compiler
+// exposes under their intrinsic names only.
+typedef unsigned int sre_u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned int sre_u32x4 __attribute__((ext_vector_type(4)));
+extern "C" __device__ sre_u32x3 sre_struct_load_v3(__amdgpu_buffer_rsrc_t rsrc, int vindex,
+                                                   int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v3i32");
+extern "C" __device__ sre_u32x4 sre_struct_load_v4(__amdgpu_buffer_rsrc_t rsrc, int vindex,
+                                                   int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc_idx(const uint4 *p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(p), (short)16, n * 16,
+                                           0x00020000);
 }
 template <bool W>
-__device__ __forceinline__ uint4 gather_pos(__amdgpu_buffer_rsrc_t r, int j) {
+__device__ __forceinline__ uint4 gather_pos_idx(__amdgpu_buffer_rsrc_t r, int j) {
   if (W) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, j * 16, 0, 0);
+    const sre_u32x4 v = sre_struct_load_v4(r, j, 0, 0, 0);
     return make_uint4(v[0], v[1], v[2], v[3]);
   } else {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, j * 16, 0, 0);
+    const sre_u32x3 v = sre_struct_load_v3(r, j, 0, 0, 0);
     return make_uint4(v[0], v[1], v[2], 0u);
   }
 }
@@ -230,7 +242,7 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
 // pair component c is already c*2^32 in FP32 and its fixed-point value is the
 // bit pattern of double(c*2^32) + 1.5*2^52.
-template <int T, int U, bool STATS, bool TAIL, bool MULTI>
+template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // the position refresh of this step has read the counter (stream order)
   if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
@@ -258,6 +270,9 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   const float rc = a.rcut[si];
   const float rc2 = rc * rc;
   const int nn = act ? a.nnb[si] : 0;
+  // entry (g U + u) T + l of this lane exists iff (g U + u) T < lim, a
+  // comparison of a wave-uniform index with one per-lane bound
+  const int lim = nn - l;
   const int ti = (int)(pi.w & 0xffu) * a.n_types;
   const int groups = __builtin_amdgcn_readfirstlane(
       wave_max_int((nn + T * U - 1) / (T * U)));
@@ -269,33 +284,45 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // 2^53 (checked at the end)
   double ax = 0.0, ay = 0.0, az = 0.0;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
-  const int *lp = a.list + (size_t)wave * (size_t)a.cap * A + lane;
-  const __amdgpu_buffer_rsrc_t prs = pos_rsrc(a.pos4, a.n);
+  // list rows through a resource based at this wave's tile: the lane's byte
+  // offset is fixed and the row offset is a scalar, so a row load needs no
+  // per-lane address arithmetic; the resource ends with the list allocation
+  const size_t tile0 = (size_t)__builtin_amdgcn_readfirstlane(wave) * (size_t)a.cap * A;
+  const size_t lrem = a.list_len > tile0 ? a.list_len - tile0 : 0;
+  const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<int *>(a.list + tile0), (short)0,
+      (int)min(lrem * 4, (size_t)0x7fffffff), 0x00020000);
+  const int loff = lane * 4;
+  auto row = [&](int r) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(lrs, loff, r * (kWave * 4), 0);
+  };
+  const __amdgpu_buffer_rsrc_t prs = pos_rsrc_idx(a.pos4, a.n);
   // software pipeline over two position buffers: while group g is evaluated
   // from one, the gathers of group g+1 fill the other and the list rows of
   // group g+2 are in flight
   auto eval_group = [&](const uint4 *pj, int g) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool v = (g * U + u) * T + l < nn;
+      const bool v = (g * U + u) * T < lim;
       const float dx = disp(pj[u].x, pi.x, qx);
       const float dy = disp(pj[u].y, pi.y, qy);
       const float dz = disp(pj[u].z, pi.z, qz);
       const float d2 = dist2(dx, dy, dz);
       const bool in = v && (d2 < rc2);
       if (MULTI) P = sp[ti + (int)(pj[u].w & 0xffu)];
-      const float ri2 = __builtin_amdgcn_rcpf(d2);
+      // a pair outside the cutoff (or past the list) gets 1/r^2 = 0, which
+      // zeroes its force and its unshifted energy
+      const float ri2 = in ? __builtin_amdgcn_rcpf(d2) : 0.f;
       const float ri6 = ri2 * ri2 * ri2;
-      float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
-      fs = in ? fs : 0.f;
+      const float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
       fsmax = fmaxf(fsmax, fabsf(fs));
       const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
       ax += (double)__builtin_rintf(fpx);
       ay += (double)__builtin_rintf(fpy);
       az += (double)__builtin_rintf(fpz);
       if (STATS) {
-        const float ep = __builtin_fmaf(ri6, __builtin_fmaf(P.A, ri6, -P.B), P.shift);
-        e += in ? ep : 0.f;
+        const float t = __builtin_fmaf(P.A, ri6, -P.B);
+        e += SHIFT ? __builtin_fmaf(ri6, t, in ? P.shift : 0.f) : ri6 * t;
         wx = __builtin_fmaf(fpx, dx, wx);
         wy = __builtin_fmaf(fpy, dy, wy);
         wz = __builtin_fmaf(fpz, dz, wz);
@@ -308,17 +335,17 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   auto prefetch = [&](uint4 *p, int *jr, int g) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      p[u] = gather_pos<MULTI>(prs, ((g * U + u) * T + l < nn) ? jr[u] : si);
+      p[u] = gather_pos_idx<MULTI>(prs, (g * U + u) * T < lim ? jr[u] : si);
 #pragma unroll
-    for (int u = 0; u < U; ++u) jr[u] = lp[(size_t)((g + 2) * U + u) * kWave];
+    for (int u = 0; u < U; ++u) jr[u] = row((g + 2) * U + u);
     asm volatile("" ::: "memory");
   };
   int ja[U], jb[U];   // list rows of the groups due in buffers a and b
   uint4 pa[U], pb[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) ja[u] = lp[(size_t)u * kWave];
+  for (int u = 0; u < U; ++u) ja[u] = row(u);
 #pragma unroll
-  for (int u = 0; u < U; ++u) jb[u] = lp[(size_t)(U + u) * kWave];
+  for (int u = 0; u < U; ++u) jb[u] = row(U + u);
   prefetch(pa, ja, 0);
   for (int g = 0; g < groups; g += 2) {
     prefetch(pb, jb, g + 1);
@@ -472,15 +499,25 @@ int force_num_waves(int n, int lanes) {
   return (n + A - 1) / A;
 }
 
+template <int T, int U, bool STATS, bool TAIL, bool SHIFT>
+static void launch_mixed_s(int grid, int block, bool multi, const ForceArgs &a,
+                           hipStream_t st) {
+  if (multi)
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, true, SHIFT>), dim3(grid),
+                       dim3(block), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, false, SHIFT>), dim3(grid),
+                       dim3(block), 0, st, a);
+}
+
+// the energy shift only enters the statistics
 template <int T, int U, bool STATS, bool TAIL>
 static void launch_mixed_u(int grid, int block, bool multi, const ForceArgs &a,
                            hipStream_t st) {
-  if (multi)
-    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, true>), dim3(grid),
-                       dim3(block), 0, st, a);
+  if (STATS && a.has_shift)
+    launch_mixed_s<T, U, STATS, TAIL, STATS>(grid, block, multi, a, st);
   else
-    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, false>), dim3(grid),
-                       dim3(block), 0, st, a);
+    launch_mixed_s<T, U, STATS, TAIL, false>(grid, block, multi, a, st);
 }
 
 template <int T, bool STATS, bool TAIL>

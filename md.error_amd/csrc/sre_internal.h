@@ -295,6 +295,7 @@ struct ForceArgs {
   const float *rcut;
   const int *nnb;
   const int *list;
+  size_t list_len;             // ints allocated for list
   const RepInfo *reps;
   const PairParam *params;     // n_types^2
   const PairParamD *paramsd;   // n_types^2 (FP64 form)
@@ -307,6 +308,7 @@ struct ForceArgs {
   unsigned *err_flag;
   uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
+  int has_shift;               // some pair type has a nonzero energy shift
 };
 // unroll: list rows per group of the mixed kernel (1, 2 or 4)
 void launch_force(int precision, int lanes, int unroll, int block, bool stats,
