@@ -35,8 +35,14 @@ __global__ __launch_bounds__(256) void k_update_positions(
     const uint4 *__restrict__ backup, float4 *__restrict__ res4,
     float *__restrict__ skin_part, PerturbArgs pert, DriftArgs drift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = i / kWave;  // slots exist for the waves holding atoms
+  const bool wslot = w * kWave < n;
+  // issued first: this wave's running maximum, compared at the end
+  const float prev = ((threadIdx.x & (kWave - 1)) == 0 && wslot) ? skin_part[w] : 0.f;
   float d2 = 0.f;
-  if (i < n) {
+  // R: the atom's replica.  With one replica it is the same for every atom
+  // and is read once per wave instead of after each atom's backup word.
+  auto body = [&](const RepInfo &R) {
     const uint4 b = backup[i];
     const uint32_t s = inv_perm[i];
     float *p = pos + (size_t)i * stride;
@@ -62,13 +68,12 @@ __global__ __launch_bounds__(256) void k_update_positions(
       v[0] = vx;
       v[1] = vy;
       v[2] = vz;
-      const RepInfo &R0 = reps[b.w >> 8];
       x = p[0] + vx * drift.dt;
       y = p[1] + vy * drift.dt;
       z = p[2] + vz * drift.dt;
-      x = x < 0.f ? x + R0.L[0] : (x >= R0.L[0] ? x - R0.L[0] : x);
-      y = y < 0.f ? y + R0.L[1] : (y >= R0.L[1] ? y - R0.L[1] : y);
-      z = z < 0.f ? z + R0.L[2] : (z >= R0.L[2] ? z - R0.L[2] : z);
+      x = x < 0.f ? x + R.L[0] : (x >= R.L[0] ? x - R.L[0] : x);
+      y = y < 0.f ? y + R.L[1] : (y >= R.L[1] ? y - R.L[1] : y);
+      z = z < 0.f ? z + R.L[2] : (z >= R.L[2] ? z - R.L[2] : z);
       p[0] = x;
       p[1] = y;
       p[2] = z;
@@ -77,7 +82,6 @@ __global__ __launch_bounds__(256) void k_update_positions(
       y = p[1];
       z = p[2];
     }
-    const RepInfo &R = reps[b.w >> 8];
     const uint32_t ux = quantize(x, R.invq[0]);
     const uint32_t uy = quantize(y, R.invq[1]);
     const uint32_t uz = quantize(z, R.invq[2]);
@@ -88,13 +92,17 @@ __global__ __launch_bounds__(256) void k_update_positions(
                             quant_residual(z, R.invq[2], R.qd[2]), 0.f);
     d2 = dist2(disp(ux, b.x, R.q[0]), disp(uy, b.y, R.q[1]),
                disp(uz, b.z, R.q[2]));
+  };
+  if (i < n) {
+    if (reps[0].n_atoms == n)   // wave-uniform
+      body(reps[0]);
+    else
+      body(reps[backup[i].w >> 8]);
   }
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1)
     d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
-  const int w = i / kWave;  // slots exist for the waves holding atoms
-  if ((threadIdx.x & (kWave - 1)) == 0 && w * kWave < n && d2 > skin_part[w])
-    skin_part[w] = d2;
+  if ((threadIdx.x & (kWave - 1)) == 0 && wslot && d2 > prev) skin_part[w] = d2;
 }
 
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
