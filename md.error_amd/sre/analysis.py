@@ -9,6 +9,10 @@ compute them (R = shortRange.inhomo/template/tools):
                  atoms whose x (wrapped once into [0, Lx)) lies in
                  [l1, l2] (liquid) or in [0, g1] or [g2, Lx) (gas), divided by
                  the region volumes.
+  ErrorProfile   R/cal.real.error/src/ErrorProfile.cpp:6-122 — the RMS of
+                 a per-atom force error over the atoms of each x slab of
+                 width ~refh (an odd number of slabs), as print_x_avg writes
+                 it: sqrt(sum |dF|^2 / count) over the slab.
   tension        column 11 of the MD driver's thermo line,
                  (P_xx - (P_yy + P_zz) / 2) * Lx / 2
                  (R/simulation.uniform.cutoff/app/lj.nhc.cu:158-171), averaged
@@ -61,3 +65,32 @@ def pressure_diag(stats, kinetic_diag, box):
 
 def tension(p, lx):
     return (p[0] - 0.5 * (p[1] + p[2])) * 0.5 * float(lx)
+
+
+class ErrorProfile:
+    """RMS force error per x slab (the reference's real-error profile)."""
+
+    def __init__(self, box, refh):
+        self.lx = float(box[0])
+        nx = int(self.lx / refh)
+        nx += 1 - nx % 2
+        self.nx = nx
+        self.hx = self.lx / nx
+        self.sum = np.zeros(nx)
+        self.count = np.zeros(nx, np.int64)
+
+    def deposit(self, x, dF):
+        t = np.asarray(x, np.float64).copy()
+        t[t >= self.lx] -= self.lx
+        t[t < 0] += self.lx
+        ix = np.minimum((t / self.hx).astype(np.int64), self.nx - 1)
+        np.add.at(self.sum, ix, (np.asarray(dF, np.float64) ** 2).sum(1))
+        np.add.at(self.count, ix, 1)
+
+    def x_avg(self):
+        """(slab centres, RMS error) as print_x_avg."""
+        xs = (np.arange(self.nx) + 0.5) * self.hx
+        rms = np.sqrt(np.divide(self.sum, self.count, out=np.zeros(self.nx),
+                                where=self.count > 0))
+        return xs, rms
+

@@ -15,7 +15,10 @@ positions and boxes as .npz files next to this script:
                   pubilsh.pre/resubmit/fig/t0.85-n16000-adapt-e0.0045/): the
                   run-averaged density along x (a.density.x.out), the rc averaged
                   over y, z (a.rcut.x.out), and the estimated error and rc of the
-                  y = z = 0 column at the last update (error.x.out: x, error, rc);
+                  y = z = 0 column at the last update (error.x.out: x, error, rc),
+                  the run-averaged estimated error along x (a.error.x.out) and
+                  the run-averaged real error, the RMS over each 2-wide x slab
+                  of the force beyond each atom's cutoff (a.real.x.out);
                   and of its force-corrected run at uniform rc 7.5
                   (…/t0.85-n16000-fcorr-rc07.5-feq0200/): that run's averaged
                   density (a.density.x.out) and force correction of the
@@ -66,6 +69,8 @@ def arc_fixture():
     for key, d, name in (("a_density", ARC_DIR, "a.density.x.out"),
                          ("a_rcut", ARC_DIR, "a.rcut.x.out"),
                          ("error_x", ARC_DIR, "error.x.out"),
+                         ("a_error", ARC_DIR, "a.error.x.out"),
+                         ("a_real", ARC_DIR, "a.real.x.out"),
                          ("fc_density", FC_DIR, "a.density.x.out"),
                          ("afc", FC_DIR, "afc.x.out")):
         src = os.path.join(d, name)

@@ -37,7 +37,10 @@ steps dropped); their tension tables hold the corrected tension, its error,
 the correction, so the simulated tension is column 2 minus column 4.  A
 quantity agrees when |ours - ref| <= 1.5 sqrt(err_ours^2 + err_ref^2) (both
 errors are 2 sigma); the correction, which the tables give without an
-error, when it is within 10 %.  Exits non-zero when one does not.
+error, when it is within 10 %.  Adaptive runs at PREC 0.0045 also sample the
+real error (the force of the pairs beyond each atom's cutoff, RMS per x
+slab, R/cal.real.error) every 20000 steps and compare it with the
+reference run's a.real.x.out.  Exits non-zero when one does not agree.
 """
 import argparse
 import json
@@ -53,8 +56,13 @@ sys.path.insert(0, os.path.join(HERE, "..", "md.error_amd"))
 
 from sre import analysis, apps, io  # noqa: E402
 from sre.arc import Arc  # noqa: E402
+from sre.engine import Engine  # noqa: E402
 
 L1, L2, G1, G2 = 70.0, 80.0, 20.0, 130.0
+RC_FAR = 10.45       # "all" pairs for the real error: just below half the 21-wide box
+REAL_EVERY = 20000   # steps between real-error frames
+# regions of the real-error profile: (name, x ranges)
+REGIONS = (("liquid", ((62.0, 86.0),)), ("vapour", ((0.0, 36.0), (114.0, 150.0))))
 # main.tex Table I: mean pairwise interactions per step (1e6), by rc and method
 PAPER_COST = {("uni", 3.5): 2.9, ("adapt", 3.5): 1.9, ("fcorr", 3.5): 3.1,
               ("uni", 5.0): 7.8, ("adapt", 5.0): 4.2, ("fcorr", 5.0): 7.9,
@@ -95,11 +103,35 @@ def main():
     ref_rc = args.rc if args.ref_rc is None else args.ref_rc
     fx = np.load(os.path.join(HERE, "golden", "slab_t085.npz"))
     box = fx["box"]
+    xyz0 = fx["xyz"]
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     thermo_path = os.path.splitext(args.out)[0] + ".thermo"
     gas, liq = [], []
     prof = Arc(box, 1.0)
     prof.set_candidates(args.rc, args.rc, 1.0)
+
+    # real error of the adaptive cutoffs: per atom, the force of the pairs
+    # beyond rc_i (rc_FAR minus rc_i forces), RMS per 2-wide x slab, as the
+    # reference's cal.real.error tool (R/cal.real.error/app/lj.xtc.adapt.cu)
+    real = analysis.ErrorProfile(box, 2.0)
+    far = []
+
+    def real_error(xyz, engine):
+        if not far:
+            n = len(xyz)
+            f = Engine(n_types=1, skin=0.02)
+            f.set_system([n], [box])
+            f.set_types(np.zeros(n, np.int32))
+            one = np.ones((1, 1), np.float32)
+            f.set_pair_params(one, one, np.zeros((1, 1), np.float32))
+            f.set_rc_table(np.full((1, 1, 1), RC_FAR, np.float32))
+            far.append(f)
+        f = far[0]
+        f.upload_positions(xyz.astype(np.float32))
+        f.rebuild()
+        f.compute()
+        engine.compute()
+        real.deposit(xyz[:, 0], f.forces() - engine.forces())
 
     def on_frame(step, t, xyz, engine):
         if step > args.equil:
@@ -107,6 +139,10 @@ def main():
             gas.append(g)
             liq.append(l)
             prof.deposit(engine)
+            # midway between cutoff updates (an update leaves the list to
+            # be rebuilt at the next step)
+            if args.adapt and step % REAL_EVERY == REAL_EVERY // 2:
+                real_error(xyz, engine)
         if step % 20000 == 0:
             print(f"step {step}: {time.time() - t0:.1f} s", flush=True)
 
@@ -152,17 +188,41 @@ def main():
            "temperature": float(rows[keep, 4].mean()),
            "pairs_per_step": float(rows[keep, -1].mean()),
            "paper_cost": 1e6 * PAPER_COST.get((kind, ref_rc), float("nan"))}
+    ok = True
     if args.adapt:
         # the run-averaged rc along x against the reference's a.rcut.x.out
         # (its run at PREC 0.0045; tests/golden/arc_t085.npz)
         res["rc_mean_x"] = [round(float(v), 4) for v in rc_mean_x]
+        xs, rms = real.x_avg()
+        res["real_error_x"] = [round(float(v), 6) for v in rms]
+        res["real_error_frames"] = int(real.count.sum()) // len(xyz0)
         if args.adapt == 0.0045:
-            a = np.load(os.path.join(HERE, "golden", "arc_t085.npz"))["a_rcut"][:, 1]
+            golden = np.load(os.path.join(HERE, "golden", "arc_t085.npz"))
+            a = golden["a_rcut"][:, 1]
             res["rc_mean_x_vs_reference"] = {
                 "mean_abs_diff": float(np.abs(rc_mean_x - a).mean()),
                 "max_abs_diff": float(np.abs(rc_mean_x - a).max()),
                 "mean_ours": float(rc_mean_x.mean()), "mean_ref": float(a.mean())}
-    ok = True
+            # real error in the bulk liquid and vapour against the reference
+            # run's a.real.x.out (same slabs), within 10 %; and everywhere at
+            # most PREC, the method's contract (the reference's peak is
+            # 0.00443 for PREC 0.0045)
+            ref_real = golden["a_real"][:, 1]
+            cmp = {}
+            for name, ranges in REGIONS:
+                m = np.zeros(len(xs), bool)
+                for lo, hi in ranges:
+                    m |= (xs > lo) & (xs < hi)
+                cmp[name] = (float(rms[m].mean()), float(ref_real[m].mean()))
+            ok_real = all(abs(o - r) <= 0.1 * r for o, r in cmp.values())
+            cmp["peak"] = (float(rms.max()), float(ref_real.max()))
+            ok_real &= cmp["peak"][0] <= args.adapt
+            res["real_error_vs_reference"] = {k: {"ours": o, "ref": r}
+                                              for k, (o, r) in cmp.items()}
+            res["real_error_vs_reference"]["agree"] = bool(ok_real)
+            for k, (o, r) in cmp.items():
+                print(f"real error {k:8s} ours {o:.5f}  ref {r:.5f}", flush=True)
+            ok &= ok_real
     for name, data in (("liquid", liq), ("gas", gas), ("tension", rows[keep, 10])):
         b = analysis.BlockAverage(data, NBLOCK)
         ref = reference_row(fx, f"{name}_{kind}", ref_rc)

@@ -48,3 +48,17 @@ def test_thermo_line_layout_and_tension():
     _, row = apps.thermo_line(200, 0.005, 16000, box, [-5.0e4, *w], [6800.0] * 3, 0.0, 0)
     assert abs(row[7] - row[8] - dP) < 1e-12
     assert abs(row[10] - dP * 75.0) < 1e-9
+
+
+def test_error_profile_bins_and_rms():
+    ep = analysis.ErrorProfile((150.0, 21.0, 21.0), 2.0)
+    assert ep.nx == 75 and ep.hx == 2.0
+    x = np.array([0.5, 1.5, 151.0, -1.0, 3.0])       # 151 wraps to 1, -1 to 149
+    dF = np.array([[3.0, 4.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0],
+                   [0.0, 2.0, 0.0], [0.0, 0.0, 0.0]])
+    ep.deposit(x, dF)
+    xs, rms = ep.x_avg()
+    assert xs[0] == 1.0 and xs[-1] == 149.0
+    assert abs(rms[0] - np.sqrt((25.0 + 1.0 + 1.0) / 3.0)) < 1e-15
+    assert rms[1] == 0.0 and rms[74] == 2.0
+    assert ep.count.sum() == 5
