@@ -354,15 +354,22 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   for (int u = 0; u < U; ++u) ja[u] = row(u);
 #pragma unroll
   for (int u = 0; u < U; ++u) jb[u] = row(U + u);
+  // issue order at the loop entry as on the back edge (rows of g+1, then
+  // the gathers of g and the rows of g+2), so that the loop waits only for
+  // the loads it is about to use
+  asm volatile("" ::: "memory");
   prefetch(pa, ja, 0);
-  for (int g = 0; g < groups; g += 2) {
+  // whole pairs of groups in the loop (no branch inside its body, so the
+  // loads of each half are waited for only where they are used), the odd
+  // last group after it
+  int g = 0;
+  for (; g + 1 < groups; g += 2) {
     prefetch(pb, jb, g + 1);
     eval_group(pa, g);
-    if (g + 1 < groups) {  // wave-uniform
-      prefetch(pa, ja, g + 2);
-      eval_group(pb, g + 1);
-    }
+    prefetch(pa, ja, g + 2);
+    eval_group(pb, g + 1);
   }
+  if (g < groups) eval_group(pa, g);  // wave-uniform
   const int iters = groups * U;
   int64_t Fx = (int64_t)ax, Fy = (int64_t)ay, Fz = (int64_t)az;
   Fx = group_sum(Fx, T);
