@@ -202,37 +202,6 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return x * q + min(x, r) + (b >> 3);
 }
 
-// Position gathers through a buffer resource over pos4 in index mode: a
-// resource with a 16-byte record stride takes the atom index itself
-// (buffer_load ... idxen), so a gather needs neither a 64-bit address nor a
-// per-pair shift (pos4 holds fewer than 2^27 atoms, checked when the system
-// is set).  gfx9 resource word 3 = 0x00020000 (32-bit data format, no
-// swizzle).  These are the LLVM structured buffer loads, which this compiler
-// exposes under their intrinsic names only.
-typedef unsigned int sre_u32x3 __attribute__((ext_vector_type(3)));
-typedef unsigned int sre_u32x4 __attribute__((ext_vector_type(4)));
-extern "C" __device__ sre_u32x3 sre_struct_load_v3(__amdgpu_buffer_rsrc_t rsrc, int vindex,
-                                                   int voffset, int soffset, int aux)
-    __asm("llvm.amdgcn.struct.ptr.buffer.load.v3i32");
-extern "C" __device__ sre_u32x4 sre_struct_load_v4(__amdgpu_buffer_rsrc_t rsrc, int vindex,
-                                                   int voffset, int soffset, int aux)
-    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc_idx(const uint4 *p, int n) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(p), (short)16, n * 16,
-                                           0x00020000);
-}
-template <bool W>
-__device__ __forceinline__ uint4 gather_pos_idx(__amdgpu_buffer_rsrc_t r, int j) {
-  if (W) {
-    const sre_u32x4 v = sre_struct_load_v4(r, j, 0, 0, 0);
-    return make_uint4(v[0], v[1], v[2], v[3]);
-  } else {
-    const sre_u32x3 v = sre_struct_load_v3(r, j, 0, 0, 0);
-    return make_uint4(v[0], v[1], v[2], 0u);
-  }
-}
-
 // Wave-wide maximum of an int (butterfly over the 64 lanes).
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256) void k_build_list(
   const int si = act ? s : n - 1;
 
   const uint4 pi = pos4[si];
+  const __amdgpu_buffer_rsrc_t prs = pos_rsrc_idx(pos4, n);
   const RepInfo &R = reps[pi.w >> 8];
   const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
   const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
@@ -172,9 +173,9 @@ __global__ __launch_bounds__(256) void k_build_list(
     // starts at sorted atom p0c (kept in registers; LDS is read only when
     // the cursor moves to the next run)
     int cur = 0, beg = 0, end = send[0], p0c = sp0[0];
-    for (int k0 = 0; k0 < rounds; ++k0) {
-      uint4 cj[U];
-      int jj[U];
+    // candidates of round k0 and their gathers (a lane past the sequence
+    // takes j = -1 and gathers its own atom)
+    auto load = [&](int k0, int *jj, uint4 *cj) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int v = l + TB * (k0 * U + u);
@@ -189,10 +190,13 @@ __global__ __launch_bounds__(256) void k_build_list(
           j = p0c + (v - beg);
         }
         jj[u] = j;
-        cj[u] = pos4[j >= 0 ? j : si];
+        cj[u] = gather_pos_idx<false>(prs, j >= 0 ? j : si);
       }
-      // accepted candidates of this round as a bit mask per lane; their list
-      // positions follow (lane, u) order: an exclusive prefix over the group
+      asm volatile("" ::: "memory");
+    };
+    // the accepted candidates of a round, placed in (lane, u) order by an
+    // exclusive prefix over the group
+    auto accept = [&](const int *jj, const uint4 *cj) {
       unsigned okm = 0u;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -219,7 +223,22 @@ __global__ __launch_bounds__(256) void k_build_list(
         }
       }
       count += __shfl(incl, TB - 1, TB);
+    };
+    // software pipeline: the gathers of round k0 + 1 are in flight while
+    // round k0 is tested; whole pairs of rounds in the loop (no branch in its
+    // body, so each gather is waited for only where it is used), the odd
+    // last round after it (every branch is wave-uniform)
+    int ja[U], jb[U];
+    uint4 ca[U], cb[U];
+    load(0, ja, ca);
+    int k0 = 0;
+    for (; k0 + 1 < rounds; k0 += 2) {
+      load(k0 + 1, jb, cb);
+      accept(ja, ca);
+      load(k0 + 2, ja, ca);
+      accept(jb, cb);
     }
+    if (k0 < rounds) accept(ja, ca);
     __builtin_amdgcn_wave_barrier();  // LDS reuse by the next chunk
   }
   if (act && l == 0) nnb[s] = count;
