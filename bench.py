@@ -62,25 +62,41 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
         tune_s = time.perf_counter() - t0
     e.set_perturbation(PERTURB_SIGMA, SEED, 0, anchored=True)
 
+    def check_skin(disp):
+        # every atom must have stayed within skin/2 of its position at the
+        # rebuild before its segment
+        if disp > 0.5 * w.skin:
+            raise RuntimeError(f"skin exceeded between rebuilds ({disp:.4f})")
+
     def segment_loop(n_steps):
+        """Segments of `every` steps, each ended by a rebuild; between two
+        statistics read-backs (every THERMO_EVERY steps) the segments and
+        their rebuilds are queued with one synchronisation (sre_run_segments,
+        which checks every rebuild's list capacity and displacement at the
+        end), the last segment is followed by the read-back and a rebuild."""
         done = 0
         pairs = 0
         rebuilds = 0
         nl, _, _ = e.neighbor_count(in_rc=False)   # known since the last rebuild
         while done < n_steps:
-            n = min(every, n_steps - done)
+            block = min(THERMO_EVERY, n_steps - done)
+            nseg = max(1, -(-block // every))
+            if nseg > 1:
+                nls, disp = e.run_segments(nseg - 1, every, stats=True, tail=True,
+                                           perturb=True)
+                check_skin(disp)
+                for k in range(nseg - 1):
+                    pairs += nl * every
+                    nl = int(nls[k])
+                rebuilds += nseg - 1
+            n = block - (nseg - 1) * every
             e.graph_run(n, stats=True, tail=True, perturb=True)
             pairs += nl * n
-            done += n
-            if done // THERMO_EVERY > (done - n) // THERMO_EVERY:
-                e.stats()                     # reduce + read back; raises on a force overflow
-            e.rebuild()                       # one synchronisation per rebuild
+            done += block
+            e.stats()                     # reduce + read back; raises on a force overflow
+            e.rebuild()
             rebuilds += 1
-            # the skin test recorded during the segment: every atom must have
-            # stayed within skin/2 of its position at the previous rebuild
-            disp = e.last_rebuild_displacement()
-            if disp > 0.5 * w.skin:
-                raise RuntimeError(f"skin exceeded between rebuilds ({disp:.4f})")
+            check_skin(e.last_rebuild_displacement())
             nl, _, _ = e.neighbor_count(in_rc=False)
         return pairs, rebuilds
 

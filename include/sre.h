@@ -244,6 +244,18 @@ sre_status sre_autotune(sre_engine *e, unsigned flags, const char *cache_path);
  * sre_set_perturbation, advancing its step counter on the device. */
 #define SRE_PERTURB 4u
 sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
+/* n_segments times: sre_graph_run(e, flags, steps) then a list rebuild, all
+ * queued without a synchronisation in between (the rebuild of the reference
+ * driver's loop, H/app/lj.nhc-xtc.cu:169-181, at a fixed period).  The checks
+ * sre_rebuild makes after each rebuild are made once at the end: n_list[g]
+ * (optional) receives the list length built at the end of segment g,
+ * max_disp (optional) the largest displacement of any segment from its
+ * rebuild positions (compare with skin/2).  If a rebuild's longest list
+ * exceeded the capacity, the lists from it on were cut: the capacity is
+ * regrown and the lists rebuilt, and SRE_ERR_LIST_OVERFLOW reports those
+ * segments as invalid. */
+sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segments, int steps,
+                            int64_t *n_list, float *max_disp);
 /* Perturbation of the bound positions applied by SRE_PERTURB replays, one
  * Gaussian draw (std sigma per component, counter-based on seed and step) per
  * replay.  anchored = 0: a random walk, x += sigma N(0,1).  anchored = 1:

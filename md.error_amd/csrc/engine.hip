@@ -133,6 +133,11 @@ struct sre_engine {
   // [0] longest list, [1] total length, [2] low word: max displacement^2
   unsigned long long *d_counters = nullptr;
   uint64_t *h_info = nullptr;               // pinned readback (4 words)
+  // sre_run_segments: per-segment rebuild info (3 words each), device and
+  // pinned host copies
+  unsigned long long *d_seg_info = nullptr;
+  uint64_t *h_seg_info = nullptr;
+  int seg_cap = 0;
   float last_disp2 = 0.f;                   // max displacement^2 before the last rebuild
   int max_count = 0;
   int64_t n_list = 0;
@@ -220,6 +225,8 @@ sre_engine::~sre_engine() {
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
+  if (h_seg_info) (void)hipHostFree(h_seg_info);
+  dfree(d_seg_info);
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
@@ -1706,6 +1713,89 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
     e->forces_valid = true;
     e->stats_valid = e->atom_stats_valid = (flags & SRE_STATS) != 0;
     e->stats_pending = e->stats_valid;
+  }
+  return SRE_OK;
+}
+
+// Segments of `steps` graph steps each followed by a list rebuild, queued
+// back to back with one synchronisation at the end: each rebuild's longest
+// list, total length and the largest displacement of the segment before it
+// are copied on the device and checked once all segments have run.
+extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segments,
+                                       int steps, int64_t *n_list, float *max_disp) {
+  if (!e || n_segments < 0 || steps < 1)
+    return fail(SRE_ERR_INVALID, "sre_run_segments: bad arguments");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "segments before the first rebuild");
+  if (n_segments == 0) {
+    if (max_disp) *max_disp = 0.f;
+    return SRE_OK;
+  }
+  HIP_TRY(hipSetDevice(e->device));
+  // the captured rebuild must match the current layout; otherwise one
+  // ordinary rebuild captures it again
+  const int n_parts = build_part_count(e->tune.build_mode, e->n_atoms,
+                                       e->tune.build_lanes, e->n_cols);
+  if (!e->rebuild_exec || rebuild_signature(e, n_parts) != e->rebuild_sig) {
+    sre_status s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+  }
+  if (e->seg_cap < n_segments) {
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    dfree(e->d_seg_info);
+    if (e->h_seg_info) (void)hipHostFree(e->h_seg_info);
+    e->h_seg_info = nullptr;
+    e->seg_cap = 0;
+    HIP_TRY(dalloc(&e->d_seg_info, (size_t)3 * n_segments));
+    HIP_TRY(hipHostMalloc((void **)&e->h_seg_info, sizeof(uint64_t) * 3 * n_segments,
+                          hipHostMallocDefault));
+    e->seg_cap = n_segments;
+  }
+  for (int g = 0; g < n_segments; ++g) {
+    sre_status s = sre_graph_run(e, flags, steps);
+    if (s != SRE_OK) return s;
+    HIP_TRY(hipGraphLaunch(e->rebuild_exec, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_seg_info + 3 * (size_t)g, e->d_counters,
+                           3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                           e->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(e->h_seg_info, e->d_seg_info,
+                         sizeof(uint64_t) * 3 * n_segments, hipMemcpyDeviceToHost,
+                         e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->built = true;
+  e->md_since_rebuild = 0;
+  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  float d2max = 0.f;
+  int worst = -1;
+  for (int g = 0; g < n_segments; ++g) {
+    const uint64_t *h = e->h_seg_info + 3 * (size_t)g;
+    const unsigned bits = (unsigned)(h[2] & 0xffffffffu);
+    float d2;
+    memcpy(&d2, &bits, sizeof d2);
+    d2max = std::max(d2max, d2);
+    if (n_list) n_list[g] = (int64_t)h[1];
+    if (worst < 0 && (int)h[0] > e->cap) worst = g;
+  }
+  const uint64_t *last = e->h_seg_info + 3 * (size_t)(n_segments - 1);
+  e->max_count = (int)last[0];
+  e->n_list = (int64_t)last[1];
+  e->last_disp2 = d2max;
+  if (max_disp) *max_disp = sqrtf(d2max);
+  if (worst >= 0) {
+    // lists were cut at the capacity from rebuild `worst` on: regrow now so
+    // that the next call is exact, and report the segments as invalid
+    int mc = 0;
+    for (int g = worst; g < n_segments; ++g)
+      mc = std::max(mc, (int)e->h_seg_info[3 * (size_t)g]);
+    sre_status s = ensure_list(
+        e, round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes));
+    if (s != SRE_OK) return s;
+    s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+    return fail(SRE_ERR_LIST_OVERFLOW,
+                "neighbor list overflow at the rebuild ending segment %d (%d > %d); "
+                "the capacity has been regrown, the later segments are invalid",
+                worst, mc, e->cap);
   }
   return SRE_OK;
 }

@@ -241,7 +241,10 @@ __global__ __launch_bounds__(256) void k_build_list(
     if (k0 < rounds) accept(ja, ca);
     __builtin_amdgcn_wave_barrier();  // LDS reuse by the next chunk
   }
-  if (act && l == 0) nnb[s] = count;
+  // an overflowing list keeps its first cap entries (the partials below carry
+  // the full count, from which the host regrows), so no reader of the list
+  // ever runs past its rows
+  if (act && l == 0) nnb[s] = min(count, cap);
   // this wave's longest list and total list length
   int m = act ? count : 0;
   int t = (act && l == 0) ? count : 0;
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(kWave) void k_build_tiles(
       }
     }
   }
-  if (act) nnb[s] = count;
+  if (act) nnb[s] = min(count, cap);   // as in k_build_list
   int mx = count, tot = count;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) {

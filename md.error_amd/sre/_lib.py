@@ -93,6 +93,7 @@ SIGNATURES = {
     "sre_set_tuning": (_st, [_vp, C.POINTER(Tuning)]),
     "sre_autotune": (_st, [_vp, C.c_uint, C.c_char_p]),
     "sre_graph_run": (_st, [_vp, C.c_uint, C.c_int]),
+    "sre_run_segments": (_st, [_vp, C.c_uint, C.c_int, C.c_int, _i64p, _f32p]),
     "sre_set_perturbation": (_st, [_vp, C.c_float, C.c_uint64, C.c_uint64, C.c_int]),
     "sre_check_skin_flag": (_st, [_vp]),
     "sre_last_rebuild_displacement": (_st, [_vp, _f32p]),

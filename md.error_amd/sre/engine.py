@@ -305,6 +305,18 @@ class Engine:
         f = _flags(stats, tail, fcorr) | (_lib.PERTURB if perturb else 0)
         check(lib().sre_graph_run(self._h, f, int(n_steps)))
 
+    def run_segments(self, n_segments, steps, stats=False, tail=False, perturb=False,
+                     fcorr=False):
+        """n_segments x (graph_run(steps) + rebuild) with one synchronisation;
+        returns (list length built after each segment, largest displacement
+        of any segment)."""
+        nl = np.zeros(max(int(n_segments), 1), np.int64)
+        d = C.c_float()
+        fl = _flags(stats, tail, fcorr) | (_lib.PERTURB if perturb else 0)
+        check(lib().sre_run_segments(self._h, fl, int(n_segments), int(steps),
+                                     _ptr(nl, C.c_int64), C.byref(d)))
+        return nl[:int(n_segments)], d.value
+
     def set_perturbation(self, sigma, seed, first_step=0, anchored=False):
         """Perturbation applied by graph_run(perturb=True): a random walk
         (x += sigma N(0,1) per replay) or, anchored, vibration about the

@@ -313,3 +313,36 @@ def test_overlapping_atoms_raise_force_range(torch_cuda):
     e.compute(stats=True)
     with pytest.raises(ForceRange):
         e.stats()
+
+
+def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab):
+    """sre_run_segments (segments and rebuilds queued, checks at the end) does
+    what graph_run + rebuild per segment does: the same positions, lists and
+    forces bit for bit, the same list lengths and largest displacement."""
+    import torch
+    res = []
+    for queued in (False, True):
+        e = Engine(skin=slab.skin)
+        slab.setup(e, tail=True)
+        pos = torch.from_numpy(slab.xyz.copy()).cuda()
+        torch.cuda.synchronize()
+        e.bind_positions(pos)
+        e.rebuild()
+        e.set_perturbation(0.01, 21, 0, anchored=True)
+        if queued:
+            nls, disp = e.run_segments(3, 5, stats=True, tail=True, perturb=True)
+        else:
+            nls, disp = [], 0.0
+            for _ in range(3):
+                e.graph_run(5, stats=True, tail=True, perturb=True)
+                e.rebuild()
+                nls.append(e.neighbor_count(in_rc=False)[0])
+                disp = max(disp, e.last_rebuild_displacement())
+        e.graph_run(5, stats=True, tail=True, perturb=True)
+        res.append((list(nls), disp, e.forces_fixed(), e.export_pairs(),
+                    pos.cpu().numpy()))
+        e.close()
+    (n0, d0, f0, p0, x0), (n1, d1, f1, p1, x1) = res
+    assert n0 == n1 and d0 == d1 and 0.0 < d1 <= 0.5 * slab.skin
+    assert np.array_equal(x0, x1) and np.array_equal(f0, f1)
+    assert np.array_equal(p0[0], p1[0]) and np.array_equal(p0[1], p1[1])
