@@ -227,7 +227,9 @@ typedef struct {
   int build_mode;       /* 0: per-atom column search (build_lanes lanes per atom);
                            1: one wavefront per 64-atom tile of a cell column */
   int xcd_remap;        /* 1: map contiguous atom ranges to each XCD (L2 locality) */
-  int force_unroll;     /* list rows per group in the mixed force kernel: 1, 2, 4 */
+  int force_unroll;     /* list rows per group in the mixed force kernel: 1, 2, 4;
+                           FP64 form: 1 = the mixed kernel's pipelined loop,
+                           2 or 4 = the plain per-lane loop (same sums) */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */

@@ -478,6 +478,134 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   }
 }
 
+// The FP64 form in the mixed kernel's structure: the same wave-uniform,
+// software-pipelined loop (list rows two groups ahead through a per-wave
+// resource, indexed gathers of the next group's lattice positions and
+// sub-lattice residuals, one list row per group), the same pair set (the
+// FP32 predicate) and FP64 arithmetic and sums as k_force_fp64.  Selected
+// for the FP64 form by force_unroll = 1; measured beside the plain form.
+template <int T, bool STATS, bool TAIL, bool MULTI>
+__global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
+  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
+  __shared__ PairParamD sp[MULTI ? kMaxPairTypes : 1];
+  if (MULTI) {
+    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
+      sp[t] = a.paramsd[t];
+    __syncthreads();
+  }
+  constexpr int A = kWave / T;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l = lane % T;
+  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
+  const int s = wave * A + lane / T;
+  const bool act = s < a.n;
+  const int s0 = wave * A;
+  if (s0 >= a.n) return;
+  const int si = act ? s : s0;
+  const uint4 pi = a.pos4[si];
+  const RepInfo &R = a.reps[pi.w >> 8];
+  const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  const double qdx = R.qd[0], qdy = R.qd[1], qdz = R.qd[2];
+  const float4 ri = a.res4[si];
+  const float rc = a.rcut[si];
+  const float rc2 = rc * rc;
+  const int nn = act ? a.nnb[si] : 0;
+  const int lim = nn - l;
+  const int ti = (int)(pi.w & 0xffu) * a.n_types;
+  const int groups = __builtin_amdgcn_readfirstlane(wave_max_int((nn + T - 1) / T));
+  PairParamD P = a.paramsd[0];
+  const size_t tile0 = (size_t)__builtin_amdgcn_readfirstlane(wave) * (size_t)a.cap * A;
+  const size_t lrem = a.list_len > tile0 ? a.list_len - tile0 : 0;
+  const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<int *>(a.list + tile0), (short)0,
+      (int)min(lrem * 4, (size_t)0x7fffffff), 0x00020000);
+  const int loff = lane * 4;
+  auto row = [&](int r) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(lrs, loff, r * (kWave * 4), 0);
+  };
+  const __amdgpu_buffer_rsrc_t prs = pos_rsrc_idx(a.pos4, a.n);
+  const __amdgpu_buffer_rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float4 *>(a.res4), (short)16, a.n * 16, 0x00020000);
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  struct Pj { uint4 p; float3 r; };
+  auto fetch = [&](Pj &q, int &jr, int g) {
+    const int j = g * T < lim ? jr : si;
+    q.p = gather_pos_idx<MULTI>(prs, j);
+    const sre_u32x3 v = sre_struct_load_v3(rrs, j, 0, 0, 0);
+    q.r = make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+    jr = row(g + 2);
+    asm volatile("" ::: "memory");
+  };
+  auto eval = [&](const Pj &q, int g) {
+    const bool v = g * T < lim;
+    const uint4 pj = q.p;
+    const float d2f = dist2(disp(pj.x, pi.x, qx), disp(pj.y, pi.y, qy),
+                            disp(pj.z, pi.z, qz));
+    const bool in = v && (d2f < rc2);
+    if (MULTI) P = sp[ti + (int)(pj.w & 0xffu)];
+    const double dx = (double)(int32_t)(pj.x - pi.x) * qdx + ((double)q.r.x - (double)ri.x);
+    const double dy = (double)(int32_t)(pj.y - pi.y) * qdy + ((double)q.r.y - (double)ri.y);
+    const double dz = (double)(int32_t)(pj.z - pi.z) * qdz + ((double)q.r.z - (double)ri.z);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double ri2 = in ? 1.0 / d2 : 0.0;
+    const double ri6 = ri2 * ri2 * ri2;
+    const double fs = ri2 * ri6 * (P.B6 - P.A12 * ri6);
+    fx += fs * dx;
+    fy += fs * dy;
+    fz += fs * dz;
+    if (STATS) {
+      e += in ? ri6 * (P.A * ri6 - P.B) + P.shift : 0.0;
+      wx += fs * dx * dx;
+      wy += fs * dy * dy;
+      wz += fs * dz * dz;
+    }
+  };
+  int ja = row(0), jb = row(1);
+  asm volatile("" ::: "memory");
+  Pj qa, qb;
+  fetch(qa, ja, 0);
+  int g = 0;
+  for (; g + 1 < groups; g += 2) {
+    fetch(qb, jb, g + 1);
+    eval(qa, g);
+    fetch(qa, ja, g + 2);
+    eval(qb, g + 1);
+  }
+  if (g < groups) eval(qa, g);
+  fx = group_sum(fx, T);
+  fy = group_sum(fy, T);
+  fz = group_sum(fz, T);
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
+  if (STATS) {
+    E = group_sum(e, T) * 0.5;
+    Wx = group_sum(wx, T) * 0.5;
+    Wy = group_sum(wy, T) * 0.5;
+    Wz = group_sum(wz, T) * 0.5;
+  }
+  if (act && l == 0) {
+    if (TAIL) {
+      double2 tv = make_double2(0.0, 0.0);
+      if (a.tail_atom) {
+        tv = a.tail_atom[s];
+      } else {
+        const int tix = a.tail_idx[s];
+        if (tix >= 0) tv = a.tail[tix];
+      }
+      fz += tv.x;
+      E += tv.y;
+    }
+    a.fxd[s] = fx;
+    a.fyd[s] = fy;
+    a.fzd[s] = fz;
+    if (STATS) {
+      double2 *d = reinterpret_cast<double2 *>(a.atom_stats + (size_t)s * 4);
+      d[0] = make_double2(E, Wx);
+      d[1] = make_double2(Wy, Wz);
+    }
+  }
+}
+
 int force_num_waves(int n, int lanes) {
   const int A = kWave / lanes;
   return (n + A - 1) / A;
@@ -514,6 +642,13 @@ static void launch_force_t(int precision, int unroll, int block,
     if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
     else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);
     else launch_mixed_u<T, 4, STATS, TAIL>(grid, block, multi, a, st);
+  } else if (unroll == 1) {   // the FP64 form in the pipelined structure
+    if (multi)
+      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, true>), dim3(grid),
+                         dim3(block), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, false>), dim3(grid),
+                         dim3(block), 0, st, a);
   } else {
     if (multi)
       hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, true>), dim3(grid),

@@ -126,3 +126,15 @@ def test_forces_bitwise_independent_of_tuning(torch_cuda, slab16):
             base = f
         else:
             assert np.array_equal(f, base), (fl, bl, cxy, cz, blk, bm, xr, fu)
+
+
+def test_fp64_pipelined_equals_plain(torch_cuda, slab16):
+    """The FP64 form in the mixed kernel's pipelined structure (force_unroll 1)
+    sums each lane's pairs in the same order as the plain FP64 kernel, so
+    forces and statistics agree bit for bit."""
+    out = []
+    for fu in (4, 1):
+        e = engine_for(slab16, precision="fp64", force_unroll=fu)
+        out.append((e.forces(), e.stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
