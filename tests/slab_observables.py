@@ -90,6 +90,8 @@ def main():
     # the reference's adaptive run chose rc 3.25, 4.75, 5.25 (error.x.out of
     # fig/t0.85-n16000-adapt-e0.0045): candidates 0.25 apart
     ap.add_argument("--rcstep", type=float, default=0.25)
+    ap.add_argument("--update-rc", type=int, default=20000,
+                    help="steps between cutoff updates of the adaptive driver")
     # reassign cutoffs every N steps without a list rebuild (the reference's
     # rcutAssignFeq) instead of at every rebuild
     ap.add_argument("--assign-feq", type=int, default=None)
@@ -130,6 +132,7 @@ def main():
         f.upload_positions(xyz.astype(np.float32))
         f.rebuild()
         f.compute()
+        engine.rebuild()    # a cutoff update may have just replaced the lists
         engine.compute()
         real.deposit(xyz[:, 0], f.forces() - engine.forces())
 
@@ -139,8 +142,6 @@ def main():
             gas.append(g)
             liq.append(l)
             prof.deposit(engine)
-            # midway between cutoff updates (an update leaves the list to
-            # be rebuilt at the next step)
             if args.adapt and step % REAL_EVERY == REAL_EVERY // 2:
                 real_error(xyz, engine)
         if step % 20000 == 0:
@@ -168,7 +169,8 @@ def main():
 
                 rows, rc_last, _ = apps.lj_nhc_adapt(
                     conf, nstep, rcut=args.rc, ref_T=0.85, tau_T=1.0, rcmin=3.0, rcmax=10.0,
-                    rcstep=args.rcstep, target_prec=args.adapt, update_rc_feq=20000,
+                    rcstep=args.rcstep, target_prec=args.adapt,
+                    update_rc_feq=args.update_rc,
                     n_refine=2,
                     out=out, on_frame=on_frame, conf_feq=FRAME_EVERY, confstart=None,
                     confout=None, on_update=on_update, assign_feq=args.assign_feq)
@@ -183,6 +185,7 @@ def main():
     res = {"kind": kind, "rc": args.rc, "prec": args.adapt, "ref_rc": ref_rc,
            "rcstep": args.rcstep if args.adapt else None,
            "assign_feq": args.assign_feq,
+           "update_rc": args.update_rc if args.adapt else None,
            "equil": args.equil, "steps": args.steps,
            "wall_s": wall, "ms_per_step": 1e3 * wall / nstep,
            "temperature": float(rows[keep, 4].mean()),
