@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // are summed in FP64, exact while the lane's sum of magnitudes stays below
   // 2^53 (checked at the end)
   double ax = 0.0, ay = 0.0, az = 0.0;
-  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsmax = 0.f;
+  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsabs = 0.f;
   // list rows through a resource based at this wave's tile: the lane's byte
   // offset is fixed and the row offset is a scalar, so a row load needs no
   // per-lane address arithmetic; the resource ends with the list allocation
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       const float ri2 = in ? __builtin_amdgcn_rcpf(d2) : 0.f;
       const float ri6 = ri2 * ri2 * ri2;
       const float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
-      fsmax = fmaxf(fsmax, fabsf(fs));
+      fsabs += fabsf(fs);
       const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
       ax += (double)__builtin_rintf(fpx);
       ay += (double)__builtin_rintf(fpy);
@@ -339,14 +339,15 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
     eval_group(pb, g + 1);
   }
   if (g < groups) eval_group(pa, g);  // wave-uniform
-  const int iters = groups * U;
   int64_t Fx = (int64_t)ax, Fy = (int64_t)ay, Fz = (int64_t)az;
   Fx = group_sum(Fx, T);
   Fy = group_sum(Fy, T);
   Fz = group_sum(Fz, T);
-  // every lane's sum of |pair component| * 2^32 (at most iters terms of at
-  // most fsmax * rc each) must stay below 2^53 for the FP64 sums to be exact
-  if (__ballot(act && fsmax * rc * (float)max(iters, 4) >= 9007199254740992.0f) &&
+  // every lane's sum of |pair component| * 2^32 is at most rc * sum |fs|
+  // (each component is fs * d with |d| < rc), and must stay below 2^53 for
+  // the FP64 sums of integers to be exact; 2^52 leaves room for the FP32
+  // rounding of sum |fs| and of the per-term roundings to integers
+  if (__ballot(act && fsabs * rc >= 4503599627370496.0f) &&
       lane == 0)
     atomicOr(a.err_flag, 1u);
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
