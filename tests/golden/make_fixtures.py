@@ -29,7 +29,8 @@ positions and boxes as .npz files next to this script:
                   run-averaged observables of that system
                   (…/fig/converge/t0.85.{liquid,gas,tension}.{uni,adapt,fcorr}.out:
                   rc, value, error, large-rc reference value, its error; the
-                  tension tables carry the tail correction in column 4)
+                  tension tables carry the tail correction in column 4), and
+                  the same tables at T* = 0.70, 1.10 and 1.20 (keys t<T>_...)
 GROMACS .gro layout: title line, atom count, one fixed-column line per atom
 (positions in columns 21-44, 8 characters each), box line.
 """
@@ -108,6 +109,9 @@ def slab_fixture():
             f = FIG + f"converge/t0.85.{obs}.{kind}.out"
             # commented lines are runs left out of the published figure
             tables[f"{obs}_{kind}"] = np.loadtxt(f, comments="#", ndmin=2)
+            for temp in ("0.70", "1.10", "1.20"):
+                f = FIG + f"converge/t{temp}.{obs}.{kind}.out"
+                tables[f"t{temp}_{obs}_{kind}"] = np.loadtxt(f, comments="#", ndmin=2)
     np.savez_compressed(os.path.join(HERE, "slab_t085.npz"), xyz=xyz, vel=vel, box=box,
                         **tables)
     print("slab_t085.npz", xyz.shape, box, {k: v.shape for k, v in tables.items()})

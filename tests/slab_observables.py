@@ -73,8 +73,8 @@ FRAME_EVERY = 1000
 NBLOCK = 16
 
 
-def reference_row(fx, name, rc):
-    t = fx[name]
+def reference_row(fx, name, rc, temp="0.85"):
+    t = fx[name if temp == "0.85" else f"t{temp}_{name}"]
     hit = np.nonzero(np.isclose(t[:, 0], rc))[0]
     if hit.size == 0:
         raise SystemExit(f"no reference row for rc {rc} in {name}")
@@ -84,6 +84,9 @@ def reference_row(fx, name, rc):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rc", type=float, default=5.0)
+    # another temperature of the reference's tables (the run starts from the
+    # T* = 0.85 slab, so --equil must cover the change of coexistence)
+    ap.add_argument("--temp", default="0.85", choices=("0.70", "0.85", "1.10", "1.20"))
     ap.add_argument("--fcorr", action="store_true")
     ap.add_argument("--update-fc", type=int, default=20000)
     ap.add_argument("--adapt", type=float, default=None, metavar="PREC")
@@ -152,7 +155,7 @@ def main():
         io.write_gro(conf, fx["xyz"], box, fx["vel"], resname="lj", atomname="lj")
         t0 = time.time()
         kind = "adapt" if args.adapt else "fcorr" if args.fcorr else "uni"
-        print(f"slab T*=0.85 {kind} rc {args.rc} prec {args.adapt}: "
+        print(f"slab T*={args.temp} {kind} rc {args.rc} prec {args.adapt}: "
               f"{args.equil} + {args.steps} steps", flush=True)
         nstep = args.equil + args.steps
         with open(thermo_path, "w") as out:
@@ -168,7 +171,7 @@ def main():
                         rc_sum[1] += 1
 
                 rows, rc_last, _ = apps.lj_nhc_adapt(
-                    conf, nstep, rcut=args.rc, ref_T=0.85, tau_T=1.0, rcmin=3.0, rcmax=10.0,
+                    conf, nstep, rcut=args.rc, ref_T=float(args.temp), tau_T=1.0, rcmin=3.0, rcmax=10.0,
                     rcstep=args.rcstep, target_prec=args.adapt,
                     update_rc_feq=args.update_rc,
                     n_refine=2,
@@ -176,13 +179,14 @@ def main():
                     confout=None, on_update=on_update, assign_feq=args.assign_feq)
                 rc_mean_x = rc_sum[0] / rc_sum[1]
             else:
-                rows = apps.lj_nhc(conf, nstep, rcut=args.rc, ref_T=0.85, out=out,
+                rows = apps.lj_nhc(conf, nstep, rcut=args.rc, ref_T=float(args.temp), out=out,
                                    on_frame=on_frame, conf_feq=FRAME_EVERY, confstart=None,
                                    confout=None, fcorr=args.fcorr,
                                    update_fc_feq=args.update_fc)
         wall = time.time() - t0
     keep = rows[:, 0] > args.equil
-    res = {"kind": kind, "rc": args.rc, "prec": args.adapt, "ref_rc": ref_rc,
+    res = {"kind": kind, "temp": args.temp, "rc": args.rc, "prec": args.adapt,
+           "ref_rc": ref_rc,
            "rcstep": args.rcstep if args.adapt else None,
            "assign_feq": args.assign_feq,
            "update_rc": args.update_rc if args.adapt else None,
@@ -199,7 +203,7 @@ def main():
         xs, rms = real.x_avg()
         res["real_error_x"] = [round(float(v), 6) for v in rms]
         res["real_error_frames"] = int(real.count.sum()) // len(xyz0)
-        if args.adapt == 0.0045:
+        if args.adapt == 0.0045 and args.temp == "0.85":
             golden = np.load(os.path.join(HERE, "golden", "arc_t085.npz"))
             a = golden["a_rcut"][:, 1]
             res["rc_mean_x_vs_reference"] = {
@@ -228,7 +232,7 @@ def main():
             ok &= ok_real
     for name, data in (("liquid", liq), ("gas", gas), ("tension", rows[keep, 10])):
         b = analysis.BlockAverage(data, NBLOCK)
-        ref = reference_row(fx, f"{name}_{kind}", ref_rc)
+        ref = reference_row(fx, f"{name}_{kind}", ref_rc, args.temp)
         val = ref[1] - ref[3] if name == "tension" else ref[1]
         err = 2.0 * b.error
         bound = 1.5 * np.hypot(err, ref[2])
@@ -248,7 +252,7 @@ def main():
         prof.cal_rcut(1.0)
         pc = prof.pressure()
         corr = analysis.tension(pc, box[0])
-    ref = reference_row(fx, f"tension_{kind}", ref_rc)
+    ref = reference_row(fx, f"tension_{kind}", ref_rc, args.temp)
     agree = bool(abs(corr - ref[3]) <= 0.1 * abs(ref[3]))
     ok &= agree
     res["tension_correction"] = {"ours": float(corr), "ref": float(ref[3]),
