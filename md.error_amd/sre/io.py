@@ -15,10 +15,44 @@ tools/simulation.hetero.cutoff), written and read byte for byte alike:
               time (float32) + density (float64 x nx ny nz).
   density.save DensityProfile_PiecewiseConst::save / load (:225-258): text
               "%f %f %f  %d %d %d\\n" then "%e " per cell.
+  rcut.save   AdaptRCut::save_rc (H/src/AdaptRCut.cu:697-726): text
+              "%d " (candidate count), "%f " per candidate, "\\n%f %f %f\\n"
+              (box), "%d %d %d\\n" (grid), "%d " per cell (candidate index,
+              cells in the order iz + nz (iy + ny ix)), "\\n".
 Native byte order, as the reference's fwrite.  (xtc, compressed by
 libxdrfile, is not provided.)
 """
 import numpy as np
+
+
+def write_rc_save(path, rcs, box, dims, idx):
+    """AdaptRCut::save_rc: candidates, box, grid and per-cell candidate index."""
+    idx = np.asarray(idx).ravel()
+    dims = tuple(int(v) for v in dims)
+    if idx.size != int(np.prod(dims)):
+        raise ValueError("index count does not match the grid")
+    parts = ["%d " % len(rcs), "".join("%f " % float(r) for r in rcs),
+             "\n%f %f %f\n" % tuple(float(v) for v in box[:3]),
+             "%d %d %d\n" % dims,
+             " ".join(str(int(v)) for v in idx), " \n" if idx.size else "\n"]
+    with open(path, "w") as f:
+        f.write("".join(parts))
+
+
+def read_rc_save(path):
+    """Inverse of write_rc_save -> (rcs float64, box float64[3], dims, idx int32[nx, ny, nz])."""
+    with open(path) as f:
+        tok = f.read().split()
+    nrc = int(tok[0])
+    rcs = np.array([float(v) for v in tok[1:1 + nrc]])
+    box = np.array([float(v) for v in tok[1 + nrc:4 + nrc]])
+    dims = tuple(int(v) for v in tok[4 + nrc:7 + nrc])
+    n = int(np.prod(dims))
+    body = tok[7 + nrc:]
+    if len(body) != n:
+        raise ValueError(f"{path}: {len(body)} cell indices for a {dims} grid")
+    idx = np.array(body, dtype=np.int64).astype(np.int32).reshape(dims)
+    return rcs, box, dims, idx
 
 
 def read_gro(path, velocities=False):

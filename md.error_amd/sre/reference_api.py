@@ -285,22 +285,14 @@ class AdaptRCut:
 
     def save_rc(self, path):
         """Text state of H/src/AdaptRCut.cu:698-726: candidates, box, grid, indices."""
-        b = self.dp.arc.box
-        with open(path, "w") as f:
-            f.write("%d " % len(self.rcList) + "".join("%f " % r for r in self.rcList))
-            f.write("\n%f %f %f\n" % tuple(b))
-            f.write("%d %d %d\n" % self.dp.arc.dims)
-            f.write("".join("%d " % v for v in self.rcutIndex.ravel()) + "\n")
+        _io.write_rc_save(path, self.rcList, self.dp.arc.box, self.dp.arc.dims,
+                         self.rcutIndex)
 
     def load_rc(self, path):
         """Reads save_rc's text back; the candidates must be equally spaced."""
-        with open(path) as f:
-            tok = f.read().split()
-        nrc = int(tok[0])
-        rcs = np.array([float(v) for v in tok[1:1 + nrc]])
-        box = [float(v) for v in tok[1 + nrc:4 + nrc]]
-        dims = tuple(int(v) for v in tok[4 + nrc:7 + nrc])
-        idx = np.array([int(v) for v in tok[7 + nrc:]], np.int32).reshape(dims)
+        rcs, box, dims, idx = _io.read_rc_save(path)
+        nrc = len(rcs)
+        box = [float(v) for v in box]
         if self.dp is None or tuple(self.dp.arc.dims) != dims:
             raise ValueError(f"{path}: grid {dims} differs from the density grid")
         step = rcs[1] - rcs[0] if nrc > 1 else 1.0

@@ -117,9 +117,46 @@ def slab_fixture():
     print("slab_t085.npz", xyz.shape, box, {k: v.shape for k, v in tables.items()})
 
 
+RCSAVE = ("/root/reference/shortRange.inhomo/alltext/writeDown/fig/"
+          "adaptColl-x0.60-u2.20-rc5.0-63/plots/rcut.save")
+
+
+def formats_fixture():
+    """formats.npz: what pins the file writers byte for byte without shipping
+    the files: the parsed content of the reference's rcut.save (AdaptRCut::
+    save_rc output of its cluster-collision run, a 128 x 64 x 128 grid) and
+    the header of the T* = 0.85 slab's confout.gro (its atoms are in
+    slab_t085.npz), each with the SHA-256 and size of the original bytes."""
+    import hashlib
+    out = {}
+    if os.path.exists(RCSAVE):
+        raw = open(RCSAVE, "rb").read()
+        tok = raw.decode().split()
+        nrc = int(tok[0])
+        dims = [int(v) for v in tok[4 + nrc:7 + nrc]]
+        out.update(rc_rcs=np.array([float(v) for v in tok[1:1 + nrc]]),
+                   rc_box=np.array([float(v) for v in tok[1 + nrc:4 + nrc]]),
+                   rc_dims=np.array(dims, np.int32),
+                   rc_idx=np.array(tok[7 + nrc:], dtype=np.int64).astype(np.int8),
+                   rc_sha256=hashlib.sha256(raw).hexdigest(), rc_bytes=len(raw))
+    else:
+        print(f"missing {RCSAVE}", file=sys.stderr)
+    src = FIG + "t0.85-n16000-rc07.5uni/confout.gro"
+    if os.path.exists(src):
+        raw = open(src, "rb").read()
+        lines = raw.decode().splitlines()
+        first = lines[2]
+        out.update(gro_title=lines[0], gro_resname=first[5:10].strip(),
+                   gro_atomname=first[10:15].strip(),
+                   gro_sha256=hashlib.sha256(raw).hexdigest(), gro_bytes=len(raw))
+    np.savez_compressed(os.path.join(HERE, "formats.npz"), **out)
+    print("formats.npz", sorted(out))
+
+
 def main():
     arc_fixture()
     slab_fixture()
+    formats_fixture()
     for out, rel in FILES.items():
         src = os.path.join(REF, rel)
         if not os.path.exists(src):
