@@ -68,39 +68,58 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
         if disp > 0.5 * w.skin:
             raise RuntimeError(f"skin exceeded between rebuilds ({disp:.4f})")
 
+    state = {"step": 0}     # steps run since the first rebuild (warmup included)
+
     def segment_loop(n_steps):
-        """Segments of `every` steps, each ended by a rebuild; between two
-        statistics read-backs (every THERMO_EVERY steps) the segments and
-        their rebuilds are queued with one synchronisation (sre_run_segments,
-        which checks every rebuild's list capacity and displacement at the
-        end), the last segment is followed by the read-back and a rebuild."""
+        """Runs n_steps steps in segments of `every` steps, each ended by a
+        rebuild.  The energy and virial are reduced and read back at every
+        multiple of THERMO_EVERY steps (counted from the start), like the
+        reference driver's thermo output; between two read-backs the segments
+        and their rebuilds are queued with one synchronisation
+        (sre_run_segments, which checks every rebuild's list capacity and
+        displacement at the end).  A block at a read-back ends with the
+        read-back and a rebuild; a run that ends between two read-backs ends
+        with the queued segments' own check (or the skin-flag check of a
+        trailing partial segment)."""
         done = 0
         pairs = 0
         rebuilds = 0
         nl, _, _ = e.neighbor_count(in_rc=False)   # known since the last rebuild
         while done < n_steps:
-            block = min(THERMO_EVERY, n_steps - done)
-            nseg = max(1, -(-block // every))
-            if nseg > 1:
-                nls, disp = e.run_segments(nseg - 1, every, stats=True, tail=True,
+            to_thermo = THERMO_EVERY - state["step"] % THERMO_EVERY
+            block = min(to_thermo, n_steps - done)
+            thermo = block == to_thermo
+            nfull, rem = divmod(block, every)
+            if thermo and rem == 0:     # the read-back and rebuild end the last segment
+                nfull, rem = nfull - 1, every
+            if nfull > 0:
+                nls, disp = e.run_segments(nfull, every, stats=True, tail=True,
                                            perturb=True)
                 check_skin(disp)
-                for k in range(nseg - 1):
+                for k in range(nfull):
                     pairs += nl * every
                     nl = int(nls[k])
-                rebuilds += nseg - 1
-            n = block - (nseg - 1) * every
-            e.graph_run(n, stats=True, tail=True, perturb=True)
-            pairs += nl * n
+                rebuilds += nfull
+            if rem > 0:
+                e.graph_run(rem, stats=True, tail=True, perturb=True)
+                pairs += nl * rem
+                if thermo:
+                    e.stats()             # reduce + read back; raises on a force overflow
+                    e.rebuild()
+                    rebuilds += 1
+                    check_skin(e.last_rebuild_displacement())
+                    nl, _, _ = e.neighbor_count(in_rc=False)
+                else:
+                    e.check_skin_flag()   # synchronises; raises past skin/2
             done += block
-            e.stats()                     # reduce + read back; raises on a force overflow
-            e.rebuild()
-            rebuilds += 1
-            check_skin(e.last_rebuild_displacement())
-            nl, _, _ = e.neighbor_count(in_rc=False)
+            state["step"] += block
         return pairs, rebuilds
 
-    segment_loop(warmup)
+    # untimed warm-up: at least one whole read-back block, rounded up to
+    # whole blocks, so that every graph the timed steps replay is captured,
+    # every buffer is sized and the timed steps start just after a read-back
+    warm = -(-max(warmup, THERMO_EVERY) // THERMO_EVERY) * THERMO_EVERY
+    segment_loop(warm)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -125,39 +144,71 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
     dist = force_time_distribution(e, torch, stream)
     rb_ms = e.time_rebuild(10)
     e.compute(stats=True, tail=True)
-    e.stats()                   # raises if any pair force left the fixed-point range
+    e.stats()    # raises if any pair force since the warm-up left the fixed-point range
+                 # (the flag is kept until it is read)
     nl, nin, mx = e.neighbor_count()
-    return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs,
+    return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs, warm=warm,
                 rebuilds=rebuilds, force_ms=k_ms, n_list=nl, n_in_rc=nin, max_list=mx,
                 force_us_dist=dist, rebuild_ms=rb_ms, autotune_s=tune_s,
                 force_piped_ms=k_piped_ms,
                 pair_evals_per_s=pairs / wall, ms_per_step=1e3 * wall / steps)
 
 
-def accuracy(w, torch, Engine):
-    """Mixed path vs the FP64 form at the same (initial) positions."""
+def _errors(F, S, F_ref, S_ref):
     import numpy as np
-    out = {}
+    rms = np.sqrt((F_ref ** 2).sum(1).mean())
+    d = np.sqrt(((F - F_ref) ** 2).sum(1))
+    floor = np.maximum(np.sqrt((F_ref ** 2).sum(1)), rms)
+    return {"max_rel_force_err": float(d.max() / rms),
+            "rms_rel_force_err": float(np.sqrt((d ** 2).mean()) / rms),
+            "max_per_atom_rel_err_floored": float((d / floor).max()),
+            "energy_rel_err": float(abs(S[0] - S_ref[0]) / abs(S_ref[0])),
+            "virial_rel_err": float(np.max(np.abs(S[1:] - S_ref[1:]) / np.abs(S_ref[1:])))}
+
+
+def accuracy(w, torch, Engine):
+    """Mixed path and FP64 GPU form at the initial positions against the FP64
+    oracle (C++ on the host: same rc assignment, list and tail), and the mixed
+    path against the FP64 GPU form.  Errors: max_i |F_i - F_i^ref| / RMS_i
+    |F_i^ref|, the RMS of the same, per atom with the floor max(|F_i^ref|, RMS),
+    and the relative energy and virial errors.  Runs outside every timed
+    region."""
+    import numpy as np
+    from oracle import oracle as O
     res = {}
     for prec in ("mixed", "fp64"):
         e = Engine(n_types=w.n_types, skin=w.skin, precision=prec)
         w.setup(e, tail=True)
         e.rebuild()
         e.compute(stats=True, tail=True)
-        res[prec] = (e.forces(), e.stats()[0], e.atom_stats())
+        res[prec] = (e.forces(), e.stats())
         e.close()
-    F, S, A = res["mixed"]
-    F64, S64, A64 = res["fp64"]
-    rms = np.sqrt((F64 ** 2).sum(1).mean())
-    d = np.sqrt(((F - F64) ** 2).sum(1))
-    out["max_rel_force_err"] = float(d.max() / rms)
-    out["rms_rel_force_err"] = float(np.sqrt((d ** 2).mean()) / rms)
-    floor = np.maximum(np.sqrt((F64 ** 2).sum(1)), rms)
-    out["max_per_atom_rel_err_floored"] = float((d / floor).max())
-    out["energy_rel_err"] = float(abs(S[0] - S64[0]) / abs(S64[0]))
-    out["virial_rel_err"] = float(np.max(np.abs(S[1:] - S64[1:]) / np.abs(S64[1:])))
-    out["reference"] = "FP64 GPU form (itself checked against the FP64 oracle in tests/)"
-    return out
+    ref_F, ref_S = [], []
+    for r in range(len(w.n_atoms)):
+        sl = w.replica_slice(r)
+        x, box, types = w.xyz[sl], w.boxes[r], w.types[sl]
+        rc, _ = O.assign_rc(x, box, w.rc_tables[r])
+        off, j = O.build_pairs(x, box, rc, w.skin)
+        tail = None
+        if w.rho and w.rho[r] is not None:
+            tail = O.tail_atoms(x, box, types, w.eps, w.sigma, w.rc_tables[r].ravel(),
+                                w.rho[r])
+        F, E, W, _ = O.forces(x, box, types, w.eps, w.sigma, rc, off, j, tail=tail)
+        ref_F.append(F)
+        ref_S.append(np.concatenate([[E.sum()], W.sum(0)]))
+    F64 = np.concatenate(ref_F)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        F, S = res[prec]
+        per = [_errors(F[w.replica_slice(r)], S[r], ref_F[r], ref_S[r])
+               for r in range(len(w.n_atoms))]
+        whole = _errors(F, S[0], F64, ref_S[0])
+        # max over replicas for the energy/virial figures
+        for k in ("energy_rel_err", "virial_rel_err"):
+            whole[k] = max(p[k] for p in per)
+        out[prec] = whole
+    mixed_vs_fp64 = _errors(res["mixed"][0], res["mixed"][1][0], res["fp64"][0], res["fp64"][1][0])
+    return out, mixed_vs_fp64
 
 
 def measured_traffic(workload):
@@ -436,8 +487,11 @@ def main():
                         autotune=args.autotune)
         log(f"[bench] fp64:  {fp64['pair_evals_per_s']:.4g} pair evals/s, "
             f"{fp64['ms_per_step']:.4f} ms/step, force kernel {fp64['force_ms'] * 1e3:.2f} us")
-    acc = accuracy(w, torch, Engine)
-    log(f"[bench] accuracy: {acc}")
+    acc_oracle, acc = accuracy(w, torch, Engine)
+    acc["reference"] = "FP64 GPU form"
+    acc_oracle["reference"] = ("FP64 oracle (C++ on the host) at the initial positions; "
+                               "mixed and fp64 are the two GPU forms")
+    log(f"[bench] accuracy vs oracle: {acc_oracle}")
 
     L, N = mixed["n_list"], w.n
     b_force = 4.0 * L + 84.0 * N
@@ -462,6 +516,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": mixed["warm"],
         "ms_per_step": round(mixed["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
@@ -483,6 +538,7 @@ def main():
         "rebuild_us_per_step": round(mixed["rebuild_ms"] * 1e3 / args.rebuild_every, 2),
         "force_pair_evals_per_s": round(L / (mixed["force_ms"] * 1e-3), 1),
         "roofline": roofline,
+        "accuracy_vs_oracle": acc_oracle,
         "accuracy_vs_fp64": acc,
         "cpu_baseline": cpu,
     }

@@ -51,7 +51,11 @@ typedef enum {
   SRE_ERR_INVALID = 4,         /* bad argument or call order */
   SRE_ERR_HIP = 5,             /* a HIP runtime call failed (cf. checkCUDAError, H/include/error.h:8-16) */
   SRE_ERR_BOX_TOO_SMALL = 6,   /* rc_max + skin >= L/2 in some direction: minimum image is ambiguous */
-  SRE_ERR_FORCE_RANGE = 7,     /* pair forces beyond the exact fixed-point range (overlap) */
+  SRE_ERR_FORCE_RANGE = 7,     /* pair forces beyond the exact fixed-point range (overlap);
+                                  reported once by the read that finds it, which also marks the
+                                  forces and statistics invalid; the flag is cleared then and
+                                  by sre_set_system, sre_upload_positions, sre_bind_positions
+                                  and an MD rollback, so the next compute starts clean */
   SRE_ERR_SKIN = 8,            /* an atom moved more than skin/2 since the last rebuild (fixed-period mode) */
   SRE_ERR_NOT_READY = 9        /* compute requested before the first rebuild */
 } sre_status;
@@ -255,7 +259,9 @@ sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
  * rebuild positions (compare with skin/2).  If a rebuild's longest list
  * exceeded the capacity, the lists from it on were cut: the capacity is
  * regrown and the lists rebuilt, and SRE_ERR_LIST_OVERFLOW reports those
- * segments as invalid. */
+ * segments as invalid.  If an atom moved more than skin/2 within a segment,
+ * SRE_ERR_SKIN names the first such segment (its forces may have missed
+ * pairs); the outputs are filled in either case and the lists are current. */
 sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segments, int steps,
                             int64_t *n_list, float *max_disp);
 /* Perturbation of the bound positions applied by SRE_PERTURB replays, one
@@ -327,7 +333,9 @@ sre_status sre_set_velocities(sre_engine *e, const float *v);
 sre_status sre_get_velocities(sre_engine *e, float *v);
 /* Time step, thermostat (SRE_THERMO_*; NoseHoover_Chains2::reinit with
  * reference temperature ref_T and period tau_T), remove_com 1 = remove the
- * centre-of-mass velocity at the start of every step.  Resets the chain. */
+ * centre-of-mass velocity at the start of every step.  Resets the chain.
+ * sre_set_system discards the integrator (its 3N degrees of freedom and
+ * chain masses belong to the old system): call this again after it. */
 sre_status sre_set_integrator(sre_engine *e, double dt, int thermostat,
                               double ref_T, double tau_T, int remove_com);
 /* n_steps MD steps.  rebuild_every > 0: the lists are rebuilt every that many

@@ -204,7 +204,7 @@ struct sre_engine {
   double *d_snap_part = nullptr;
   MDState *d_snap_md = nullptr;
   size_t snap_pos_cap = 0;
-  int md_adapt_k = 8;              // steps of the next adaptive stretch
+  int md_adapt_k = 8;              // steps of the next adaptive stretch (reset by sre_set_system)
   int md_rollbacks = 0;            // stretches redone after a skin violation
 
   ~sre_engine();
@@ -312,6 +312,24 @@ extern "C" void sre_destroy(sre_engine *e) {
 
 // ------------------------------------------------------ system description --
 
+// The force-range flag (d_flags[1]) is set by a force kernel whose fixed-point
+// sums left the exact range.  It describes the forces computed since it was
+// last cleared: it is cleared (a stream-ordered fill, safe inside captured
+// work) when it is reported, when the system is set and when new positions
+// are uploaded, bound or restored from an MD snapshot.  A report also marks
+// the forces and statistics invalid, so they cannot be read afterwards.
+static void clear_force_flag(sre_engine *e) {
+  launch_fill_u32(e->d_flags + 1, 0u, 1, e->stream);
+}
+
+static sre_status report_force_range(sre_engine *e) {
+  clear_force_flag(e);
+  e->forces_valid = e->stats_valid = e->atom_stats_valid = false;
+  return fail(SRE_ERR_FORCE_RANGE,
+              "pair forces beyond the exact fixed-point range (overlapping atoms?)");
+}
+
+
 extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
                                      const int *n_atoms, const float *box) {
   if (!e || n_replicas < 1 || !n_atoms || !box)
@@ -378,6 +396,12 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_snap_part);
   e->snap_pos_cap = 0;
   e->vel_set = false;
+  // the integrator's state (3N degrees of freedom, chain masses) belongs to
+  // the old system: the caller sets it again with sre_set_integrator
+  e->md_set = false;
+  e->md_since_rebuild = 0;
+  e->md_adapt_k = 8;
+  clear_force_flag(e);
   dfree(e->sb.key_a); dfree(e->sb.val_a); dfree(e->sb.key_b); dfree(e->sb.val_b);
   dfree(e->sb.tkey); dfree(e->sb.tval);
   HIP_TRY(dalloc(&e->d_rep_of, N));
@@ -621,6 +645,7 @@ extern "C" sre_status sre_upload_positions(sre_engine *e, const float *xyz,
   }
   HIP_TRY(hipMemcpyAsync(e->d_pos_own, xyz, nf * sizeof(float),
                          hipMemcpyHostToDevice, e->stream));
+  clear_force_flag(e);
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (e->d_pos != e->d_pos_own || e->pos_stride != stride) e->graph_epoch++;
   e->d_pos = e->d_pos_own;
@@ -632,9 +657,11 @@ extern "C" sre_status sre_bind_positions(sre_engine *e, const void *d_xyz,
                                          int stride) {
   if (!e || !e->system_set || !d_xyz || stride < 3)
     return fail(SRE_ERR_INVALID, "sre_bind_positions: bad arguments");
+  HIP_TRY(hipSetDevice(e->device));
   if (e->d_pos != d_xyz || e->pos_stride != stride) e->graph_epoch++;
   e->d_pos = (const float *)d_xyz;
   e->pos_stride = stride;
+  clear_force_flag(e);
   return SRE_OK;
 }
 
@@ -1115,9 +1142,7 @@ static sre_status check_errors(sre_engine *e) {
   HIP_TRY(hipMemcpyAsync(&f, e->d_flags + 1, sizeof(unsigned),
                          hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
-  if (f & 1u)
-    return fail(SRE_ERR_FORCE_RANGE,
-                "pair forces beyond the exact fixed-point range (overlapping atoms?)");
+  if (f & 1u) return report_force_range(e);
   return SRE_OK;
 }
 
@@ -1787,6 +1812,7 @@ extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segm
     int mc = 0;
     for (int g = worst; g < n_segments; ++g)
       mc = std::max(mc, (int)e->h_seg_info[3 * (size_t)g]);
+    const int old_cap = e->cap;
     sre_status s = ensure_list(
         e, round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes));
     if (s != SRE_OK) return s;
@@ -1794,8 +1820,21 @@ extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segm
     if (s != SRE_OK) return s;
     return fail(SRE_ERR_LIST_OVERFLOW,
                 "neighbor list overflow at the rebuild ending segment %d (%d > %d); "
-                "the capacity has been regrown, the later segments are invalid",
-                worst, mc, e->cap);
+                "the capacity has been regrown to %d, the later segments are invalid",
+                worst, mc, old_cap, e->cap);
+  }
+  // a segment in which an atom moved more than skin/2 from its rebuild
+  // position may have missed pairs: its forces are invalid (the lists are
+  // rebuilt, so the next call starts from a consistent state)
+  const float half = 0.5f * e->skin;
+  for (int g = 0; g < n_segments; ++g) {
+    const unsigned bits = (unsigned)(e->h_seg_info[3 * (size_t)g + 2] & 0xffffffffu);
+    float d2;
+    memcpy(&d2, &bits, sizeof d2);
+    if (d2 > half * half)
+      return fail(SRE_ERR_SKIN,
+                  "segment %d: max displacement %g exceeds skin/2 = %g; its forces "
+                  "may have missed pairs", g, sqrt((double)d2), (double)half);
   }
   return SRE_OK;
 }
@@ -2052,9 +2091,7 @@ static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
   const float half = 0.5f * e->skin;
   *ok = !(d2 > half * half);
   if (disp2) *disp2 = d2;
-  if (*ok && (bits[1] & 1u))
-    return fail(SRE_ERR_FORCE_RANGE,
-                "pair forces beyond the exact fixed-point range (overlapping atoms?)");
+  if (*ok && (bits[1] & 1u)) return report_force_range(e);
   return SRE_OK;
 }
 
@@ -2090,6 +2127,7 @@ static sre_status md_snapshot(sre_engine *e, bool restore) {
     ++c.count;
   }
   launch_md_copy(c, e->stream);     // one launch for all five buffers
+  if (restore) clear_force_flag(e);
   HIP_TRY(hipGetLastError());
   return SRE_OK;
 }

@@ -4,7 +4,8 @@
                  uniform rc 2.5, skin 0.3 (the uniform-cutoff "plain method")
   slab           64000-atom liquid-vapour slab in 30 x 30 x 120, rho 0.73/0.02,
                  rc(z) in 256 bins from 2.5 (bulk liquid) to 5.0 (interfaces)
-  binary_slab    the slab with A/B species 80:20, eps 1.0/1.5/0.5, sigma 1.0/0.8/0.88
+  binary_slab    the slab with A/B species 80:20, eps 1.0/1.5/0.5, sigma 1.0/0.8/0.88;
+                 perturbed_params gives its force-matching target's parameters
   replicas       256 slabs, N log-uniform in [2000, 32000], aspect 3-5, 64-bin rc(z)
                  from 2.5 to 4.0
   large_slab     2,048,000 binary atoms in 96 x 96 x 384, rc(z) 2.5 -> 6.0 in 1024 bins
@@ -136,6 +137,24 @@ def slab(seed=2, n=64000, box=(30.0, 30.0, 120.0), nz=256, rc_min=2.5, rc_max=5.
 
 def binary_slab(seed=3, n=64000):
     return slab(seed=seed, n=n, binary=True)
+
+
+def perturbed_params(w, rel=0.02, seed=11):
+    """The force-matching target's parameters of the differentiation
+    workload: every type-pair parameter (n_types (n_types + 1) / 2 epsilons
+    and as many sigmas, the pair (a, b) shared by both matrix entries)
+    multiplied by 1 + rel N(0, 1), seeded.  Returns symmetric float32
+    (eps, sigma) matrices; the target forces are the FP64 forces of the
+    workload's positions at these parameters."""
+    rng = np.random.default_rng(seed)
+    nt = w.n_types
+    eps = np.asarray(w.eps, np.float64).copy()
+    sig = np.asarray(w.sigma, np.float64).copy()
+    for m in (eps, sig):
+        for a in range(nt):
+            for b in range(a, nt):
+                m[a, b] = m[b, a] = m[a, b] * (1.0 + rel * rng.normal())
+    return eps.astype(np.float32), sig.astype(np.float32)
 
 
 def replicas(seed=4, count=256, n_lo=2000, n_hi=32000, nz=64, rc_min=2.5, rc_max=4.0,

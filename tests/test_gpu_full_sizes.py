@@ -74,9 +74,19 @@ def test_all_256_replicas(torch_cuda):
     assert worst <= TOL_F_MIXED
 
 
-@pytest.mark.timeout(600)
-def test_large_slab_2M(torch_cuda):
+@pytest.fixture(scope="module")
+def large():
+    """The 2,048,000-atom binary slab with the oracle's rc and lists."""
     w = configs.large_slab()
+    box = w.boxes[0]
+    rc, _ = O.assign_rc(w.xyz, box, w.rc_tables[0])
+    off, j = O.build_pairs(w.xyz, box, rc, w.skin)
+    return dict(w=w, rc=rc, off=off, j=j)
+
+
+@pytest.mark.timeout(600)
+def test_large_slab_2M(torch_cuda, large):
+    w = large["w"]
     e = Engine(n_types=2, skin=w.skin)
     w.setup(e, tail=True)
     e.rebuild()
@@ -84,10 +94,42 @@ def test_large_slab_2M(torch_cuda):
     f1 = e.forces_fixed()
     e.compute(stats=True, tail=True)
     assert np.array_equal(e.forces_fixed(), f1)          # bitwise reproducible
-    F64, E64, W64, L = check_against_oracle(e, w, counts=atom_counts(e))
+    xyz, types, box = w.xyz, w.types, w.boxes[0]
+    off, j = large["off"], large["j"]
+    assert np.array_equal(atom_counts(e), np.diff(off))
+    tail = O.tail_atoms(xyz, box, types, w.eps, w.sigma, w.rc_tables[0].ravel(), w.rho[0])
+    F64, E64, W64, nin = O.forces(xyz, box, types, w.eps, w.sigma, large["rc"], off, j,
+                                  tail=tail)
+    L = int(off[-1])
     assert e.neighbor_count()[0] == L
     err = rel_err(f1 * 2.0 ** -32, F64)
     print(f"large_slab_2M: L={L}, max rel force err {err:.3e}")
     assert err <= TOL_F_MIXED
     st = e.stats()[0]
     assert abs(st[0] - E64.sum()) / np.abs(E64).sum() <= 1e-6
+
+
+@pytest.mark.timeout(600)
+def test_large_slab_2M_adjoint(torch_cuda, large):
+    """The epsilon/sigma adjoint of the large-slab workload (mixed precision)
+    against the FP64 oracle's analytic gradient: dL/dE = 0.37 and a seeded
+    Gaussian dL/dF, pair terms and tail."""
+    import torch
+    w = large["w"]
+    e = Engine(n_types=2, skin=w.skin)
+    w.setup(e, tail=True)
+    e.rebuild()
+    g = np.random.default_rng(23).normal(size=(w.n, 3))
+    dLdE = 0.37
+    ge, gs, _ = e.adjoint(dLdE=[dLdE], dLdF=torch.from_numpy(g).cuda(), tail=True)
+    box = w.boxes[0]
+    pe, ps = O.grad_pairs(w.xyz, box, w.types, w.eps, w.sigma, large["rc"], large["off"],
+                          large["j"], dLdE, g)
+    te, ts, _ = O.grad_tail(w.xyz, box, w.types, w.eps, w.sigma, w.rc_tables[0].ravel(),
+                            w.rho[0], dLdE, g, h=1e-4)
+    re, rs = pe + te, ps + ts
+    tol = 2e-5
+    for a, b in [(0, 0), (0, 1), (1, 1)]:
+        assert abs(ge[a, b] - re[a, b]) <= tol * np.abs(re).max(), ("eps", a, b)
+        assert abs(gs[a, b] - rs[a, b]) <= tol * np.abs(rs).max(), ("sigma", a, b)
+    print(f"large_slab_2M adjoint: eps {ge.ravel()}, sigma {gs.ravel()}")
