@@ -418,6 +418,34 @@ sre_status sre_arc_apply_force_correction(sre_arc *a, sre_engine *e, int replica
 /* AssignRCut::getRCut: installs the result as the rc lattice of a replica. */
 sre_status sre_arc_apply(sre_arc *a, sre_engine *e, int replica);
 
+/* ---- one force evaluation without device allocation ------------------------ */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc): the allocation-free variant of sre_step (rebuild + compute)
+ * followed by sre_get_forces(dst_on_device) and sre_get_stats, for one
+ * periodic system in mixed precision with an rc(z) table and no tail term.
+ * Every buffer comes from a 40 MiB device array held by each GPU the library
+ * runs on until the process ends; no call allocates device memory.  Calls
+ * that overlap on one GPU race (they share that array).  Limits:
+ * n_atoms <= SRE_STATIC_MAX_ATOMS, nz <= SRE_STATIC_MAX_RC_BINS, at most
+ * SRE_STATIC_MAX_LIST neighbor-list entries per atom (else
+ * SRE_ERR_LIST_OVERFLOW; no regrowing).  Inputs: d_xyz device positions
+ * (stride floats per atom), d_types device int32 types in [0, n_types) or
+ * NULL (all 0), box[3], eps/sigma symmetric n_types^2 (host), the rc(z) table
+ * rc_z[nz] (host), skin.  Outputs: d_forces device double[N][3] (caller
+ * order; the same values as the engine path), stats host double[4] (energy,
+ * virial xx, yy, zz) or NULL.  Synchronises the stream (NULL: the null
+ * stream) before returning. */
+#define SRE_STATIC_MAX_ATOMS 16384
+#define SRE_STATIC_MAX_RC_BINS 4096
+#define SRE_STATIC_MAX_LIST 512
+sre_status sre_forces_static(int n_atoms, const float *d_xyz, int stride,
+                             const int32_t *d_types, const float box[3], int n_types,
+                             const float *eps, const float *sigma, int nz,
+                             const float *rc_z, float skin, void *stream,
+                             double *d_forces, double *stats);
+
 /* ---- synthetic configurations (host only, no GPU use) --------------------- */
 
 /* FCC lattice of ncell^3 unit cells with lattice constant a, every coordinate

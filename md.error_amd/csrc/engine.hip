@@ -478,16 +478,8 @@ static sre_status upload_params(sre_engine *e) {
   const int nt = e->n_types;
   std::vector<PairParam> p(nt * nt);
   std::vector<PairParamD> pd(nt * nt);
-  for (int a = 0; a < nt * nt; ++a) {
-    const double ep = e->eps[a], sg = e->sig[a], sh = e->shift[a];
-    const double s6 = sg * sg * sg * sg * sg * sg, s12 = s6 * s6;
-    const double A = 4.0 * ep * s12, B = 4.0 * ep * s6;
-    pd[a] = PairParamD{12.0 * A, 6.0 * B, A, B, sh, ep, sg, 0.0};
-    // the mixed kernel's force constants carry the fixed-point scale 2^32
-    // (an exact power-of-two factor), so pair components come out pre-scaled
-    p[a] = PairParam{(float)(12.0 * A * kFixScale), (float)(6.0 * B * kFixScale),
-                     (float)A, (float)B, (float)sh, (float)ep, (float)sg, 0.f};
-  }
+  for (int a = 0; a < nt * nt; ++a)
+    make_pair_params(e->eps[a], e->sig[a], e->shift[a], &p[a], &pd[a]);
   HIP_TRY(hipMemcpy(e->d_params, p.data(), sizeof(PairParam) * nt * nt,
                     hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(e->d_paramsd, pd.data(), sizeof(PairParamD) * nt * nt,
@@ -713,27 +705,11 @@ static sre_status prepare_tables(sre_engine *e) {
         return fail(SRE_ERR_BOX_TOO_SMALL,
                     "replica %d: rc_max + skin = %g is not below half the box "
                     "edge %d (%g)", r, rlmax, d, R.L[d]);
-    const double rlmin = R.rc_min + e->skin;
-    double hxy = std::max(0.05, (double)e->tune.cell_xy) * rlmin;
-    double hz = std::max(0.05, (double)e->tune.cell_z) * rlmin;
-    int nc[3];
-    for (int tries = 0;; ++tries) {
-      nc[0] = std::max(1, (int)std::floor(R.L[0] / hxy));
-      nc[1] = std::max(1, (int)std::floor(R.L[1] / hxy));
-      nc[2] = std::max(1, (int)std::floor(R.L[2] / hz));
-      const long long c = (long long)nc[0] * nc[1] * nc[2];
-      if (c <= 8ll * R.n_atoms + 4096 || tries > 40) break;
-      hxy *= 1.1;
-      hz *= 1.1;
-    }
-    for (int d = 0; d < 3; ++d) {
-      R.nc[d] = nc[d];
-      R.h[d] = R.L[d] / (float)nc[d];
-    }
+    choose_cell_grid(R, e->skin, e->tune.cell_xy, e->tune.cell_z);
     R.cell_off = (int)cells;
     R.col_off = (int)cols;
-    cells += (long long)nc[0] * nc[1] * nc[2];
-    cols += (long long)nc[0] * nc[1];
+    cells += (long long)R.nc[0] * R.nc[1] * R.nc[2];
+    cols += (long long)R.nc[0] * R.nc[1];
   }
   if (cells >= (1ll << 31) - 2)
     return fail(SRE_ERR_CELL, "cell grid too large (%lld cells)", cells);

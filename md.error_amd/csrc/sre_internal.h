@@ -58,6 +58,30 @@ struct RepInfo {
   int fc_n[3];      // force-correction grid dims
 };
 
+// Cell grid of a replica: edges of about cell_xy (x, y) and cell_z (z) times
+// rc_min + skin, coarsened by 10 % steps while there are more than
+// 8 N + 4096 cells; sets R.nc and R.h (R.L, R.n_atoms and R.rc_min set).
+inline void choose_cell_grid(RepInfo &R, float skin, float cell_xy, float cell_z) {
+  const double rlmin = R.rc_min + skin;
+  double hxy = (cell_xy > 0.05f ? cell_xy : 0.05) * rlmin;
+  double hz = (cell_z > 0.05f ? cell_z : 0.05) * rlmin;
+  int nc[3];
+  for (int tries = 0;; ++tries) {
+    for (int d = 0; d < 3; ++d) {
+      const int v = (int)floor(R.L[d] / (d < 2 ? hxy : hz));
+      nc[d] = v > 1 ? v : 1;
+    }
+    const long long c = (long long)nc[0] * nc[1] * nc[2];
+    if (c <= 8ll * R.n_atoms + 4096 || tries > 40) break;
+    hxy *= 1.1;
+    hz *= 1.1;
+  }
+  for (int d = 0; d < 3; ++d) {
+    R.nc[d] = nc[d];
+    R.h[d] = R.L[d] / (float)nc[d];
+  }
+}
+
 // LJ pair constants in the form used by the kernels:
 //   u = ri6*(A*ri6 - B) + shift,  F_i = d * ri2*ri6*(B6 - A12*ri6)
 // with A = 4 eps sig^12, B = 4 eps sig^6, A12 = 12 A, B6 = 6 B.
@@ -69,6 +93,18 @@ struct PairParamD {
   double A12, B6, A, B;
   double shift, eps, sig, pad;
 };
+
+// The kernels' constants of one type pair from (eps, sigma, shift); the
+// mixed kernel's force constants carry the fixed-point scale 2^32 (an exact
+// power-of-two factor), so its pair components come out pre-scaled.
+inline void make_pair_params(double ep, double sg, double sh, PairParam *p,
+                             PairParamD *pd) {
+  const double s6 = sg * sg * sg * sg * sg * sg, s12 = s6 * s6;
+  const double A = 4.0 * ep * s12, B = 4.0 * ep * s6;
+  *pd = PairParamD{12.0 * A, 6.0 * B, A, B, sh, ep, sg, 0.0};
+  *p = PairParam{(float)(12.0 * A * kFixScale), (float)(6.0 * B * kFixScale),
+                 (float)A, (float)B, (float)sh, (float)ep, (float)sg, 0.f};
+}
 
 // ---------------------------------------------------------------- helpers --
 

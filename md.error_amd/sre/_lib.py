@@ -132,6 +132,8 @@ SIGNATURES = {
     "sre_arc_apply_force_correction": (_st, [_vp, _vp, C.c_int]),
     "sre_set_force_grid": (_st, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _f32p]),
     "sre_arc_pressure_integrals": (_st, [_vp, _f64p, _f64p]),
+    "sre_forces_static": (_st, [C.c_int, _vp, C.c_int, _vp, _f32p, C.c_int, _f32p, _f32p,
+                                C.c_int, _f32p, C.c_float, _vp, _vp, _f64p]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_double, C.c_double, C.c_uint64, _f32p]),

@@ -398,6 +398,36 @@ class Engine:
         check(lib().sre_set_md_state(self._h, C.byref(s)))
 
 
+def forces_static(xyz, box, rc_z, skin=0.3, types=None, eps=None, sigma=None,
+                  n_types=1, stream=None, forces=None):
+    """sre_forces_static: one force evaluation (rebuild + mixed-precision
+    compute) with no device allocation by the library.  xyz: CUDA float32
+    tensor (N, 3 or 4); types: CUDA int32 tensor (N,) or None; forces: CUDA
+    float64 tensor (N, 3) to fill (allocated here if None).  Returns
+    (forces, stats[4] = energy, virial xx, yy, zz)."""
+    import torch
+    assert xyz.is_cuda and xyz.dtype == torch.float32 and xyz.is_contiguous()
+    n = xyz.shape[0]
+    if types is not None:
+        assert types.is_cuda and types.dtype == torch.int32 and types.is_contiguous()
+    if forces is None:
+        forces = torch.empty((n, 3), dtype=torch.float64, device=xyz.device)
+    assert forces.is_cuda and forces.dtype == torch.float64 and forces.is_contiguous()
+    nt = int(n_types)
+    e = np.ascontiguousarray(np.ones((nt, nt)) if eps is None else eps, np.float32)
+    s = np.ascontiguousarray(np.ones((nt, nt)) if sigma is None else sigma, np.float32)
+    b = np.ascontiguousarray(np.asarray(box, np.float32).ravel()[:3])
+    rc = np.ascontiguousarray(np.asarray(rc_z, np.float32).ravel())
+    st = np.zeros(4)
+    check(lib().sre_forces_static(n, C.c_void_p(xyz.data_ptr()), xyz.shape[1],
+                                  C.c_void_p(types.data_ptr()) if types is not None else None,
+                                  _ptr(b, C.c_float), nt, _ptr(e, C.c_float), _ptr(s, C.c_float),
+                                  rc.size, _ptr(rc, C.c_float), float(skin),
+                                  C.c_void_p(stream) if stream else None,
+                                  C.c_void_p(forces.data_ptr()), _ptr(st, C.c_double)))
+    return forces, st
+
+
 def gen_fcc(ncell, a, jitter, seed):
     n = 4 * ncell ** 3
     out = np.empty((n, 4), np.float32)
