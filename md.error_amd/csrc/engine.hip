@@ -1045,6 +1045,11 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
   a.err_flag = e->d_flags + 1;
   a.step_ctr = nullptr;
+  // the per-atom builder writes one (longest list, total) pair per wave of
+  // build_lanes lanes per atom: the force kernel's waves when the lane
+  // counts agree
+  a.wave_len = (e->tune.build_mode == 0 && e->tune.build_lanes == e->list_lanes)
+                   ? e->d_build_part : nullptr;
   a.xcd_remap = e->tune.xcd_remap;
   a.has_shift = e->has_shift ? 1 : 0;
   return a;

@@ -217,8 +217,32 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // behind a group's arithmetic.  Rows past the end of a wave's tile (up to 2U)
 // read the next tile or the list's tail padding; those values are never used.
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
-// pair component c is already c*2^32 in FP32 and its fixed-point value is the
-// bit pattern of double(c*2^32) + 1.5*2^52.
+// pair component c is already c*2^32 in FP32.
+//
+// Fixed-point sums: each lane keeps, per component, an FP64 accumulator
+// started at C = 1.5*2^52.  While it stays in [2^52, 2^53) the representable
+// values are exactly the integers, and with the FP64 rounding mode set to
+// round toward zero, acc + c rounds to acc + floor(c): one conversion and
+// one add per component give the exact integer sum C + sum floor(c), the same
+// in any order.  (floor instead of round-to-nearest: a bias of half a unit,
+// 2^-33, per pair component.)  The lane value is bits(acc) - bits(C).
+// The rounding mode is switched by hand around the pair loop, and the
+// conversions and adds that depend on it are written as one asm statement
+// each, so that the compiler, which assumes the default mode for every FP64
+// instruction it emits, neither places its own FP64 code under the switched
+// mode nor switches the mode back in front of these adds.
+constexpr double kAccBase = 6755399441055744.0;   // 1.5 * 2^52
+__device__ __forceinline__ void acc_floor(double &acc, float c) {
+  double t;
+  asm("v_cvt_f64_f32 %1, %2\n\tv_add_f64 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(c));
+}
+// MODE register field 3:2 (FP64/FP16 rounding): 3 = toward zero, 0 = nearest even
+__device__ __forceinline__ void round_f64_toward_zero() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 3" ::: "memory");
+}
+__device__ __forceinline__ void round_f64_nearest() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0" ::: "memory");
+}
 template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // the position refresh of this step has read the counter (stream order)
@@ -251,16 +275,25 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // comparison of a wave-uniform index with one per-lane bound
   const int lim = nn - l;
   const int ti = (int)(pi.w & 0xffu) * a.n_types;
-  const int groups = __builtin_amdgcn_readfirstlane(
-      wave_max_int((nn + T * U - 1) / (T * U)));
+  // the wave's longest list: recorded by the list build, else reduced here
+  int groups;
+  if (a.wave_len) {
+    const int m = min(a.wave_len[__builtin_amdgcn_readfirstlane(wave)].x, a.cap);
+    groups = __builtin_amdgcn_readfirstlane((m + T * U - 1) / (T * U));
+  } else {
+    groups = __builtin_amdgcn_readfirstlane(wave_max_int((nn + T * U - 1) / (T * U)));
+  }
   PairParam P = a.params[0];
 
-  // fixed-point sums: each pair component (already in units of 2^-32) is
-  // rounded to the nearest integer (ties to even) in FP32 and the integers
-  // are summed in FP64, exact while the lane's sum of magnitudes stays below
-  // 2^53 (checked at the end)
-  double ax = 0.0, ay = 0.0, az = 0.0;
+  // fixed-point sums (see above); exact while the lane's partial sums stay
+  // within 2^51 of C, i.e. its sum of |pair component| below 2^51 (checked
+  // at the end)
+  double ax = kAccBase, ay = kAccBase, az = kAccBase;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsabs = 0.f;
+  // single-type energy without shift: sums of 1/r^12 and 1/r^6, combined
+  // with A and B at the end
+  constexpr bool SPLIT_E = STATS && !MULTI && !SHIFT;
+  float e12 = 0.f, e6 = 0.f;
   // list rows through a resource based at this wave's tile: the lane's byte
   // offset is fixed and the row offset is a scalar, so a row load needs no
   // per-lane address arithmetic; the resource ends with the list allocation
@@ -294,12 +327,17 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       const float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
       fsabs += fabsf(fs);
       const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
-      ax += (double)__builtin_rintf(fpx);
-      ay += (double)__builtin_rintf(fpy);
-      az += (double)__builtin_rintf(fpz);
+      acc_floor(ax, fpx);    // round toward zero: acc + floor(c), exact
+      acc_floor(ay, fpy);
+      acc_floor(az, fpz);
       if (STATS) {
-        const float t = __builtin_fmaf(P.A, ri6, -P.B);
-        e += SHIFT ? __builtin_fmaf(ri6, t, in ? P.shift : 0.f) : ri6 * t;
+        if (SPLIT_E) {
+          e12 = __builtin_fmaf(ri6, ri6, e12);
+          e6 += ri6;
+        } else {
+          const float t = __builtin_fmaf(P.A, ri6, -P.B);
+          e += SHIFT ? __builtin_fmaf(ri6, t, in ? P.shift : 0.f) : ri6 * t;
+        }
         wx = __builtin_fmaf(fpx, dx, wx);
         wy = __builtin_fmaf(fpy, dy, wy);
         wz = __builtin_fmaf(fpz, dz, wz);
@@ -328,6 +366,9 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // the loads it is about to use
   asm volatile("" ::: "memory");
   prefetch(pa, ja, 0);
+  // the accumulators' FP64 adds round toward zero inside the pair loop
+  // (FP32 rounding is a separate field of the mode register, untouched)
+  round_f64_toward_zero();
   // whole pairs of groups in the loop (no branch inside its body, so the
   // loads of each half are waited for only where they are used), the odd
   // last group after it
@@ -339,20 +380,25 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
     eval_group(pb, g + 1);
   }
   if (g < groups) eval_group(pa, g);  // wave-uniform
-  int64_t Fx = (int64_t)ax, Fy = (int64_t)ay, Fz = (int64_t)az;
+  round_f64_nearest();
+  const int64_t acc0 = __double_as_longlong(kAccBase);
+  int64_t Fx = __double_as_longlong(ax) - acc0;
+  int64_t Fy = __double_as_longlong(ay) - acc0;
+  int64_t Fz = __double_as_longlong(az) - acc0;
   Fx = group_sum(Fx, T);
   Fy = group_sum(Fy, T);
   Fz = group_sum(Fz, T);
   // every lane's sum of |pair component| * 2^32 is at most rc * sum |fs|
-  // (each component is fs * d with |d| < rc), and must stay below 2^53 for
-  // the FP64 sums of integers to be exact; 2^52 leaves room for the FP32
-  // rounding of sum |fs| and of the per-term roundings to integers
-  if (__ballot(act && fsabs * rc >= 4503599627370496.0f) &&
+  // (each component is fs * d with |d| < rc), and must stay below 2^51 for
+  // the accumulators to stay in [2^52, 2^53); 2^50 leaves room for the FP32
+  // rounding of sum |fs|
+  if (__ballot(act && fsabs * rc >= 1125899906842624.0f) &&
       lane == 0)
     atomicOr(a.err_flag, 1u);
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {
-    E = group_sum((double)e, T) * 0.5;
+    E = SPLIT_E ? group_sum((double)P.A * (double)e12 - (double)P.B * (double)e6, T) * 0.5
+                : group_sum((double)e, T) * 0.5;
     Wx = group_sum((double)wx, T) * (0.5 * kFixInv);
     Wy = group_sum((double)wy, T) * (0.5 * kFixInv);
     Wz = group_sum((double)wz, T) * (0.5 * kFixInv);

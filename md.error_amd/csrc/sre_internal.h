@@ -16,8 +16,8 @@
 //    (H/src/InteractionEngine_interface.cu:1224-1245).
 //  * rc bin of an atom: one-image wrap, ix = int((x*nx)/L) in FP32 with an
 //    IEEE division (H/src/AssignRCut.cu:92-101), clamped to [0, nx-1].
-//  * Forces: int64 fixed point, scale 2^32, each pair component rounded to
-//    nearest-even before the integer sum (order-independent).
+//  * Forces: int64 fixed point, scale 2^32, each pair component rounded
+//    down to an integer (floor) before the integer sum (order-independent).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -374,6 +374,8 @@ struct ForceArgs {
   double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
   unsigned *err_flag;
   uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
+  const int2 *wave_len;        // per force wavefront: (longest list, total) from a
+                               // list build of the same tiling, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
   int has_shift;               // some pair type has a nonzero energy shift
 };
