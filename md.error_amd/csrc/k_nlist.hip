@@ -194,10 +194,15 @@ __global__ __launch_bounds__(256) void k_build_list(
       }
       asm volatile("" ::: "memory");
     };
-    // the accepted candidates of a round, placed in (lane, u) order by an
-    // exclusive prefix over the group
+    // the accepted candidates of a round, placed in flat-sequence order
+    // (v = l + TB (k0 U + u): for each u the group's lanes in turn), so that
+    // consecutive list entries are consecutive candidates -- neighbouring
+    // atoms of one cell column -- and the force kernel's lanes of one atom
+    // gather from the same few cache lines
     auto accept = [&](const int *jj, const uint4 *cj) {
-      unsigned okm = 0u;
+      const int gl = lane & ~(TB - 1);             // first lane of the group
+      const unsigned lt_mask = (1u << l) - 1u;
+      int base = count;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = jj[u];
@@ -205,24 +210,22 @@ __global__ __launch_bounds__(256) void k_build_list(
         const float dy = disp(cj[u].y, pi.y, qy);
         const float dz2 = disp(cj[u].z, pi.z, qz);
         const bool ok = (j >= 0) & (j != s) & (dist2(dx, dy, dz2) < rl2);
-        okm |= ok ? (1u << u) : 0u;
-      }
-      const int c = __popc(okm);
-      int incl = c;
-#pragma unroll
-      for (int o = 1; o < TB; o <<= 1) {
-        const int t = __shfl_up(incl, o, TB);
-        if (l >= o) incl += t;
-      }
-      int k = count + incl - c;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if ((okm >> u) & 1u) {
-          if (k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = jj[u];
-          ++k;
+        const unsigned long long wb = __ballot(ok);
+        constexpr unsigned gmask = TB >= 32 ? 0xffffffffu : (1u << (TB & 31)) - 1u;
+        const unsigned grp = TB == 64 ? 0u : (unsigned)(wb >> gl) & gmask;
+        if (TB == 64) {
+          // the whole wave is one group: lanes below this one from the mask
+          const unsigned long long below = l == 0 ? 0ull : (wb & (~0ull >> (64 - l)));
+          const int k = base + __popcll(below);
+          if (ok && k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = j;
+          base += __popcll(wb);
+        } else {
+          const int k = base + __popc(grp & lt_mask);
+          if (ok && k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = j;
+          base += __popc(grp);
         }
       }
-      count += __shfl(incl, TB - 1, TB);
+      count = base;
     };
     // software pipeline: the gathers of round k0 + 1 are in flight while
     // round k0 is tested; whole pairs of rounds in the loop (no branch in its
