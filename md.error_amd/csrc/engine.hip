@@ -457,6 +457,11 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
 
 static sre_status md_update_total_mass(sre_engine *e);
 
+// The host geometry of the only replica (position refresh argument), or NULL
+static const RepInfo *rep0(const sre_engine *e) {
+  return e->n_rep == 1 ? &e->reps[0] : nullptr;
+}
+
 extern "C" sre_status sre_set_types(sre_engine *e, const int32_t *types) {
   if (!e || !types || !e->system_set)
     return fail(SRE_ERR_INVALID, "sre_set_types: system not set");
@@ -1002,7 +1007,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   if (!e->built) return fail(SRE_ERR_NOT_READY, "update before the first rebuild");
   HIP_TRY(hipSetDevice(e->device));
   launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
-                          e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
+                          e->pos_stride, e->d_reps, rep0(e), e->d_pos4, e->d_backup,
                           e->d_res4, e->d_skin_part, nullptr, nullptr, e->stream);
   HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
@@ -1617,7 +1622,7 @@ static void enqueue_md_drift(sre_engine *e) {
   DriftArgs d{e->d_vel, e->d_fcall, e->d_mass, e->d_types, e->d_md,
               (float)e->md.dt};
   launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
-                          e->pos_stride, e->d_reps, e->d_pos4, e->d_backup,
+                          e->pos_stride, e->d_reps, rep0(e), e->d_pos4, e->d_backup,
                           e->d_res4, e->d_skin_part, nullptr, &d, e->stream);
 }
 
@@ -1668,7 +1673,7 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
       }
       launch_update_positions(e->n_atoms, e->d_inv_perm,
                               const_cast<float *>(e->d_pos), e->pos_stride,
-                              e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                              e->d_reps, rep0(e), e->d_pos4, e->d_backup, e->d_res4,
                               e->d_skin_part, pert ? &pa : nullptr, nullptr,
                               e->stream);
       s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
@@ -2131,7 +2136,7 @@ extern "C" sre_status sre_md_run(sre_engine *e, int n_steps, int rebuild_every,
     // refresh the lattice copy and rebuild if they moved beyond skin/2
     launch_update_positions(e->n_atoms, e->d_inv_perm,
                             const_cast<float *>(e->d_pos), e->pos_stride,
-                            e->d_reps, e->d_pos4, e->d_backup, e->d_res4,
+                            e->d_reps, rep0(e), e->d_pos4, e->d_backup, e->d_res4,
                             e->d_skin_part, nullptr, nullptr, e->stream);
     HIP_TRY(hipGetLastError());
     bool ok;

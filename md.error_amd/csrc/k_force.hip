@@ -29,75 +29,100 @@ namespace sre {
 // and centre-of-mass factors of the step: v <- a2 (a1 v - vcom) + F dt/(2m),
 // x <- x + v dt, then x is wrapped back into [0, L) by one add or subtract of
 // L (the reference wraps at each rebuild, H/src/MDSystem_interface.cu:19-36).
+// Box geometry the refresh needs; passed by value when all atoms share one
+// replica, so that no load of the replica table precedes the atom's loads.
+struct Geom {
+  float L[3], q[3];
+  double invq[3], qd[3];
+};
+__device__ __forceinline__ Geom geom_of(const RepInfo &R) {
+  Geom g;
+  for (int d = 0; d < 3; ++d) {
+    g.L[d] = R.L[d];
+    g.q[d] = R.q[d];
+    g.invq[d] = R.invq[d];
+    g.qd[d] = R.qd[d];
+  }
+  return g;
+}
+
+// Every load an atom needs is issued before any of them is used (the
+// refresh runs one wavefront per SIMD at 64k atoms, so the memory round trip
+// is its cost); with one replica the geometry comes in the kernel arguments.
 __global__ __launch_bounds__(256) void k_update_positions(
     int n, const uint32_t *__restrict__ inv_perm, float *__restrict__ pos,
-    int stride, const RepInfo *__restrict__ reps, uint4 *__restrict__ pos4,
-    const uint4 *__restrict__ backup, float4 *__restrict__ res4,
-    float *__restrict__ skin_part, PerturbArgs pert, DriftArgs drift) {
+    int stride, const RepInfo *__restrict__ reps, Geom g1, int one_rep,
+    uint4 *__restrict__ pos4, const uint4 *__restrict__ backup,
+    float4 *__restrict__ res4, float *__restrict__ skin_part, PerturbArgs pert,
+    DriftArgs drift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int w = i / kWave;  // slots exist for the waves holding atoms
   const bool wslot = w * kWave < n;
   // issued first: this wave's running maximum, compared at the end
   const float prev = ((threadIdx.x & (kWave - 1)) == 0 && wslot) ? skin_part[w] : 0.f;
   float d2 = 0.f;
-  // R: the atom's replica.  With one replica it is the same for every atom
-  // and is read once per wave instead of after each atom's backup word.
-  auto body = [&](const RepInfo &R) {
+  if (i < n) {
     const uint4 b = backup[i];
     const uint32_t s = inv_perm[i];
     float *p = pos + (size_t)i * stride;
     float x, y, z;
+    bool wrap = false;
     if (pert.step_ctr) {
-      const float3 g = gauss3(pert.seed, *pert.step_ctr, (uint64_t)i);
       const float *o = pert.anchor ? pert.anchor + (size_t)i * stride : p;
-      x = o[0] + pert.sigma * g.x;
-      y = o[1] + pert.sigma * g.y;
-      z = o[2] + pert.sigma * g.z;
+      const float ox = o[0], oy = o[1], oz = o[2];
+      const float3 g = gauss3(pert.seed, *pert.step_ctr, (uint64_t)i);
+      x = ox + pert.sigma * g.x;
+      y = oy + pert.sigma * g.y;
+      z = oz + pert.sigma * g.z;
       p[0] = x;
       p[1] = y;
       p[2] = z;
     } else if (drift.vel) {
-      const MDState &m = *drift.state;
-      const float a1 = (float)m.a1, a2 = (float)m.a2;
-      const float c = 0.5f * drift.dt / drift.mass[drift.types[i]];
       float *v = drift.vel + (size_t)i * 3;
       const float *f = drift.fcall + (size_t)i * 3;
-      const float vx = a2 * (a1 * v[0] - (float)m.vcom[0]) + c * f[0];
-      const float vy = a2 * (a1 * v[1] - (float)m.vcom[1]) + c * f[1];
-      const float vz = a2 * (a1 * v[2] - (float)m.vcom[2]) + c * f[2];
+      const float v0 = v[0], v1 = v[1], v2 = v[2];
+      const float f0 = f[0], f1 = f[1], f2 = f[2];
+      const float p0 = p[0], p1 = p[1], p2 = p[2];
+      const float m = drift.mass[drift.types[i]];
+      const MDState &ms = *drift.state;
+      const float a1 = (float)ms.a1, a2 = (float)ms.a2;
+      const float c = 0.5f * drift.dt / m;
+      const float vx = a2 * (a1 * v0 - (float)ms.vcom[0]) + c * f0;
+      const float vy = a2 * (a1 * v1 - (float)ms.vcom[1]) + c * f1;
+      const float vz = a2 * (a1 * v2 - (float)ms.vcom[2]) + c * f2;
       v[0] = vx;
       v[1] = vy;
       v[2] = vz;
-      x = p[0] + vx * drift.dt;
-      y = p[1] + vy * drift.dt;
-      z = p[2] + vz * drift.dt;
-      x = x < 0.f ? x + R.L[0] : (x >= R.L[0] ? x - R.L[0] : x);
-      y = y < 0.f ? y + R.L[1] : (y >= R.L[1] ? y - R.L[1] : y);
-      z = z < 0.f ? z + R.L[2] : (z >= R.L[2] ? z - R.L[2] : z);
-      p[0] = x;
-      p[1] = y;
-      p[2] = z;
+      x = p0 + vx * drift.dt;
+      y = p1 + vy * drift.dt;
+      z = p2 + vz * drift.dt;
+      wrap = true;
     } else {
       x = p[0];
       y = p[1];
       z = p[2];
     }
-    const uint32_t ux = quantize(x, R.invq[0]);
-    const uint32_t uy = quantize(y, R.invq[1]);
-    const uint32_t uz = quantize(z, R.invq[2]);
+    // R: the atom's replica; with one replica its geometry is an argument
+    const Geom G = one_rep ? g1 : geom_of(reps[b.w >> 8]);
+    if (wrap) {
+      // x + v dt wrapped back into [0, L) by one add or subtract of L (the
+      // reference wraps at each rebuild, H/src/MDSystem_interface.cu:19-36)
+      x = x < 0.f ? x + G.L[0] : (x >= G.L[0] ? x - G.L[0] : x);
+      y = y < 0.f ? y + G.L[1] : (y >= G.L[1] ? y - G.L[1] : y);
+      z = z < 0.f ? z + G.L[2] : (z >= G.L[2] ? z - G.L[2] : z);
+      p[0] = x;
+      p[1] = y;
+      p[2] = z;
+    }
+    const uint32_t ux = quantize(x, G.invq[0]);
+    const uint32_t uy = quantize(y, G.invq[1]);
+    const uint32_t uz = quantize(z, G.invq[2]);
     pos4[s] = make_uint4(ux, uy, uz, b.w);
     if (res4)
-      res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
-                            quant_residual(y, R.invq[1], R.qd[1]),
-                            quant_residual(z, R.invq[2], R.qd[2]), 0.f);
-    d2 = dist2(disp(ux, b.x, R.q[0]), disp(uy, b.y, R.q[1]),
-               disp(uz, b.z, R.q[2]));
-  };
-  if (i < n) {
-    if (reps[0].n_atoms == n)   // wave-uniform
-      body(reps[0]);
-    else
-      body(reps[backup[i].w >> 8]);
+      res4[s] = make_float4(quant_residual(x, G.invq[0], G.qd[0]),
+                            quant_residual(y, G.invq[1], G.qd[1]),
+                            quant_residual(z, G.invq[2], G.qd[2]), 0.f);
+    d2 = dist2(disp(ux, b.x, G.q[0]), disp(uy, b.y, G.q[1]), disp(uz, b.z, G.q[2]));
   }
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1)
@@ -106,8 +131,8 @@ __global__ __launch_bounds__(256) void k_update_positions(
 }
 
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
-                             int stride, const RepInfo *reps, uint4 *pos4,
-                             const uint4 *backup, float4 *res4,
+                             int stride, const RepInfo *reps, const RepInfo *host_rep0,
+                             uint4 *pos4, const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
                              const DriftArgs *drift, hipStream_t st) {
   if (n <= 0) return;
@@ -115,9 +140,17 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
   if (pert) p = *pert;
   DriftArgs d{};
   if (drift) d = *drift;
+  Geom g{};
+  if (host_rep0)
+    for (int k = 0; k < 3; ++k) {
+      g.L[k] = host_rep0->L[k];
+      g.q[k] = host_rep0->q[k];
+      g.invq[k] = host_rep0->invq[k];
+      g.qd[k] = host_rep0->qd[k];
+    }
   hipLaunchKernelGGL(k_update_positions, dim3((n + 255) / 256), dim3(256), 0,
-                     st, n, inv_perm, pos, stride, reps, pos4, backup, res4,
-                     skin_part, p, d);
+                     st, n, inv_perm, pos, stride, reps, g, host_rep0 ? 1 : 0, pos4,
+                     backup, res4, skin_part, p, d);
 }
 
 // ------------------------------------------------------- rebuild info --

@@ -171,7 +171,11 @@ __host__ __device__ inline int rc_bin(float x, float L, int n) {
 // Counter-based Gaussian deviates of (seed, step, atom): SplitMix64 (Steele,
 // Lea and Flood, "Fast splittable pseudorandom number generators", OOPSLA
 // 2014) hashes followed by the Box-Muller transform, so a perturbation does
-// not depend on launch shape or on the order atoms are visited in.
+// not depend on launch shape or on the order atoms are visited in.  Two
+// 64-bit hashes give four 24-bit uniforms in (0, 1]; the logarithm and the
+// sine and cosine are the hardware's (v_log_f32 is log2; v_sin_f32 and
+// v_cos_f32 take their argument in turns), accurate to a few ulp, which a
+// synthetic thermal displacement does not need more of.
 __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -179,18 +183,21 @@ __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
   return z ^ (z >> 31);
 }
 
-__device__ inline float u01(uint64_t x) {
-  // (0, 1]: never zero, so log() is finite
-  return ((float)(x >> 40) + 1.0f) * (1.0f / 16777216.0f);
+__device__ inline float u24(uint32_t bits24) {
+  // (0, 1]: never zero, so the logarithm is finite
+  return ((float)bits24 + 1.0f) * (1.0f / 16777216.0f);
 }
 
 __device__ inline float3 gauss3(uint64_t seed, uint64_t step, uint64_t i) {
   const uint64_t base = splitmix64(seed ^ splitmix64(step * 0x632BE59BD9B4E019ull ^ i));
-  const float u1 = u01(splitmix64(base + 1)), u2 = u01(splitmix64(base + 2));
-  const float u3 = u01(splitmix64(base + 3)), u4 = u01(splitmix64(base + 4));
-  const float r1 = sqrtf(-2.f * logf(u1)), r2 = sqrtf(-2.f * logf(u3));
-  const float t1 = 6.2831853f * u2, t2 = 6.2831853f * u4;
-  return make_float3(r1 * cosf(t1), r1 * sinf(t1), r2 * cosf(t2));
+  const uint64_t h1 = splitmix64(base + 1), h2 = splitmix64(base + 2);
+  const float u1 = u24((uint32_t)(h1 >> 40)), u2 = u24((uint32_t)(h1 >> 8) & 0xffffffu);
+  const float u3 = u24((uint32_t)(h2 >> 40)), u4 = u24((uint32_t)(h2 >> 8) & 0xffffffu);
+  // sqrt(-2 ln u) = sqrt(-2 ln2 log2 u)
+  const float r1 = __builtin_sqrtf(-1.3862943611f * __builtin_amdgcn_logf(u1));
+  const float r2 = __builtin_sqrtf(-1.3862943611f * __builtin_amdgcn_logf(u3));
+  return make_float3(r1 * __builtin_amdgcn_cosf(u2), r1 * __builtin_amdgcn_sinf(u2),
+                     r2 * __builtin_amdgcn_cosf(u4));
 }
 
 __device__ __forceinline__ float disp(uint32_t uj, uint32_t ui, float q) {
@@ -347,9 +354,10 @@ void launch_md_finish(int n, const double *part, MDState *state, float *vel,
 // (skin_part_count(n) entries).
 inline int skin_part_count(int n) { return (n + kWave - 1) / kWave; }
 // backup and inv_perm are in caller order (written by the sorted gather)
+// host_rep0: the host copy of the only replica's geometry (NULL with several)
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
-                             int stride, const RepInfo *reps, uint4 *pos4,
-                             const uint4 *backup, float4 *res4,
+                             int stride, const RepInfo *reps, const RepInfo *host_rep0,
+                             uint4 *pos4, const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
                              const DriftArgs *drift, hipStream_t st);
 
