@@ -1333,7 +1333,8 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
   // one persistent workspace: [wave partials | dLdE | out | tout | g_rc |
   // tail contributions] doubles + tail weights (fixed point); the results
   // (out, tout, g_rc) are contiguous for one readback
-  const size_t n_wp = (size_t)gw * 2 * npt, n_res = (size_t)4 * npt + rc_n;
+  // pair part: 4 integer term totals per ordered type pair (k_adjoint.hip)
+  const size_t n_wp = (size_t)gw * 4 * npt, n_res = (size_t)6 * npt + rc_n;
   const size_t n_ctb = (size_t)std::max(1, e->tail_bins) * 2 * npt;
   const size_t need = n_wp + e->n_rep + n_res + n_ctb;
   if (e->adj_cap < need) {
@@ -1341,13 +1342,14 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
     HIP_TRY(dalloc(&e->d_adj, need));
     e->adj_cap = need;
   }
-  if (e->adj_tw_cap < (size_t)2 * n_entries) {
+  const size_t tw_need = (size_t)2 * n_entries + tail_w_scratch(n_entries);
+  if (e->adj_tw_cap < tw_need) {
     dfree(e->d_adj_tw);
-    HIP_TRY(dalloc(&e->d_adj_tw, (size_t)2 * n_entries));
-    e->adj_tw_cap = (size_t)2 * n_entries;
+    HIP_TRY(dalloc(&e->d_adj_tw, tw_need));
+    e->adj_tw_cap = tw_need;
   }
   double *d_wp = e->d_adj, *d_dLdE = d_wp + n_wp, *d_out = d_dLdE + e->n_rep;
-  double *d_tout = d_out + 2 * npt, *d_grc = d_tout + 2 * npt;
+  double *d_tout = d_out + 4 * npt, *d_grc = d_tout + 2 * npt;
   double *d_contrib = d_grc + rc_n;
   int64_t *d_tw = e->d_adj_tw;
   sre_status s = SRE_OK;
@@ -1355,15 +1357,17 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
   if (dLdE) hE.assign(dLdE, dLdE + e->n_rep);
   HIP_TRY(hipMemcpyAsync(d_dLdE, hE.data(), sizeof(double) * e->n_rep,
                          hipMemcpyHostToDevice, e->stream));
-  // zero tout and g_rc (all-zero bits), and the tail weights
+  // zero tout and g_rc (all-zero bits)
   launch_fill_i64(reinterpret_cast<int64_t *>(d_tout), 0, (size_t)2 * npt + rc_n,
                   e->stream);
-  launch_fill_i64(d_tw, 0, (size_t)2 * n_entries, e->stream);
   AdjointArgs a{};
   a.n = e->n_atoms;
   a.n_types = nt;
   a.cap = e->cap;
   a.n_rep = e->n_rep;
+  a.list_len = e->list_size;
+  a.wave_len = (e->tune.build_mode == 0 && e->tune.build_lanes == e->list_lanes)
+                   ? e->d_build_part : nullptr;
   a.pos4 = e->d_pos4;
   a.rcut = e->d_rcut;
   a.nnb = e->d_nnb;
@@ -1375,20 +1379,38 @@ extern "C" sre_status sre_adjoint(sre_engine *e, const double *dLdE,
   a.dLdF = d_dLdF;
   a.tail_idx = e->d_tail_idx;
   a.use_tail = use_tail ? 1 : 0;
-  a.wave_part = d_wp;
+  a.wave_part = reinterpret_cast<long long *>(d_wp);
   a.tail_w = d_tw;
-  launch_adjoint_pairs(a, e->precision, e->list_lanes, d_out, e->stream);
-  if (use_tail)
+  launch_adjoint_pairs(a, e->precision, e->list_lanes, reinterpret_cast<long long *>(d_out),
+                       e->stream);
+  if (use_tail) {
+    launch_tail_weights(e->n_atoms, n_entries, e->d_tail_idx, e->sb.val_a, e->d_pos4,
+                        d_dLdF, d_dLdE, reinterpret_cast<long long *>(d_tw + 2 * n_entries),
+                        d_tw, e->stream);
     launch_adjoint_tail(e->tail_bins, e->d_tail_entry_rep, e->d_reps, nt,
                         e->d_paramsd, e->d_rc, e->d_rho, d_tw, d_contrib, d_grc,
                         d_tout, e->stream);
+  }
   HIP_TRY(hipGetLastError());
   std::vector<double> res(n_res);
   HIP_TRY(hipMemcpyAsync(res.data(), d_out, sizeof(double) * n_res,
                          hipMemcpyDeviceToHost, e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
-  const double *out = res.data(), *tout = out + 2 * npt, *grc = tout + 2 * npt;
+  const double *tout = res.data() + 4 * npt, *grc = tout + 2 * npt;
   if (s != SRE_OK) return s;
+  // pair part from the exact term totals T1..T4 (scale 2^32):
+  // d/d eps = 4 T1 - 24 T2,  d/d sigma = eps / sigma (24 T3 - 144 T4)
+  std::vector<double> out(2 * npt);
+  for (int p = 0; p < npt; ++p) {
+    double T[4];
+    for (int q = 0; q < 4; ++q) {
+      long long v;
+      memcpy(&v, &res[(size_t)q * npt + p], sizeof v);
+      T[q] = (double)v * (1.0 / 4294967296.0);
+    }
+    out[p] = 4.0 * T[0] - 24.0 * T[1];
+    out[npt + p] = (double)e->eps[p] / (double)e->sig[p] * (24.0 * T[2] - 144.0 * T[3]);
+  }
   // Values are per ordered type pair; a symmetric parameter (a,b) = (b,a)
   // collects both orders.
   for (int pa = 0; pa < nt; ++pa)

@@ -252,30 +252,8 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
 // pair component c is already c*2^32 in FP32.
 //
-// Fixed-point sums: each lane keeps, per component, an FP64 accumulator
-// started at C = 1.5*2^52.  While it stays in [2^52, 2^53) the representable
-// values are exactly the integers, and with the FP64 rounding mode set to
-// round toward zero, acc + c rounds to acc + floor(c): one conversion and
-// one add per component give the exact integer sum C + sum floor(c), the same
-// in any order.  (floor instead of round-to-nearest: a bias of half a unit,
-// 2^-33, per pair component.)  The lane value is bits(acc) - bits(C).
-// The rounding mode is switched by hand around the pair loop, and the
-// conversions and adds that depend on it are written as one asm statement
-// each, so that the compiler, which assumes the default mode for every FP64
-// instruction it emits, neither places its own FP64 code under the switched
-// mode nor switches the mode back in front of these adds.
-constexpr double kAccBase = 6755399441055744.0;   // 1.5 * 2^52
-__device__ __forceinline__ void acc_floor(double &acc, float c) {
-  double t;
-  asm("v_cvt_f64_f32 %1, %2\n\tv_add_f64 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(c));
-}
-// MODE register field 3:2 (FP64/FP16 rounding): 3 = toward zero, 0 = nearest even
-__device__ __forceinline__ void round_f64_toward_zero() {
-  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 3" ::: "memory");
-}
-__device__ __forceinline__ void round_f64_nearest() {
-  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0" ::: "memory");
-}
+// Fixed-point sums: exact FP64 accumulators under round-toward-zero
+// (acc_floor, sre_internal.h); the lane value is bits(acc) - bits(C).
 template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // the position refresh of this step has read the counter (stream order)

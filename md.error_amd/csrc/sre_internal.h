@@ -223,6 +223,31 @@ __device__ __forceinline__ int64_t to_fix(double v) {
   return (int64_t)(fix_bits(v) - magic_bits());
 }
 
+// Exact integer sums of FP32 terms: a lane keeps an FP64 accumulator
+// started at C = 1.5*2^52.  While it stays in [2^52, 2^53) the representable
+// values are exactly the integers, and with the FP64 rounding mode set to
+// round toward zero, acc + c rounds to acc + floor(c): one conversion and
+// one add per term give the exact integer sum C + sum floor(c), the same in
+// any order (floor instead of round-to-nearest: a bias of half a unit per
+// term).  The lane value is bits(acc) - bits(C).
+// The rounding mode is switched by hand around the pair loop, and the
+// conversions and adds that depend on it are written as one asm statement
+// each, so that the compiler, which assumes the default mode for every FP64
+// instruction it emits, neither places its own FP64 code under the switched
+// mode nor switches the mode back in front of these adds.
+constexpr double kAccBase = 6755399441055744.0;   // 1.5 * 2^52
+__device__ __forceinline__ void acc_floor(double &acc, float c) {
+  double t;
+  asm("v_cvt_f64_f32 %1, %2\n\tv_add_f64 %0, %0, %1" : "+v"(acc), "=&v"(t) : "v"(c));
+}
+// MODE register field 3:2 (FP64/FP16 rounding): 3 = toward zero, 0 = nearest even
+__device__ __forceinline__ void round_f64_toward_zero() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 3" ::: "memory");
+}
+__device__ __forceinline__ void round_f64_nearest() {
+  asm volatile("s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0" ::: "memory");
+}
+
 // list slot of entry k of sorted atom s: waves of 64/T atoms, entry rows of 64
 __host__ __device__ inline size_t list_slot(int s, int k, int T, int cap) {
   int A = kWave / T;
@@ -428,6 +453,8 @@ void launch_tail_exact(int n, int n_types, const uint4 *pos4,
 
 struct AdjointArgs {
   int n, n_types, cap, n_rep;
+  size_t list_len;         // ints allocated for list
+  const int2 *wave_len;    // per wavefront (longest list, total) of the same tiling, or NULL
   const uint4 *pos4;
   const float *rcut;
   const int *nnb;
@@ -439,13 +466,21 @@ struct AdjointArgs {
   const double *dLdF;      // caller order [N][3] or NULL
   const int *tail_idx;
   int use_tail;
-  double *wave_part;       // value-major: [2 * n_types^2][launched waves]
+  long long *wave_part;    // value-major: [4 * n_types^2][launched waves], scale 2^32
   int64_t *tail_w;         // per tail entry: (sum g_z, sum dLdE), scale 2^32
 };
 int adjoint_grid_waves(int n, int lanes);
-// pair part: out[2*npt] (d/d eps then d/d sigma per ordered type pair)
+// pair part: out[4*npt], the integer totals (scale 2^32) of the four pair
+// terms per ordered type pair, term-major (k_adjoint.hip); with them
+// d/d eps = 4 T1 - 24 T2 and d/d sigma = eps / sigma (24 T3 - 144 T4)
 void launch_adjoint_pairs(const AdjointArgs &a, int precision, int lanes,
-                          double *out, hipStream_t st);
+                          long long *out, hipStream_t st);
+// tail weights (sum of dLdF_z, sum of dLdE per tail entry, scale 2^32) into
+// tail_w, exact; scratch holds tail_w_scratch(n_entries) integers
+size_t tail_w_scratch(int n_entries);
+void launch_tail_weights(int n, int n_entries, const int *tail_idx, const uint32_t *perm,
+                         const uint4 *pos4, const double *dLdF, const double *dLdE,
+                         long long *scratch, int64_t *tail_w, hipStream_t st);
 // tail part: g_rc per rc lattice entry, out[2*npt] as above
 void launch_adjoint_tail(int n_bins, const int *entry_rep, const RepInfo *reps,
                          int n_types, const PairParamD *params,

@@ -295,6 +295,26 @@ def test_adjoint_matches_oracle_gradient(torch_cuda, binary, prec):
     np.testing.assert_allclose(grc, trc, rtol=1e-6, atol=1e-6 * np.abs(trc).max())
 
 
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_adjoint_bitwise_independent_of_tuning(torch_cuda, binary, prec):
+    """The adjoint's per-pair terms are summed as integers, so its gradients
+    are the same bits whatever the lanes per atom or block size."""
+    import torch
+    w = binary
+    g = torch.from_numpy(np.random.default_rng(3).normal(size=(w.n, 3))).cuda()
+    res = []
+    for lanes, block in ((4, 256), (8, 128), (2, 64)):
+        e = Engine(n_types=2, skin=w.skin, precision=prec)
+        w.setup(e, tail=True)
+        e.set_tuning(force_lanes=lanes, build_lanes=lanes, block=block)
+        e.rebuild()
+        res.append(e.adjoint(dLdE=[0.3], dLdF=g, tail=True, n_rc=w.rc_tables[0].size))
+        e.close()
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(a, b)
+
+
 def test_overlapping_atoms_raise_force_range(torch_cuda):
     """Two atoms 0.3 sigma apart: their pair force leaves the range where the
     fixed-point sums are exact; the compute finishes and the read raises."""
