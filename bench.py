@@ -300,6 +300,29 @@ def cpu_baseline(w, budget_s, every, configs):
     return out
 
 
+def sweep_launch(e, n=200):
+    """Back-to-back force kernel time (us, HIP events) of the flat launch (one
+    wavefront per tile) and of persistent launches of k wavefronts per SIMD
+    (each taking a contiguous chunk of tiles); the fixed-point forces must
+    not change."""
+    import numpy as np
+    base = e.tuning()
+    out = {}
+    f0 = None
+    for k in (0, 1, 2, 4, 8):
+        e.set_tuning(persistent=k)
+        e.time_compute(20, stats=True, tail=True)
+        out[str(k)] = round(e.time_compute(n, stats=True, tail=True) * 1e3, 3)
+        e.compute(stats=True, tail=True)
+        f = e.forces_fixed()
+        if f0 is None:
+            f0 = f
+        elif not np.array_equal(f, f0):
+            raise RuntimeError(f"persistent launch {k} changed the forces")
+    e.set_tuning(**base)
+    return {"force_kernel_us_by_persistent_waves_per_simd": out}
+
+
 def force_time_distribution(e, torch, stream, n=200):
     """Per-launch force kernel durations (HIP events around each launch on the
     engine stream): median, p10, p90 in microseconds."""
@@ -450,6 +473,9 @@ def main():
                     help="skip timing the adaptive-cutoff producer")
     ap.add_argument("--no-md", action="store_true",
                     help="skip timing the device MD loop (VV + NHC2 + COM removal)")
+    ap.add_argument("--sweep-launch", action="store_true",
+                    help="also time the mixed force kernel as a persistent launch of "
+                         "1, 2, 4, 8 wavefronts per SIMD against the flat launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -546,6 +572,8 @@ def main():
         result["autotune_s"] = round(mixed["autotune_s"], 2)
     if args.adjoint:
         result["adjoint"] = time_adjoint(mixed["engine"], w, torch)
+    if args.sweep_launch:
+        result["launch_sweep"] = sweep_launch(mixed["engine"])
     if not args.no_arc and len(w.n_atoms) == 1:
         try:
             result["arc"] = time_arc(w, torch, cpu=not args.no_cpu)

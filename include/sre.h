@@ -234,6 +234,10 @@ typedef struct {
   int force_unroll;     /* list rows per group in the mixed force kernel: 1, 2, 4;
                            FP64 form: 1 = the mixed kernel's pipelined loop,
                            2 or 4 = the plain per-lane loop (same sums) */
+  int persistent;       /* mixed force kernel: 0 = one wavefront per tile of 64/lanes
+                           atoms (flat launch over all replicas); k = 1..16: a
+                           persistent launch of k wavefronts per SIMD, each taking a
+                           contiguous chunk of tiles (same sums) */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */

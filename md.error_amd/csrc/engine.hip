@@ -52,7 +52,8 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2, 0};
+  int n_simd = 1024;               // SIMDs of the device (4 per CU)
 
   // system
   int n_rep = 0, n_atoms = 0;
@@ -274,6 +275,12 @@ extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
   if (hipGetDevice(&e->device) != hipSuccess) {
     delete e;
     return fail(SRE_ERR_HIP, "hipGetDevice failed (no GPU?)");
+  }
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) ==
+            hipSuccess && cus > 0)
+      e->n_simd = 4 * cus;
   }
   if (opt->stream) {
     e->stream = (hipStream_t)opt->stream;
@@ -1057,6 +1064,11 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
                    ? e->d_build_part : nullptr;
   a.xcd_remap = e->tune.xcd_remap;
   a.has_shift = e->has_shift ? 1 : 0;
+  // persistent launch: whole blocks of the tuned size
+  const int per_block = e->tune.block / kWave;
+  a.persist_waves = e->tune.persistent > 0
+                        ? (e->tune.persistent * e->n_simd + per_block - 1) / per_block * per_block
+                        : 0;
   return a;
 }
 
@@ -1453,6 +1465,8 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "xcd_remap must be 0 or 1");
   if (t->force_unroll != 1 && t->force_unroll != 2 && t->force_unroll != 4)
     return fail(SRE_ERR_INVALID, "force_unroll must be 1, 2 or 4");
+  if (t->persistent < 0 || t->persistent > 16)
+    return fail(SRE_ERR_INVALID, "persistent must be in [0, 16]");
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
@@ -1544,9 +1558,9 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
         *bar = 0;
         if (strcmp(line, sig) == 0) {
           sre_tuning t;
-          if (sscanf(bar + 1, "%d %d %d %f %f %d %d %d", &t.force_lanes,
+          if (sscanf(bar + 1, "%d %d %d %f %f %d %d %d %d", &t.force_lanes,
                      &t.build_lanes, &t.block, &t.cell_xy, &t.cell_z,
-                     &t.build_mode, &t.xcd_remap, &t.force_unroll) == 8) {
+                     &t.build_mode, &t.xcd_remap, &t.force_unroll, &t.persistent) == 9) {
             fclose(f);
             sre_status s = sre_set_tuning(e, &t);
             if (s != SRE_OK) return s;
@@ -1618,6 +1632,30 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
       }
       if (ms < best_f) { best_f = ms; bestf = t; }
     }
+  // phase 3 (mixed form): flat or persistent launch of the best kernel
+  if (e->precision == SRE_PREC_MIXED) {
+    const sre_tuning flat = bestf;
+    const int persist[] = {1, 2, 4, 8};
+    for (int pw : persist) {
+      sre_tuning t = flat;
+      t.persistent = pw;
+      sre_status s = sre_set_tuning(e, &t);
+      if (s != SRE_OK) return s;
+      s = do_rebuild(e);
+      if (s != SRE_OK) return s;
+      double ms;
+      s = sre_time_compute(e, flags, 2, &ms);
+      if (s != SRE_OK) return s;
+      s = sre_time_compute(e, flags, 10, &ms);
+      if (s != SRE_OK) return s;
+      cur.resize((size_t)3 * e->n_atoms);
+      s = sre_get_forces_fixed(e, cur.data());
+      if (s != SRE_OK) return s;
+      if (ref != cur)
+        return fail(SRE_ERR_INVALID, "autotune: persistent launch changed the forces (bug)");
+      if (ms < best_f) { best_f = ms; bestf = t; }
+    }
+  }
   sre_status s = sre_set_tuning(e, &bestf);
   if (s != SRE_OK) return s;
   s = do_rebuild(e);
@@ -1625,10 +1663,10 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
   if (cache_path) {
     FILE *f = fopen(cache_path, "a");
     if (f) {
-      fprintf(f, "%s|%d %d %d %.4f %.4f %d %d %d\n", sig, bestf.force_lanes,
+      fprintf(f, "%s|%d %d %d %.4f %.4f %d %d %d %d\n", sig, bestf.force_lanes,
               bestf.build_lanes, bestf.block, (double)bestf.cell_xy,
               (double)bestf.cell_z, bestf.build_mode, bestf.xcd_remap,
-              bestf.force_unroll);
+              bestf.force_unroll, bestf.persistent);
       fclose(f);
     }
   }

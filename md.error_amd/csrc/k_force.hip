@@ -254,21 +254,15 @@ __device__ __forceinline__ int wave_max_int(int v) {
 //
 // Fixed-point sums: exact FP64 accumulators under round-toward-zero
 // (acc_floor, sre_internal.h); the lane value is bits(acc) - bits(C).
+//
+// One wave-tile (64/T consecutive sorted atoms) of the force evaluation;
+// `wave` is the tile index.
 template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
-__global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
-  // the position refresh of this step has read the counter (stream order)
-  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
-  __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
-  if (MULTI) {
-    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
-      sp[t] = a.params[t];
-    __syncthreads();
-  }
+__device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *sp,
+                                           const int wave) {
   constexpr int A = kWave / T;
   const int lane = threadIdx.x & (kWave - 1);
   const int l = lane % T;
-  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + lane / T;
   const bool act = s < a.n;
   const int s0 = wave * A;
@@ -436,6 +430,33 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       d[1] = make_double2(Wy, Wz);
     }
   }
+}
+
+// Flat launch (persist_waves == 0): one wavefront per tile, blocks mapped to
+// XCDs by xcd_block.  Persistent launch: persist_waves wavefronts, each
+// taking a contiguous chunk of tiles (wavefront order after the same block
+// map, so each XCD works through one contiguous range of atoms).
+template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
+__global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
+  // the position refresh of this step has read the counter (stream order)
+  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
+  __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
+  if (MULTI) {
+    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
+      sp[t] = a.params[t];
+    __syncthreads();
+  }
+  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane((blk * blockDim.x + threadIdx.x) / kWave);
+  if (a.persist_waves <= 0) {
+    force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, wave);
+    return;
+  }
+  constexpr int A = kWave / T;
+  const int n_tiles = (a.n + A - 1) / A;
+  const int per = (n_tiles + a.persist_waves - 1) / a.persist_waves;
+  const int t0 = wave * per, t1 = min(n_tiles, t0 + per);
+  for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, t);
 }
 
 // --------------------------------------------------------------- FP64 ----
@@ -694,8 +715,9 @@ template <int T, bool STATS, bool TAIL>
 static void launch_force_t(int precision, int unroll, int block,
                            const ForceArgs &a, hipStream_t st) {
   const int waves = force_num_waves(a.n, T);
-  const int grid = (waves * kWave + block - 1) / block;
+  int grid = (waves * kWave + block - 1) / block;
   const bool multi = a.n_types > 1;
+  if (precision == 0 && a.persist_waves > 0) grid = a.persist_waves * kWave / block;
   if (precision == 0) {
     if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
     else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);

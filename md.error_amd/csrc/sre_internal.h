@@ -411,6 +411,8 @@ struct ForceArgs {
                                // list build of the same tiling, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
   int has_shift;               // some pair type has a nonzero energy shift
+  int persist_waves;           // mixed form: > 0 = persistent launch of that many
+                               // wavefronts, each taking a contiguous chunk of tiles
 };
 // unroll: list rows per group of the mixed kernel (1, 2 or 4)
 void launch_force(int precision, int lanes, int unroll, int block, bool stats,

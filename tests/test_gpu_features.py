@@ -97,6 +97,14 @@ def test_batched_replicas_match_oracle_per_replica(torch_cuda):
         assert rel_force_err(F[sl], ref["F"]) <= TOL_F_MIXED, r
         assert abs(st[r, 0] - ref["E"].sum()) / np.abs(ref["E"]).sum() <= 1e-6, r
         np.testing.assert_allclose(st[r, 1:], ref["W"].sum(0), rtol=1e-6)
+    # the persistent launch (wavefronts taking chunks of tiles across replica
+    # boundaries) gives the same fixed-point forces and per-replica statistics
+    f0 = e.forces_fixed()
+    for pw in (1, 4):
+        e.set_tuning(persistent=pw)
+        e.compute(stats=True, tail=True)
+        assert np.array_equal(e.forces_fixed(), f0), pw
+        np.testing.assert_allclose(e.stats(), st, rtol=1e-12)
 
 
 def test_update_positions_and_skin(torch_cuda, slab):

@@ -114,18 +114,23 @@ def test_fp64_form_matches_oracle(torch_cuda, wname, request):
 
 def test_forces_bitwise_independent_of_tuning(torch_cuda, slab16):
     base = None
-    cases = [(4, 8, 0.5, 0.5, 256, 0, 0, 4), (1, 4, 0.35, 0.25, 64, 0, 1, 1),
-             (2, 16, 0.7, 1.0, 128, 1, 0, 2), (8, 2, 1.0, 0.35, 128, 1, 1, 1),
-             (16, 1, 0.5, 0.5, 256, 1, 1, 2), (4, 8, 1.0, 0.35, 64, 1, 1, 4),
-             (4, 4, 1.0, 0.35, 256, 0, 1, 2), (8, 4, 1.0, 0.35, 256, 0, 1, 1)]
-    for fl, bl, cxy, cz, blk, bm, xr, fu in cases:
+    # (lanes, build lanes, cells xy, z, block, build mode, XCD map, unroll,
+    #  persistent wavefronts per SIMD)
+    cases = [(4, 8, 0.5, 0.5, 256, 0, 0, 4, 0), (1, 4, 0.35, 0.25, 64, 0, 1, 1, 0),
+             (2, 16, 0.7, 1.0, 128, 1, 0, 2, 0), (8, 2, 1.0, 0.35, 128, 1, 1, 1, 0),
+             (16, 1, 0.5, 0.5, 256, 1, 1, 2, 0), (4, 8, 1.0, 0.35, 64, 1, 1, 4, 0),
+             (4, 4, 1.0, 0.35, 256, 0, 1, 2, 0), (8, 4, 1.0, 0.35, 256, 0, 1, 1, 0),
+             (4, 4, 1.0, 0.35, 256, 0, 1, 2, 1), (2, 2, 0.7, 0.35, 128, 0, 0, 1, 3),
+             (8, 8, 1.0, 0.5, 64, 0, 1, 4, 16)]
+    for fl, bl, cxy, cz, blk, bm, xr, fu, pw in cases:
         e = engine_for(slab16, force_lanes=fl, build_lanes=bl, cell_xy=cxy, cell_z=cz,
-                       block=blk, build_mode=bm, xcd_remap=xr, force_unroll=fu)
+                       block=blk, build_mode=bm, xcd_remap=xr, force_unroll=fu,
+                       persistent=pw)
         f = e.forces_fixed()
         if base is None:
             base = f
         else:
-            assert np.array_equal(f, base), (fl, bl, cxy, cz, blk, bm, xr, fu)
+            assert np.array_equal(f, base), (fl, bl, cxy, cz, blk, bm, xr, fu, pw)
 
 
 def test_fp64_pipelined_equals_plain(torch_cuda, slab16):
