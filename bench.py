@@ -39,6 +39,7 @@ PERTURB_SIGMA = 0.01
 SEED = 7
 THERMO_EVERY = 100   # energy/virial read back like the reference's thermo output
                      # (H/app/lj.nhc-xtc.cu:198-224)
+WARM_SECONDS = 0.25  # least wall time of the untimed warm-up of each form
 
 
 def log(*a):
@@ -117,9 +118,19 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
 
     # untimed warm-up: at least one whole read-back block, rounded up to
     # whole blocks, so that every graph the timed steps replay is captured,
-    # every buffer is sized and the timed steps start just after a read-back
+    # every buffer is sized and the timed steps start just after a read-back;
+    # then further whole blocks until the GPU has worked for WARM_SECONDS, so
+    # that a short timed run does not start on clocks still ramping up from
+    # the idle host-side set-up (measured: the first 20-step window after
+    # the set-up runs 5-10 % slower than later ones)
+    t_warm = time.perf_counter()
     warm = -(-max(warmup, THERMO_EVERY) // THERMO_EVERY) * THERMO_EVERY
     segment_loop(warm)
+    torch.cuda.synchronize()
+    while time.perf_counter() - t_warm < WARM_SECONDS:
+        segment_loop(THERMO_EVERY)
+        warm += THERMO_EVERY
+        torch.cuda.synchronize()
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)

@@ -41,6 +41,7 @@ sre_status sre::fail(sre_status s, const char *fmt, ...) {
 
 constexpr int kStepGraphSlots = 4;   // cached step graphs (flags, step count)
 constexpr int kMaxGraphSteps = 64;   // steps per captured graph
+constexpr int kRebuildLog = 1024;    // rebuild-log entries (segments per queue)
 // internal step-graph flag: the K steps are MD steps (thermostat factors,
 // kick + drift refresh, force, kick) and SRE_STATS applies to the last only
 constexpr unsigned kGraphMD = 1u << 16;
@@ -134,11 +135,16 @@ struct sre_engine {
   // [0] longest list, [1] total length, [2] low word: max displacement^2
   unsigned long long *d_counters = nullptr;
   uint64_t *h_info = nullptr;               // pinned readback (4 words)
-  // sre_run_segments: per-segment rebuild info (3 words each), device and
-  // pinned host copies
-  unsigned long long *d_seg_info = nullptr;
-  uint64_t *h_seg_info = nullptr;
-  int seg_cap = 0;
+  // rebuild log: every launch of the captured rebuild writes its (longest,
+  // total, displacement^2 bits) to entry (launch number % kRebuildLog) of
+  // this pinned host array, mapped for the device (d_log); d_log_idx is the
+  // device's launch counter and log_launches the host's count of the same
+  // launches, so a caller reads an entry after one synchronisation, with no
+  // copy queued per rebuild
+  uint64_t *h_log = nullptr;
+  unsigned long long *d_log = nullptr;
+  unsigned *d_log_idx = nullptr;
+  uint64_t log_launches = 0;
   float last_disp2 = 0.f;                   // max displacement^2 before the last rebuild
   int max_count = 0;
   int64_t n_list = 0;
@@ -226,8 +232,8 @@ sre_engine::~sre_engine() {
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
-  if (h_seg_info) (void)hipHostFree(h_seg_info);
-  dfree(d_seg_info);
+  if (h_log) (void)hipHostFree(h_log);
+  dfree(d_log_idx);
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
@@ -300,6 +306,12 @@ extern "C" sre_status sre_create(sre_engine **out, const sre_options *opt) {
   if (r == hipSuccess) r = dalloc(&e->d_counters, 3);
   if (r == hipSuccess) r = hipMemsetAsync(e->d_counters, 0, 3 * sizeof(uint64_t), e->stream);
   if (r == hipSuccess) r = hipHostMalloc((void **)&e->h_info, 4 * sizeof(uint64_t), hipHostMallocDefault);
+  if (r == hipSuccess)
+    r = hipHostMalloc((void **)&e->h_log, sizeof(uint64_t) * 3 * kRebuildLog,
+                      hipHostMallocMapped | hipHostMallocCoherent);
+  if (r == hipSuccess) r = hipHostGetDevicePointer((void **)&e->d_log, e->h_log, 0);
+  if (r == hipSuccess) r = dalloc(&e->d_log_idx, 1);
+  if (r == hipSuccess) r = hipMemsetAsync(e->d_log_idx, 0, sizeof(unsigned), e->stream);
   if (r == hipSuccess) r = dalloc(&e->d_params, kMaxPairTypes);
   if (r == hipSuccess) r = dalloc(&e->d_paramsd, kMaxPairTypes);
   if (r != hipSuccess) {
@@ -874,7 +886,7 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
                         e->d_paramsd,
                         e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
-                        e->d_counters, e->d_skin_part};
+                        e->d_counters, e->d_skin_part, e->d_log, e->d_log_idx};
   put(ptrs, sizeof ptrs);
   return v;
 }
@@ -911,7 +923,17 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   enqueue_build(e, st);
   launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_part_count(N),
                       e->d_skin_part, 1, reinterpret_cast<unsigned *>(e->d_counters + 2),
-                      st);
+                      st, e->d_log, e->d_log_idx, kRebuildLog);
+}
+
+// Queues one launch of the captured rebuild; returns its rebuild-log entry
+// (valid once the stream has been synchronised).
+static hipError_t launch_rebuild(sre_engine *e, const uint64_t **entry) {
+  hipError_t r = hipGraphLaunch(e->rebuild_exec, e->stream);
+  if (r != hipSuccess) return r;
+  *entry = e->h_log + 3 * (size_t)(e->log_launches % kRebuildLog);
+  ++e->log_launches;
+  return hipSuccess;
 }
 
 static sre_status do_rebuild(sre_engine *e) {
@@ -970,10 +992,10 @@ static sre_status do_rebuild(sre_engine *e) {
     }
     e->rebuild_sig = std::move(sig);
   }
-  HIP_TRY(hipGraphLaunch(e->rebuild_exec, st));
-  HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 3 * sizeof(uint64_t),
-                         hipMemcpyDeviceToHost, st));
+  const uint64_t *info = nullptr;
+  HIP_TRY(launch_rebuild(e, &info));
   HIP_TRY(hipStreamSynchronize(st));
+  for (int k = 0; k < 3; ++k) e->h_info[k] = info[k];
   {
     const unsigned bits = (unsigned)(e->h_info[2] & 0xffffffffu);
     memcpy(&e->last_disp2, &bits, sizeof(float));
@@ -1789,9 +1811,66 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
 }
 
 // Segments of `steps` graph steps each followed by a list rebuild, queued
-// back to back with one synchronisation at the end: each rebuild's longest
-// list, total length and the largest displacement of the segment before it
-// are copied on the device and checked once all segments have run.
+// back to back with one synchronisation per kRebuildLog segments: each
+// rebuild logs its longest list, total length and the largest displacement
+// of the segment before it (the rebuild log, host memory), checked once the
+// queued segments have run.
+static sre_status run_segment_queue(sre_engine *e, unsigned flags, int g0, int n_seg,
+                                    int steps, int64_t *n_list, float *d2_out) {
+  std::vector<const uint64_t *> info(n_seg);
+  for (int g = 0; g < n_seg; ++g) {
+    sre_status s = sre_graph_run(e, flags, steps);
+    if (s != SRE_OK) return s;
+    HIP_TRY(launch_rebuild(e, &info[g]));
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  e->built = true;
+  e->md_since_rebuild = 0;
+  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  float d2max = 0.f;
+  int worst = -1;
+  std::vector<float> d2(n_seg);
+  for (int g = 0; g < n_seg; ++g) {
+    const uint64_t *h = info[g];
+    const unsigned bits = (unsigned)(h[2] & 0xffffffffu);
+    memcpy(&d2[g], &bits, sizeof(float));
+    d2max = std::max(d2max, d2[g]);
+    if (n_list) n_list[g] = (int64_t)h[1];
+    if (worst < 0 && (int)h[0] > e->cap) worst = g;
+  }
+  const uint64_t *last = info[n_seg - 1];
+  e->max_count = (int)last[0];
+  e->n_list = (int64_t)last[1];
+  e->last_disp2 = d2max;
+  *d2_out = d2max;
+  if (worst >= 0) {
+    // lists were cut at the capacity from rebuild `worst` on: regrow now so
+    // that the next call is exact, and report the segments as invalid
+    int mc = 0;
+    for (int g = worst; g < n_seg; ++g) mc = std::max(mc, (int)info[g][0]);
+    const int old_cap = e->cap;
+    sre_status s = ensure_list(
+        e, round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes));
+    if (s != SRE_OK) return s;
+    s = do_rebuild(e);
+    if (s != SRE_OK) return s;
+    return fail(SRE_ERR_LIST_OVERFLOW,
+                "neighbor list overflow at the rebuild ending segment %d (%d > %d); "
+                "the capacity has been regrown to %d, the later segments are invalid",
+                g0 + worst, mc, old_cap, e->cap);
+  }
+  // a segment in which an atom moved more than skin/2 from its rebuild
+  // position may have missed pairs: its forces are invalid (the lists are
+  // rebuilt, so the next call starts from a consistent state)
+  const float half = 0.5f * e->skin;
+  for (int g = 0; g < n_seg; ++g)
+    if (d2[g] > half * half)
+      return fail(SRE_ERR_SKIN,
+                  "segment %d: max displacement %g exceeds skin/2 = %g; its forces "
+                  "may have missed pairs", g0 + g, sqrt((double)d2[g]), (double)half);
+  return SRE_OK;
+}
+
 extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segments,
                                        int steps, int64_t *n_list, float *max_disp) {
   if (!e || n_segments < 0 || steps < 1)
@@ -1810,78 +1889,17 @@ extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segm
     sre_status s = do_rebuild(e);
     if (s != SRE_OK) return s;
   }
-  if (e->seg_cap < n_segments) {
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    dfree(e->d_seg_info);
-    if (e->h_seg_info) (void)hipHostFree(e->h_seg_info);
-    e->h_seg_info = nullptr;
-    e->seg_cap = 0;
-    HIP_TRY(dalloc(&e->d_seg_info, (size_t)3 * n_segments));
-    HIP_TRY(hipHostMalloc((void **)&e->h_seg_info, sizeof(uint64_t) * 3 * n_segments,
-                          hipHostMallocDefault));
-    e->seg_cap = n_segments;
-  }
-  for (int g = 0; g < n_segments; ++g) {
-    sre_status s = sre_graph_run(e, flags, steps);
-    if (s != SRE_OK) return s;
-    HIP_TRY(hipGraphLaunch(e->rebuild_exec, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_seg_info + 3 * (size_t)g, e->d_counters,
-                           3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                           e->stream));
-  }
-  HIP_TRY(hipMemcpyAsync(e->h_seg_info, e->d_seg_info,
-                         sizeof(uint64_t) * 3 * n_segments, hipMemcpyDeviceToHost,
-                         e->stream));
-  HIP_TRY(hipStreamSynchronize(e->stream));
-  e->built = true;
-  e->md_since_rebuild = 0;
-  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   float d2max = 0.f;
-  int worst = -1;
-  for (int g = 0; g < n_segments; ++g) {
-    const uint64_t *h = e->h_seg_info + 3 * (size_t)g;
-    const unsigned bits = (unsigned)(h[2] & 0xffffffffu);
-    float d2;
-    memcpy(&d2, &bits, sizeof d2);
+  for (int g0 = 0; g0 < n_segments; g0 += kRebuildLog) {
+    const int m = std::min(kRebuildLog, n_segments - g0);
+    float d2 = 0.f;
+    sre_status s = run_segment_queue(e, flags, g0, m, steps,
+                                     n_list ? n_list + g0 : nullptr, &d2);
+    if (s != SRE_OK) return s;
     d2max = std::max(d2max, d2);
-    if (n_list) n_list[g] = (int64_t)h[1];
-    if (worst < 0 && (int)h[0] > e->cap) worst = g;
   }
-  const uint64_t *last = e->h_seg_info + 3 * (size_t)(n_segments - 1);
-  e->max_count = (int)last[0];
-  e->n_list = (int64_t)last[1];
   e->last_disp2 = d2max;
   if (max_disp) *max_disp = sqrtf(d2max);
-  if (worst >= 0) {
-    // lists were cut at the capacity from rebuild `worst` on: regrow now so
-    // that the next call is exact, and report the segments as invalid
-    int mc = 0;
-    for (int g = worst; g < n_segments; ++g)
-      mc = std::max(mc, (int)e->h_seg_info[3 * (size_t)g]);
-    const int old_cap = e->cap;
-    sre_status s = ensure_list(
-        e, round_up(std::max(mc + 16, (int)(mc * 1.2)), e->tune.force_lanes));
-    if (s != SRE_OK) return s;
-    s = do_rebuild(e);
-    if (s != SRE_OK) return s;
-    return fail(SRE_ERR_LIST_OVERFLOW,
-                "neighbor list overflow at the rebuild ending segment %d (%d > %d); "
-                "the capacity has been regrown to %d, the later segments are invalid",
-                worst, mc, old_cap, e->cap);
-  }
-  // a segment in which an atom moved more than skin/2 from its rebuild
-  // position may have missed pairs: its forces are invalid (the lists are
-  // rebuilt, so the next call starts from a consistent state)
-  const float half = 0.5f * e->skin;
-  for (int g = 0; g < n_segments; ++g) {
-    const unsigned bits = (unsigned)(e->h_seg_info[3 * (size_t)g + 2] & 0xffffffffu);
-    float d2;
-    memcpy(&d2, &bits, sizeof d2);
-    if (d2 > half * half)
-      return fail(SRE_ERR_SKIN,
-                  "segment %d: max displacement %g exceeds skin/2 = %g; its forces "
-                  "may have missed pairs", g, sqrt((double)d2), (double)half);
-  }
   return SRE_OK;
 }
 

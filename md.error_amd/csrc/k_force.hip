@@ -158,7 +158,8 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
 __global__ __launch_bounds__(1024) void k_rebuild_info(
     int n_build, const int2 *__restrict__ build_part,
     unsigned long long *counters, int n_skin, float *__restrict__ skin_part,
-    int clear_skin, unsigned *skin_bits) {
+    int clear_skin, unsigned *skin_bits, unsigned long long *log,
+    unsigned *log_idx, int log_cap) {
   __shared__ unsigned long long s_tot[1024 / kWave];
   __shared__ int s_max[1024 / kWave];
   __shared__ float s_d2[1024 / kWave];
@@ -198,16 +199,28 @@ __global__ __launch_bounds__(1024) void k_rebuild_info(
       counters[1] = tot;
     }
     if (n_skin > 0) *skin_bits = __float_as_uint(d2);
+    if (log) {
+      // the next entry of the rebuild log (host memory): the host counts
+      // the logged launches it queues, so it knows which entry each wrote
+      const unsigned k = *log_idx;
+      unsigned long long *dst = log + 3 * (size_t)(k % (unsigned)log_cap);
+      dst[0] = (unsigned long long)mx;
+      dst[1] = tot;
+      dst[2] = (unsigned long long)__float_as_uint(d2);
+      *log_idx = k + 1;
+      __threadfence_system();
+    }
   }
 }
 
 void launch_rebuild_info(int n_build, const int2 *build_part,
                          unsigned long long *counters, int n_skin,
                          float *skin_part, int clear_skin, unsigned *skin_bits,
-                         hipStream_t st) {
+                         hipStream_t st, unsigned long long *log,
+                         unsigned *log_idx, int log_cap) {
   hipLaunchKernelGGL(k_rebuild_info, dim3(1), dim3(1024), 0, st, n_build,
                      build_part, counters, n_skin, skin_part, clear_skin,
-                     skin_bits);
+                     skin_bits, log, log_idx, log_cap);
 }
 
 // ------------------------------------------------------ wave reductions --

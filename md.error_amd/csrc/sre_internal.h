@@ -308,11 +308,14 @@ void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
 // n_build partials (skipped when n_build == 0); *skin_bits = the largest
 // squared displacement (float bits) over the n_skin partials, which are
 // zeroed after reading when clear_skin (skipped when n_skin == 0).  One block,
-// fixed order: no atomics.
+// fixed order: no atomics.  With a log (device-mapped host memory of log_cap
+// entries of three words), the three values (longest, total, displacement^2
+// bits) also go to entry (*log_idx % log_cap), and *log_idx is advanced.
 void launch_rebuild_info(int n_build, const int2 *build_part,
                          unsigned long long *counters, int n_skin,
                          float *skin_part, int clear_skin, unsigned *skin_bits,
-                         hipStream_t st);
+                         hipStream_t st, unsigned long long *log = nullptr,
+                         unsigned *log_idx = nullptr, int log_cap = 0);
 void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                        uint32_t *tmp, hipStream_t st);
 // step_ctr (optional) is advanced by one thread after the positions are read
