@@ -57,7 +57,10 @@ typedef enum {
                                   by sre_set_system, sre_upload_positions, sre_bind_positions
                                   and an MD rollback, so the next compute starts clean */
   SRE_ERR_SKIN = 8,            /* an atom moved more than skin/2 since the last rebuild (fixed-period mode) */
-  SRE_ERR_NOT_READY = 9        /* compute requested before the first rebuild */
+  SRE_ERR_NOT_READY = 9,       /* compute requested before the first rebuild */
+  SRE_ERR_IO = 10,             /* a file could not be opened, read or written
+                                  (MDExcptCannotOpenFile, H/include/MDException.h) */
+  SRE_END_OF_FILE = 11         /* sre_xtc_read: no further frame (not an error) */
 } sre_status;
 
 typedef enum {
@@ -462,6 +465,29 @@ sre_status sre_gen_fcc(int ncell, double a, double jitter, uint64_t seed,
  * sequential addition), seeded.  xyz[4n]. */
 sre_status sre_gen_slab(int n, double lx, double ly, double lz, double h,
                         double rho_l, double dmin, uint64_t seed, float *xyz);
+
+/* ---- xtc trajectories (host only, no GPU use) ----------------------------- */
+
+/* GROMACS xtc frames (compressed coordinates), the format the reference's
+ * driver writes every few steps through libxdrfile: MDSystem::initWriteXtc
+ * (xdrfile_open(name, "w"), H/src/MDSystem_interface.cu:243-263),
+ * writeHostDataXtc (write_xtc(xd, natoms, step, time, box, x, prec),
+ * :265-284) and endWriteXtc (xdrfile_close, :286-290).  Frames carry a
+ * diagonal or general 3x3 box (row-major), natoms x 3 FP32 coordinates,
+ * rounded to 1/precision (1000 when precision <= 0; frames of at most 9
+ * atoms are stored uncompressed).  mode: "w" (truncate), "a" (append) or
+ * "r".  sre_xtc_read returns SRE_END_OF_FILE after the last frame and
+ * SRE_ERR_INVALID when the frame's atom count differs from natoms;
+ * *precision is 0 for an uncompressed frame. */
+typedef struct sre_xtc sre_xtc;
+sre_status sre_xtc_open(const char *path, const char *mode, sre_xtc **out);
+sre_status sre_xtc_write(sre_xtc *f, int natoms, int step, float time,
+                         const float box[9], const float *xyz, float precision);
+sre_status sre_xtc_read(sre_xtc *f, int natoms, int *step, float *time,
+                        float box[9], float *xyz, float *precision);
+/* atom count of the first frame (read_xtc_natoms) */
+sre_status sre_xtc_natoms(const char *path, int *natoms);
+sre_status sre_xtc_close(sre_xtc *f);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsre.so")
 SOURCES = ["k_sort.hip", "k_nlist.hip", "k_force.hip", "k_tail.hip",
            "k_misc.hip", "k_adjoint.hip", "k_md.hip", "k_arc.hip", "engine.hip", "static_path.hip",
-           "gen.cpp"]
+           "gen.cpp", "xtc.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

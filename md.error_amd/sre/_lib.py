@@ -20,6 +20,8 @@ ERR_BOX_TOO_SMALL = 6
 ERR_FORCE_RANGE = 7
 ERR_SKIN = 8
 ERR_NOT_READY = 9
+ERR_IO = 10
+END_OF_FILE = 11
 
 PREC_MIXED = 0
 PREC_FP64 = 1
@@ -134,6 +136,12 @@ SIGNATURES = {
     "sre_arc_pressure_integrals": (_st, [_vp, _f64p, _f64p]),
     "sre_forces_static": (_st, [C.c_int, _vp, C.c_int, _vp, _f32p, C.c_int, _f32p, _f32p,
                                 C.c_int, _f32p, C.c_float, _vp, _vp, _f64p]),
+    "sre_xtc_open": (_st, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "sre_xtc_write": (_st, [_vp, C.c_int, C.c_int, C.c_float, _f32p, _f32p, C.c_float]),
+    "sre_xtc_read": (_st, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float), _f32p,
+                           _f32p, C.POINTER(C.c_float)]),
+    "sre_xtc_natoms": (_st, [C.c_char_p, C.POINTER(C.c_int)]),
+    "sre_xtc_close": (_st, [_vp]),
     "sre_gen_fcc": (_st, [C.c_int, C.c_double, C.c_double, C.c_uint64, _f32p]),
     "sre_gen_slab": (_st, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_double, C.c_double, C.c_uint64, _f32p]),

@@ -25,10 +25,11 @@ refh 1, density sampling every 40 steps):
 Each prints a thermo line every `thermo_feq` steps in its program's column
 layout and writes confstart.gro / confout.gro.  The stretches between thermo
 lines, samples and updates run on the device (`sre_md_run`, adaptive rebuild
-stretches); the programs' xtc trajectory is replaced by a callback,
-on_frame(step, time, xyz, engine), called every `conf_feq` steps with the
-host positions and the engine (whose bound positions an Arc can sample
-directly).  Differences that leave the ensemble unchanged: the centre-of-mass
+stretches).  With xtc=path the programs' xtc trajectory is written (a frame
+at step 0 and every `conf_feq` steps, precision 1000, as writeHostDataXtc);
+a callback on_frame(step, time, xyz, engine) is called at the same steps
+with the host positions and the engine (whose bound positions an Arc can
+sample directly).  Differences that leave the ensemble unchanged: the centre-of-mass
 velocity is removed every step (the uniform program: every 10th), the
 correction force and the per-atom cutoff are looked up from the grids at
 every force evaluation / list rebuild (the programs assign them every 20-40
@@ -111,6 +112,19 @@ class _Run:
         io.write_gro(path, self.positions(), self.box, self.e.velocities(), step=step,
                      time=step * self.dt, resname="lj", atomname="lj")
 
+    def frames(self, xtc, on_frame):
+        """The per-frame callback: the xtc frame (if a path is given), then
+        on_frame; returns (callback or None, file or None)."""
+        if not xtc:
+            return on_frame, None
+        f = io.XtcFile(xtc, "w")
+
+        def frame(step, time, xyz, e):
+            f.write(step, time, self.box, xyz.astype(np.float32), 1000.0)
+            if on_frame is not None:
+                on_frame(step, time, xyz, e)
+        return frame, f
+
     def loop(self, nstep, thermo_feq, conf_feq, out, on_frame, hooks, pc=lambda: None,
              fcorr=lambda: False):
         """hooks: (period, fn(step)) called after every period-th step, after
@@ -147,10 +161,11 @@ class _Run:
 def lj_nhc(conf, nstep, rcut=10.0, ref_T=0.70, tau_T=1.0, dt=0.005, skin=0.49,
            thermo_feq=100, conf_feq=2000, out=sys.stdout, on_frame=None,
            confstart="confstart.gro", confout="confout.gro", autotune=True,
-           fcorr=False, refh=1.0, density_feq=40, update_fc_feq=20000):
+           fcorr=False, refh=1.0, density_feq=40, update_fc_feq=20000, xtc=None):
     """Run the uniform-cutoff (or force-corrected) program; returns the thermo
     rows (float64, one row per line, columns as printed)."""
     r = _Run(conf, rcut, ref_T, tau_T, dt, skin, autotune)
+    on_frame, xf = r.frames(xtc, on_frame)
     if confstart:
         r.write_conf(confstart, 0)
     out.write("# prepare ok, start to run\n")
@@ -171,6 +186,8 @@ def lj_nhc(conf, nstep, rcut=10.0, ref_T=0.70, tau_T=1.0, dt=0.005, skin=0.49,
         hooks = [(density_feq, lambda step: fc.deposit(r.e)), (update_fc_feq, update)]
     rows = r.loop(nstep, thermo_feq, conf_feq, out, on_frame, hooks,
                   fcorr=lambda: on[0])
+    if xf is not None:
+        xf.close()
     if confout:
         r.write_conf(confout, nstep)
     if fc is not None:
@@ -184,7 +201,7 @@ def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.
                  density_feq=40, update_rc_feq=2000, n_refine=0, thermo_feq=100,
                  conf_feq=2000, out=sys.stdout, on_frame=None, confstart="confstart.gro",
                  confout="confout.gro", autotune=True, rc_trajectory=None,
-                 on_update=None, assign_feq=None):
+                 on_update=None, assign_feq=None, xtc=None):
     """Run the adaptive-cutoff program (defaults as its source; the run
     scripts set rcutUpdateFeq 20000 and rcutNumRefine 2).  Returns the thermo
     rows (15 columns as printed) and the last installed rc lattice and its
@@ -195,6 +212,7 @@ def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.
     number reassigns every assign_feq steps without a rebuild, as the program
     does (rcutAssignFeq 40, H/app/lj.nhc-xtc.cu:184-188)."""
     r = _Run(conf, rcut, ref_T, tau_T, dt, skin, autotune)
+    on_frame, xf = r.frames(xtc, on_frame)
     arc = Arc(r.box, refh)
     arc.set_candidates(rcmin, rcmax, rcstep)
     if confstart:
@@ -226,6 +244,8 @@ def lj_nhc_adapt(conf, nstep, rcut=5.0, ref_T=1.10, tau_T=1.0, dt=0.005, skin=0.
     if rtj is not None:
         hooks.append((conf_feq, lambda step: rtj.write(step * dt, last[0])))
     rows = r.loop(nstep, thermo_feq, conf_feq, out, on_frame, hooks, pc=lambda: pc[0])
+    if xf is not None:
+        xf.close()
     if rtj is not None:
         rtj.close()
     if confout:

@@ -53,6 +53,11 @@ class InvalidArgument(SREError, ValueError):
     pass
 
 
+class MDExcptCannotOpenFile(SREError, OSError):
+    """A file could not be opened, read or written (the reference's
+    MDExcptCannotOpenFile)."""
+
+
 _EXC = {
     _lib.ERR_LIST_OVERFLOW: ShortNeighborList,
     _lib.ERR_CELL: ShortCellList,
@@ -62,6 +67,7 @@ _EXC = {
     _lib.ERR_FORCE_RANGE: ForceRange,
     _lib.ERR_SKIN: SkinExceeded,
     _lib.ERR_NOT_READY: NotReady,
+    _lib.ERR_IO: MDExcptCannotOpenFile,
 }
 
 

@@ -19,9 +19,15 @@ tools/simulation.hetero.cutoff), written and read byte for byte alike:
               "%d " (candidate count), "%f " per candidate, "\\n%f %f %f\\n"
               (box), "%d %d %d\\n" (grid), "%d " per cell (candidate index,
               cells in the order iz + nz (iy + ny ix)), "\\n".
-Native byte order, as the reference's fwrite.  (xtc, compressed by
-libxdrfile, is not provided.)
+Native byte order, as the reference's fwrite.
+  .xtc        MDSystem::initWriteXtc / writeHostDataXtc / endWriteXtc
+              (H/src/MDSystem_interface.cu:243-290) over libxdrfile's
+              write_xtc: GROMACS compressed-coordinate frames (XDR header,
+              coordinates rounded to 1/precision, bit-packed), written and
+              read by the library (csrc/xtc.cpp) -- XtcFile below.
 """
+import ctypes as C
+
 import numpy as np
 
 
@@ -148,3 +154,85 @@ def load_density(path):
     dims = tuple(int(v) for v in tok[3:6])
     rho = np.array([float(v) for v in tok[6:6 + int(np.prod(dims))]]).reshape(dims)
     return box, rho
+
+
+class XtcFile:
+    """An xtc trajectory (mode "w", "a" or "r").
+
+    write(step, time, box, xyz, precision=1000.0): box (3,) for a
+    rectangular box or (3, 3); xyz (N, 3) or (N, >=3) (extra columns
+    ignored).  read() returns (step, time, box (3, 3), xyz (N, 3),
+    precision) or None after the last frame.  Iterating yields the frames."""
+
+    def __init__(self, path, mode="w", natoms=None):
+        from ._lib import lib
+        from .engine import check
+        self._lib, self._check = lib(), check
+        h = C.c_void_p()
+        check(self._lib.sre_xtc_open(str(path).encode(), mode.encode(), C.byref(h)))
+        self._h = h
+        self.mode = mode
+        self.natoms = natoms
+        if mode == "r" and natoms is None:
+            n = C.c_int()
+            check(self._lib.sre_xtc_natoms(str(path).encode(), C.byref(n)))
+            self.natoms = n.value
+
+    @staticmethod
+    def _box(box):
+        b = np.asarray(box, np.float32)
+        if b.shape == (3,):
+            b = np.diag(b)
+        return np.ascontiguousarray(b.reshape(3, 3), np.float32)
+
+    def write(self, step, time, box, xyz, precision=1000.0):
+        x = np.ascontiguousarray(np.asarray(xyz, np.float32)[:, :3])
+        b = self._box(box)
+        self._check(self._lib.sre_xtc_write(
+            self._h, len(x), int(step), float(time),
+            b.ctypes.data_as(C.POINTER(C.c_float)),
+            x.ctypes.data_as(C.POINTER(C.c_float)), float(precision)))
+
+    def read(self):
+        from ._lib import END_OF_FILE
+        n = int(self.natoms)
+        x = np.empty((n, 3), np.float32)
+        b = np.empty((3, 3), np.float32)
+        step, t, prec = C.c_int(), C.c_float(), C.c_float()
+        st = self._lib.sre_xtc_read(self._h, n, C.byref(step), C.byref(t),
+                                    b.ctypes.data_as(C.POINTER(C.c_float)),
+                                    x.ctypes.data_as(C.POINTER(C.c_float)), C.byref(prec))
+        if st == END_OF_FILE:
+            return None
+        self._check(st)
+        return step.value, t.value, b, x, prec.value
+
+    def __iter__(self):
+        while True:
+            f = self.read()
+            if f is None:
+                return
+            yield f
+
+    def close(self):
+        if self._h:
+            self._check(self._lib.sre_xtc_close(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def read_xtc(path):
+    """All frames of an xtc file: list of (step, time, box, xyz, precision)."""
+    with XtcFile(path, "r") as f:
+        return list(f)

@@ -133,6 +133,22 @@ class MDSystem:
                 pass
         _io.write_gro(filename, xyz, self.box, vel, step, time)
 
+    def initWriteXtc(self, filename, prec=1000.0):
+        """xdrfile_open(filename, "w") and the rectangular box matrix
+        (H/src/MDSystem_interface.cu:243-263)."""
+        self._xtc = _io.XtcFile(filename, "w")
+        self._xtc_prec = float(prec)
+
+    def writeHostDataXtc(self, step, time, timer=None):
+        """write_xtc of the device positions (H/src/MDSystem_interface.cu:265-284)."""
+        xyz = self._pos[:, :3].cpu().numpy()
+        self._xtc.write(step, time, self.box, xyz, self._xtc_prec)
+
+    def endWriteXtc(self):
+        """xdrfile_close (H/src/MDSystem_interface.cu:286-290)."""
+        self._xtc.close()
+        self._xtc = None
+
     def _engine(self, n_types, skin):
         if self.engine is None:
             self.engine = Engine(n_types=n_types, skin=skin)

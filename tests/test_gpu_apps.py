@@ -22,7 +22,8 @@ def test_lj_nhc_driver_short_run(torch_cuda, tmp_path):
     rows = apps.lj_nhc(str(conf), 600, rcut=5.0, ref_T=0.85, out=out, conf_feq=200,
                        on_frame=lambda s, t, x, e: frames.append((s, x)),
                        confstart=str(tmp_path / "confstart.gro"),
-                       confout=str(tmp_path / "confout.gro"))
+                       confout=str(tmp_path / "confout.gro"),
+                       xtc=str(tmp_path / "traj.xtc"))
     text = out.getvalue().splitlines()
     assert text[1].startswith("#*     1") and text[2].startswith("#* nstep")
     assert rows.shape == (6, 12)
@@ -43,6 +44,13 @@ def test_lj_nhc_driver_short_run(torch_cuda, tmp_path):
     assert np.abs(xo - frames[-1][1]).max() < 6e-4
     xs, _ = io.read_gro(tmp_path / "confstart.gro")
     assert np.abs(xs - np.mod(x0, box)).max() < 6e-4
+    # the xtc trajectory holds the same frames at precision 1000
+    traj = io.read_xtc(str(tmp_path / "traj.xtc"))
+    assert [f[0] for f in traj] == [0, 200, 400, 600]
+    assert np.allclose([f[1] for f in traj], [0.0, 1.0, 2.0, 3.0])
+    for (s, x), f in zip(frames, traj):
+        assert np.array_equal(np.diag(f[2]), np.asarray(box, np.float32))
+        assert np.abs(f[3] - x).max() <= 0.5e-3 + 1e-5
 
 
 def test_lj_nhc_force_corrected_driver(torch_cuda, tmp_path):

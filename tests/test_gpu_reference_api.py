@@ -182,3 +182,13 @@ def test_reference_adaptive_cutoff_sequence(torch_cuda, tmp_path):
     sys.writeHostDataGro(str(tmp_path / "confout.gro"), 10, 0.05)
     x2, b2 = io.read_gro(str(tmp_path / "confout.gro"))
     assert np.abs(x2 - xyz).max() < 5.1e-4 and np.array_equal(b2, box)
+    # the driver's xtc trajectory (H/app/lj.nhc-xtc.cu:155-158, 260, 275)
+    sys.initWriteXtc(str(tmp_path / "traj.xtc"))
+    for i in range(3):
+        sys.writeHostDataXtc(i, 0.005 * i)
+    sys.endWriteXtc()
+    frames = io.read_xtc(str(tmp_path / "traj.xtc"))
+    assert [f[0] for f in frames] == [0, 1, 2]
+    for step, t, b, x3, prec in frames:
+        assert prec == 1000.0 and np.array_equal(np.diag(b), box.astype(np.float32))
+        assert np.abs(x3 - xyz).max() <= 0.5e-3 + 1e-5
