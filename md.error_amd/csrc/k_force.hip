@@ -308,10 +308,12 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   // at the end)
   double ax = kAccBase, ay = kAccBase, az = kAccBase;
   float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsabs = 0.f;
-  // single-type energy without shift: sums of 1/r^12 and 1/r^6, combined
-  // with A and B at the end
+  // single-type energy without shift: the sum of 1/r^6 and the virial
+  // trace, combined at the end (sum of A/r^12 - B/r^6 = -trace/12 -
+  // (B/2) sum 1/r^6, since each pair's trace term is fs d^2 = 6B/r^6 -
+  // 12A/r^12)
   constexpr bool SPLIT_E = STATS && !MULTI && !SHIFT;
-  float e12 = 0.f, e6 = 0.f;
+  float e6 = 0.f;
   // list rows through a resource based at this wave's tile: the lane's byte
   // offset is fixed and the row offset is a scalar, so a row load needs no
   // per-lane address arithmetic; the resource ends with the list allocation
@@ -350,7 +352,6 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
       acc_floor(az, fpz);
       if (STATS) {
         if (SPLIT_E) {
-          e12 = __builtin_fmaf(ri6, ri6, e12);
           e6 += ri6;
         } else {
           const float t = __builtin_fmaf(P.A, ri6, -P.B);
@@ -364,11 +365,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   };
   // issue the gathers of group g into p from rows jr, then load rows of
   // group g + 2 into jr; the compiler fence keeps these loads ahead of the
-  // arithmetic that follows (it would otherwise sink them to their use)
+  // arithmetic that follows (it would otherwise sink them to their use).
+  // A row entry past the lane's list holds whatever an earlier build left
+  // there: it is clamped to a valid atom and its pair is masked in
+  // eval_group.
+  const unsigned nmax = (unsigned)(a.n - 1);
   auto prefetch = [&](uint4 *p, int *jr, int g) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      p[u] = gather_pos_idx<MULTI>(prs, (g * U + u) * T < lim ? jr[u] : si);
+      p[u] = gather_pos_idx<MULTI>(prs, (int)min((unsigned)jr[u], nmax));
 #pragma unroll
     for (int u = 0; u < U; ++u) jr[u] = row((g + 2) * U + u);
     asm volatile("" ::: "memory");
@@ -415,7 +420,8 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
     atomicOr(a.err_flag, 1u);
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {
-    E = SPLIT_E ? group_sum((double)P.A * (double)e12 - (double)P.B * (double)e6, T) * 0.5
+    E = SPLIT_E ? group_sum(-((double)wx + (double)wy + (double)wz) * (kFixInv / 12.0) -
+                                0.5 * (double)P.B * (double)e6, T) * 0.5
                 : group_sum((double)e, T) * 0.5;
     Wx = group_sum((double)wx, T) * (0.5 * kFixInv);
     Wy = group_sum((double)wy, T) * (0.5 * kFixInv);

@@ -50,7 +50,8 @@ def test_lj_nhc_driver_short_run(torch_cuda, tmp_path):
     assert np.allclose([f[1] for f in traj], [0.0, 1.0, 2.0, 3.0])
     for (s, x), f in zip(frames, traj):
         assert np.array_equal(np.diag(f[2]), np.asarray(box, np.float32))
-        assert np.abs(f[3] - x).max() <= 0.5e-3 + 1e-5
+        # half a lattice step, plus the FP32 rounding of x and of the decoding
+        assert np.abs(f[3] - x).max() <= 0.5e-3 + 3e-7 * np.abs(x).max()
 
 
 def test_lj_nhc_force_corrected_driver(torch_cuda, tmp_path):

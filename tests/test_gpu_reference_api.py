@@ -191,4 +191,5 @@ def test_reference_adaptive_cutoff_sequence(torch_cuda, tmp_path):
     assert [f[0] for f in frames] == [0, 1, 2]
     for step, t, b, x3, prec in frames:
         assert prec == 1000.0 and np.array_equal(np.diag(b), box.astype(np.float32))
-        assert np.abs(x3 - xyz).max() <= 0.5e-3 + 1e-5
+        # half a lattice step, plus the FP32 rounding of the decoding
+        assert np.abs(x3 - xyz).max() <= 0.5e-3 + 3e-7 * np.abs(xyz).max()
