@@ -161,7 +161,15 @@ struct sre_engine {
   uint64_t *d_step_ctr = nullptr;  // perturbation step counter (graph mode)
   float pert_sigma = 0.f;
   uint64_t pert_seed = 0;
-  float *d_skin_part = nullptr;    // per update wavefront: max displacement^2
+  float *d_skin_part = nullptr;    // per update wavefront / force tile: max displacement^2
+  // the second coordinate buffer of graphs that refresh the next step inside
+  // the force launch (and its FP64-form residuals)
+  uint4 *d_pos4_alt = nullptr;
+  float4 *d_res4_alt = nullptr;
+  // the refresh inputs of those graphs in cell order, written by each
+  // rebuild: the rebuild-time lattice positions and the perturbation anchor
+  uint4 *d_backup_s = nullptr;
+  float4 *d_anchor_s = nullptr;
   int2 *d_build_part = nullptr;    // per build wavefront: (longest, total)
   int build_part_cap = 0;
   float *d_anchor = nullptr;       // anchored perturbation: positions at set-up
@@ -237,6 +245,7 @@ sre_engine::~sre_engine() {
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
   dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
+  dfree(d_pos4_alt); dfree(d_res4_alt); dfree(d_backup_s); dfree(d_anchor_s);
   dfree(d_adj); dfree(d_adj_tw);
   dfree(d_mass); dfree(d_vel); dfree(d_fcall); dfree(d_md_part); dfree(d_md);
   dfree(d_fc);
@@ -410,6 +419,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   dfree(e->d_stats); dfree(e->d_nin); dfree(e->d_tmp_out); dfree(e->d_tail_atom);
   dfree(e->d_stat_part);
   dfree(e->d_skin_part);
+  dfree(e->d_pos4_alt); dfree(e->d_res4_alt); dfree(e->d_backup_s); dfree(e->d_anchor_s);
   dfree(e->d_vel); dfree(e->d_fcall); dfree(e->d_md_part);
   dfree(e->d_snap_pos); dfree(e->d_snap_vel); dfree(e->d_snap_fcall);
   dfree(e->d_snap_part);
@@ -454,8 +464,14 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->sb.val_b, N));
   HIP_TRY(dalloc(&e->sb.tkey, N));
   HIP_TRY(dalloc(&e->sb.tval, N));
-  HIP_TRY(dalloc(&e->d_skin_part, skin_part_count(N)));
-  launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_part_count(N), e->stream);
+  // slots for the refresh's wavefronts and for the force tiles of any lane
+  // count (a graph step that refreshes the next step keeps one per tile)
+  const int skin_cap = std::max(skin_part_count(N), force_tile_count(N, 16));
+  HIP_TRY(dalloc(&e->d_skin_part, skin_cap));
+  launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_cap, e->stream);
+  HIP_TRY(dalloc(&e->d_pos4_alt, N));
+  HIP_TRY(dalloc(&e->d_backup_s, N));
+  if (e->precision != SRE_PREC_MIXED) HIP_TRY(dalloc(&e->d_res4_alt, N));
   HIP_TRY(hipMemcpy(e->d_rep_of, rep_of.data(), sizeof(int) * N,
                     hipMemcpyHostToDevice));
   // defaults: every atom type 0, uniform rc lattice of 1 bin (set later)
@@ -477,6 +493,13 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
 static sre_status md_update_total_mass(sre_engine *e);
 
 // The host geometry of the only replica (position refresh argument), or NULL
+// Skin partial slots in use: the refresh's wavefronts and the force tiles of
+// the list's lane count.
+static int skin_slots(const sre_engine *e) {
+  const int T = e->list_lanes > 0 ? e->list_lanes : e->tune.force_lanes;
+  return std::max(skin_part_count(e->n_atoms), force_tile_count(e->n_atoms, T));
+}
+
 static const RepInfo *rep0(const sre_engine *e) {
   return e->n_rep == 1 ? &e->reps[0] : nullptr;
 }
@@ -886,7 +909,9 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
                         e->d_paramsd,
                         e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
-                        e->d_counters, e->d_skin_part, e->d_log, e->d_log_idx};
+                        e->d_counters, e->d_skin_part, e->d_log, e->d_log_idx,
+                        e->d_backup_s, e->pert_anchored ? e->d_anchor : nullptr,
+                        e->d_anchor_s};
   put(ptrs, sizeof ptrs);
   return v;
 }
@@ -916,12 +941,13 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
                        e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
                        e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
                        e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
-                       e->rc_explicit ? e->d_rc_atom : nullptr, st);
+                       e->rc_explicit ? e->d_rc_atom : nullptr, st, e->d_backup_s,
+                       e->pert_anchored ? e->d_anchor : nullptr, e->d_anchor_s);
   if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);
   enqueue_build(e, st);
-  launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_part_count(N),
+  launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_slots(e),
                       e->d_skin_part, 1, reinterpret_cast<unsigned *>(e->d_counters + 2),
                       st, e->d_log, e->d_log_idx, kRebuildLog);
 }
@@ -1041,7 +1067,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
-    launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+    launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
                         e->d_skin_part, 0, e->d_flags, e->stream);
     unsigned bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
@@ -1078,7 +1104,8 @@ static ForceArgs force_args(sre_engine *e, bool atom_stats) {
   a.fxd = e->d_fxd; a.fyd = e->d_fyd; a.fzd = e->d_fzd;
   a.atom_stats = atom_stats ? e->d_atom_stats : nullptr;
   a.err_flag = e->d_flags + 1;
-  a.step_ctr = nullptr;
+  a.ctr_in = nullptr;
+  a.ctr_out = nullptr;
   // the per-atom builder writes one (longest list, total) pair per wave of
   // build_lanes lanes per atom: the force kernel's waves when the lane
   // counts agree
@@ -1103,7 +1130,8 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
   const int T = e->list_lanes;
   refresh_tail_values(e);
   ForceArgs a = force_args(e, stats);
-  a.step_ctr = step_ctr;
+  a.ctr_in = step_ctr;
+  a.ctr_out = step_ctr;
   launch_force(e->precision, T, e->tune.force_unroll, e->tune.block, stats, tail,
                a, e->stream);
   if ((flags & SRE_FCORR) && e->fc_cells > 0)
@@ -1494,10 +1522,19 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
                       t->build_mode != e->tune.build_mode ||
                       t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
   if (memcmp(&e->tune, t, sizeof *t) != 0) e->graph_epoch++;  // captured launches
+  const bool lanes = t->force_lanes != e->tune.force_lanes;
   e->tune = *t;
   if (layout) {
     e->built = false;
     e->cap = 0;
+  }
+  if (lanes && e->d_skin_part) {
+    // the tile count changes: every skin slot starts from zero again
+    launch_fill_u32((uint32_t *)e->d_skin_part, 0u,
+                    (size_t)std::max(skin_part_count(e->n_atoms),
+                                     force_tile_count(e->n_atoms, 16)),
+                    e->stream);
+    HIP_TRY(hipGetLastError());
   }
   return SRE_OK;
 }
@@ -1722,6 +1759,66 @@ static sre_status enqueue_md_step(sre_engine *e, unsigned flags) {
   return SRE_OK;
 }
 
+// K perturbation steps (no integrator, no correction force) with the
+// refresh of each next step folded into the force launch: one refresh of
+// the first step, then K force launches, launch j reading coordinate buffer
+// X_j and refreshing step j + 1 into X_{j+1} (j < K - 1).  The buffers
+// alternate so that the last step reads d_pos4 (X_{K-1}), which therefore
+// holds the current coordinates afterwards as after unfused steps.  The
+// step counter alternates between its two slots in the same way (launch j
+// reads slot j % 2, the refresh inside it draws step *slot + 1, one thread
+// writes the next slot; the last launch writes slot 0, which the next
+// graph's first refresh reads).  Same values as the unfused steps: the
+// refresh arithmetic is the same and no launch reads a buffer another
+// writes.
+static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
+                                      const PerturbArgs &pa) {
+  const bool stats = flags & SRE_STATS;
+  const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
+  auto pos4_of = [&](int j) { return ((K - 1 - j) % 2 == 0) ? e->d_pos4 : e->d_pos4_alt; };
+  auto res4_of = [&](int j) { return ((K - 1 - j) % 2 == 0) ? e->d_res4 : e->d_res4_alt; };
+  launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
+                          e->pos_stride, e->d_reps, rep0(e), pos4_of(0), e->d_backup,
+                          res4_of(0), e->d_skin_part, &pa, nullptr, e->stream);
+  refresh_tail_values(e);
+  NextRefresh nx{};
+  nx.active = 1;
+  nx.perm = e->sb.val_a;
+  nx.pos = const_cast<float *>(e->d_pos);
+  nx.stride = e->pos_stride;
+  nx.anchor_s = pa.anchor ? e->d_anchor_s : nullptr;
+  nx.backup_s = e->d_backup_s;
+  nx.skin_part = e->d_skin_part;
+  nx.sigma = pa.sigma;
+  nx.seed = pa.seed;
+  if (const RepInfo *r0 = rep0(e)) {
+    nx.one_rep = 1;
+    for (int d = 0; d < 3; ++d) {
+      nx.g1.L[d] = r0->L[d];
+      nx.g1.q[d] = r0->q[d];
+      nx.g1.invq[d] = r0->invq[d];
+      nx.g1.qd[d] = r0->qd[d];
+    }
+  }
+  for (int j = 0; j < K; ++j) {
+    ForceArgs a = force_args(e, stats);
+    a.pos4 = pos4_of(j);
+    a.res4 = res4_of(j);
+    a.ctr_in = e->d_step_ctr + (j % 2);
+    a.ctr_out = j == K - 1 ? e->d_step_ctr : e->d_step_ctr + ((j + 1) % 2);
+    if (j < K - 1) {
+      a.next = nx;
+      a.next.pos4 = pos4_of(j + 1);
+      a.next.res4 = res4_of(j + 1);
+    }
+    launch_force(e->precision, e->list_lanes, e->tune.force_unroll, e->tune.block, stats,
+                 tail, a, e->stream);
+  }
+  if (stats) e->stats_pending = true;
+  HIP_TRY(hipGetLastError());
+  return SRE_OK;
+}
+
 // Captures K steps of [refresh (+ perturbation) + force] into one graph, or
 // returns the cached one; the graph reads the engine's buffers by address, so
 // every reallocation or layout change bumps graph_epoch and invalidates it.
@@ -1748,18 +1845,22 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
                    e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
     const bool md = (flags & kGraphMD) != 0;
     sre_status s = SRE_OK;
-    for (int k = 0; k < K && s == SRE_OK; ++k) {
-      if (md) {
-        s = enqueue_md_step(e, k == K - 1 ? flags : flags & ~SRE_STATS);
-        continue;
+    if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s)) {
+      s = enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa);
+    } else {
+      for (int k = 0; k < K && s == SRE_OK; ++k) {
+        if (md) {
+          s = enqueue_md_step(e, k == K - 1 ? flags : flags & ~SRE_STATS);
+          continue;
+        }
+        launch_update_positions(e->n_atoms, e->d_inv_perm,
+                                const_cast<float *>(e->d_pos), e->pos_stride,
+                                e->d_reps, rep0(e), e->d_pos4, e->d_backup, e->d_res4,
+                                e->d_skin_part, pert ? &pa : nullptr, nullptr,
+                                e->stream);
+        s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
+                           pert ? e->d_step_ctr : nullptr);
       }
-      launch_update_positions(e->n_atoms, e->d_inv_perm,
-                              const_cast<float *>(e->d_pos), e->pos_stride,
-                              e->d_reps, rep0(e), e->d_pos4, e->d_backup, e->d_res4,
-                              e->d_skin_part, pert ? &pa : nullptr, nullptr,
-                              e->stream);
-      s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
-                         pert ? e->d_step_ctr : nullptr);
     }
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (s != SRE_OK) return s;
@@ -1914,7 +2015,7 @@ extern "C" sre_status sre_check_skin_flag(sre_engine *e) {
   if (!e) return fail(SRE_ERR_INVALID, "null engine");
   HIP_TRY(hipSetDevice(e->device));
   if (!e->d_skin_part) return fail(SRE_ERR_INVALID, "system not set");
-  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+  launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
                       e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
@@ -1957,15 +2058,25 @@ extern "C" sre_status sre_set_perturbation(sre_engine *e, float sigma,
                            hipMemcpyDeviceToDevice, e->stream));
     e->anchor_count = cnt;
     e->anchor_stride = e->pos_stride;
+    if (!e->d_anchor_s) {
+      HIP_TRY(dalloc(&e->d_anchor_s, e->n_atoms));
+      e->graph_epoch++;
+    }
+    // the anchor in the current cell order (later rebuilds write it too)
+    if (e->built)
+      launch_gather_anchor(e->n_atoms, e->sb.val_a, e->d_anchor, e->pos_stride,
+                           e->d_anchor_s, e->stream);
+    HIP_TRY(hipGetLastError());
   }
   if (e->pert_anchored != (anchored != 0)) e->graph_epoch++;
   e->pert_anchored = anchored != 0;
   if (!e->d_step_ctr) {
-    HIP_TRY(dalloc(&e->d_step_ctr, 1));
+    HIP_TRY(dalloc(&e->d_step_ctr, 2));   // slot 0 current; slot 1 for fused graphs
     e->graph_epoch++;
   }
-  HIP_TRY(hipMemcpyAsync(e->d_step_ctr, &first_step, sizeof(uint64_t),
-                         hipMemcpyHostToDevice, e->stream));
+  const uint64_t ctr2[2] = {first_step, first_step};
+  HIP_TRY(hipMemcpyAsync(e->d_step_ctr, ctr2, sizeof ctr2, hipMemcpyHostToDevice,
+                         e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
   if (e->pert_sigma != sigma || e->pert_seed != seed) e->graph_epoch++;
   e->pert_sigma = sigma;
@@ -2144,7 +2255,7 @@ extern "C" sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in) {
 // the force-range flag, in one readback.  A skin violation is reported first:
 // missed pairs are the likelier cause of a range error that follows it.
 static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
-  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
+  launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
                       e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(bits, e->d_flags, 2 * sizeof(unsigned),

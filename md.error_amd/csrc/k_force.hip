@@ -29,12 +29,6 @@ namespace sre {
 // and centre-of-mass factors of the step: v <- a2 (a1 v - vcom) + F dt/(2m),
 // x <- x + v dt, then x is wrapped back into [0, L) by one add or subtract of
 // L (the reference wraps at each rebuild, H/src/MDSystem_interface.cu:19-36).
-// Box geometry the refresh needs; passed by value when all atoms share one
-// replica, so that no load of the replica table precedes the atom's loads.
-struct Geom {
-  float L[3], q[3];
-  double invq[3], qd[3];
-};
 __device__ __forceinline__ Geom geom_of(const RepInfo &R) {
   Geom g;
   for (int d = 0; d < 3; ++d) {
@@ -128,6 +122,119 @@ __global__ __launch_bounds__(256) void k_update_positions(
   for (int o = kWave / 2; o > 0; o >>= 1)
     d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
   if ((threadIdx.x & (kWave - 1)) == 0 && wslot && d2 > prev) skin_part[w] = d2;
+}
+
+// ---------------------------------------- next-step refresh in a force launch --
+//
+// The atom at sorted slot si of a force tile is refreshed by its group of T
+// lanes: with T >= 4 lane c < 3 takes component c (one Gaussian each) and
+// lane 3 the type/replica word, so the 16-byte record is written by four
+// lanes; with T < 4 the group's first lane does all of it.  The inputs are
+// loaded when the launch starts (their latency hides behind the pair loop).
+struct RefreshIn {
+  int i;          // caller index
+  uint4 b;        // rebuild-time lattice position
+  float o[3];     // positions perturbed about (the lane's components)
+  float prev;     // the tile's running maximum (first lane of the wave)
+};
+
+template <int T>
+__device__ __forceinline__ RefreshIn refresh_load(const NextRefresh &nx, int si, bool act,
+                                                  int l, int lane, int tile) {
+  RefreshIn r;
+  r.i = act ? (int)nx.perm[si] : 0;   // waited for only where stored to
+  r.b = make_uint4(0u, 0u, 0u, 0u);
+  r.o[0] = r.o[1] = r.o[2] = 0.f;
+  if (act) {
+    r.b = nx.backup_s[si];
+    // anchored: the cell-order copy (no dependent load); free walk: the
+    // caller-order positions of this step
+    const float *o = nx.anchor_s ? reinterpret_cast<const float *>(nx.anchor_s + si)
+                                 : nx.pos + (size_t)r.i * nx.stride;
+    if (T >= 4) {
+      if (l < 3) r.o[0] = o[l];
+    } else if (l == 0) {
+      r.o[0] = o[0];
+      r.o[1] = o[1];
+      r.o[2] = o[2];
+    }
+  }
+  r.prev = lane == 0 ? nx.skin_part[tile] : 0.f;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t comp_of(const uint4 &v, int c) {
+  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+
+// Writes the refreshed atom and folds its squared displacement into the
+// tile's running maximum (the same operations as k_update_positions).
+template <int T>
+__device__ __forceinline__ void refresh_store(const NextRefresh &nx, const RepInfo *reps,
+                                              uint64_t step, const RefreshIn &r, int si,
+                                              bool act, int l, int lane, int tile) {
+  float d2 = 0.f;
+  if (T >= 4) {
+    float dmine = 0.f;
+    if (act && l < 4) {
+      uint32_t word = r.b.w;
+      float res = 0.f;
+      if (l < 3) {
+        // this lane's axis of the geometry (constant indices only, so that
+        // nothing is indexed in private memory)
+        float q;
+        double invq, qd;
+        if (nx.one_rep) {
+          q = l == 0 ? nx.g1.q[0] : (l == 1 ? nx.g1.q[1] : nx.g1.q[2]);
+          invq = l == 0 ? nx.g1.invq[0] : (l == 1 ? nx.g1.invq[1] : nx.g1.invq[2]);
+          qd = l == 0 ? nx.g1.qd[0] : (l == 1 ? nx.g1.qd[1] : nx.g1.qd[2]);
+        } else {
+          const RepInfo &R = reps[r.b.w >> 8];
+          q = l == 0 ? R.q[0] : (l == 1 ? R.q[1] : R.q[2]);
+          invq = l == 0 ? R.invq[0] : (l == 1 ? R.invq[1] : R.invq[2]);
+          qd = l == 0 ? R.qd[0] : (l == 1 ? R.qd[1] : R.qd[2]);
+        }
+        const float x = r.o[0] + nx.sigma * gauss_component(gauss_key(nx.seed, step),
+                                                             (uint64_t)r.i, l);
+        nx.pos[(size_t)r.i * nx.stride + l] = x;
+        word = quantize(x, invq);
+        if (nx.res4) res = quant_residual(x, invq, qd);
+        dmine = disp(word, comp_of(r.b, l), q);
+      }
+      reinterpret_cast<uint32_t *>(nx.pos4 + si)[l] = word;
+      if (nx.res4) reinterpret_cast<float *>(nx.res4 + si)[l] = res;
+    }
+    // the group's first quad holds the three components
+    const float dx = quad_lane<0>(dmine), dy = quad_lane<1>(dmine), dz = quad_lane<2>(dmine);
+    d2 = act && l < 4 ? dist2(dx, dy, dz) : 0.f;
+  } else if (act && l == 0) {
+    const RepInfo &R = reps[r.b.w >> 8];
+    Geom G;
+    for (int d = 0; d < 3; ++d) {
+      G.q[d] = nx.one_rep ? nx.g1.q[d] : R.q[d];
+      G.invq[d] = nx.one_rep ? nx.g1.invq[d] : R.invq[d];
+      G.qd[d] = nx.one_rep ? nx.g1.qd[d] : R.qd[d];
+    }
+    const float3 g = gauss3(nx.seed, step, (uint64_t)r.i);
+    const float x = r.o[0] + nx.sigma * g.x;
+    const float y = r.o[1] + nx.sigma * g.y;
+    const float z = r.o[2] + nx.sigma * g.z;
+    float *p = nx.pos + (size_t)r.i * nx.stride;
+    p[0] = x;
+    p[1] = y;
+    p[2] = z;
+    const uint32_t ux = quantize(x, G.invq[0]);
+    const uint32_t uy = quantize(y, G.invq[1]);
+    const uint32_t uz = quantize(z, G.invq[2]);
+    nx.pos4[si] = make_uint4(ux, uy, uz, r.b.w);
+    if (nx.res4)
+      nx.res4[si] = make_float4(quant_residual(x, G.invq[0], G.qd[0]),
+                                quant_residual(y, G.invq[1], G.qd[1]),
+                                quant_residual(z, G.invq[2], G.qd[2]), 0.f);
+    d2 = dist2(disp(ux, r.b.x, G.q[0]), disp(uy, r.b.y, G.q[1]), disp(uz, r.b.z, G.q[2]));
+  }
+  d2 = wave_max_nonneg(d2);
+  if (lane == 0 && d2 > r.prev) nx.skin_part[tile] = d2;
 }
 
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
@@ -270,7 +377,7 @@ __device__ __forceinline__ int wave_max_int(int v) {
 //
 // One wave-tile (64/T consecutive sorted atoms) of the force evaluation;
 // `wave` is the tile index.
-template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
+template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT, bool NEXT>
 __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *sp,
                                            const int wave) {
   constexpr int A = kWave / T;
@@ -282,6 +389,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   if (s0 >= a.n) return;  // whole wave beyond the atoms
 
   const int si = act ? s : s0;
+  // the next step's refresh of this tile: inputs loaded first (with the
+  // loads below); its arithmetic and stores run while the loop's first list
+  // rows are in flight (it needs nothing this launch computes)
+  RefreshIn rin{};
+  if (NEXT) rin = refresh_load<T>(a.next, si, act, l, lane, wave);
   const uint4 pi = a.pos4[si];
   const int rep = (int)(pi.w >> 8);
   const RepInfo &R = a.reps[rep];
@@ -388,6 +500,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   // the gathers of g and the rows of g+2), so that the loop waits only for
   // the loads it is about to use
   asm volatile("" ::: "memory");
+  if (NEXT) {
+    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, si, act, l, lane, wave);
+    asm volatile("" ::: "memory");
+  }
   prefetch(pa, ja, 0);
   // the accumulators' FP64 adds round toward zero inside the pair loop
   // (FP32 rounding is a separate field of the mode register, untouched)
@@ -455,10 +571,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
 // XCDs by xcd_block.  Persistent launch: persist_waves wavefronts, each
 // taking a contiguous chunk of tiles (wavefront order after the same block
 // map, so each XCD works through one contiguous range of atoms).
-template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
+template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT, bool NEXT>
 __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   // the position refresh of this step has read the counter (stream order)
-  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
+  if (a.ctr_out && blockIdx.x == 0 && threadIdx.x == 0) *a.ctr_out = *a.ctr_in + 1;
   __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -468,21 +584,21 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane((blk * blockDim.x + threadIdx.x) / kWave);
   if (a.persist_waves <= 0) {
-    force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, wave);
+    force_tile<T, U, STATS, TAIL, MULTI, SHIFT, NEXT>(a, sp, wave);
     return;
   }
   constexpr int A = kWave / T;
   const int n_tiles = (a.n + A - 1) / A;
   const int per = (n_tiles + a.persist_waves - 1) / a.persist_waves;
   const int t0 = wave * per, t1 = min(n_tiles, t0 + per);
-  for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, t);
+  for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT, NEXT>(a, sp, t);
 }
 
 // --------------------------------------------------------------- FP64 ----
 
-template <int T, bool STATS, bool TAIL, bool MULTI>
+template <int T, bool STATS, bool TAIL, bool MULTI, bool NEXT>
 __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
-  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
+  if (a.ctr_out && blockIdx.x == 0 && threadIdx.x == 0) *a.ctr_out = *a.ctr_in + 1;
   __shared__ PairParamD sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -498,17 +614,28 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   const bool act = s < a.n;
   const int s0 = wave * A;
   if (s0 >= a.n) return;
+  // the next step's refresh of this tile (inputs first, then this atom's
+  // loads, then the refresh while those are in flight; outside the per-atom
+  // branch, as it reduces over the wavefront)
+  RefreshIn rin{};
+  if (NEXT) rin = refresh_load<T>(a.next, act ? s : s0, act, l, lane, wave);
+  const int sa = act ? s : s0;
+  const uint4 pi = a.pos4[sa];
+  const float4 ri = a.res4[sa];
+  const float rc = a.rcut[sa];
+  const int nn = act ? a.nnb[sa] : 0;
+  if (NEXT) {
+    asm volatile("" ::: "memory");
+    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, sa, act, l, lane, wave);
+    asm volatile("" ::: "memory");
+  }
 
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
   if (act) {
-    const uint4 pi = a.pos4[s];
     const RepInfo &R = a.reps[pi.w >> 8];
     const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
     const double qdx = R.qd[0], qdy = R.qd[1], qdz = R.qd[2];
-    const float4 ri = a.res4[s];
-    const float rc = a.rcut[s];
     const float rc2 = rc * rc;
-    const int nn = a.nnb[s];
     const int ti = (int)(pi.w & 0xffu) * a.n_types;
     const int iters = (nn + T - 1) / T;
     PairParamD P = a.paramsd[0];
@@ -582,9 +709,9 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
 // sub-lattice residuals, one list row per group), the same pair set (the
 // FP32 predicate) and FP64 arithmetic and sums as k_force_fp64.  Selected
 // for the FP64 form by force_unroll = 1; measured beside the plain form.
-template <int T, bool STATS, bool TAIL, bool MULTI>
+template <int T, bool STATS, bool TAIL, bool MULTI, bool NEXT>
 __global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
-  if (a.step_ctr && blockIdx.x == 0 && threadIdx.x == 0) *a.step_ctr += 1;
+  if (a.ctr_out && blockIdx.x == 0 && threadIdx.x == 0) *a.ctr_out = *a.ctr_in + 1;
   __shared__ PairParamD sp[MULTI ? kMaxPairTypes : 1];
   if (MULTI) {
     for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
@@ -601,6 +728,8 @@ __global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
   const int s0 = wave * A;
   if (s0 >= a.n) return;
   const int si = act ? s : s0;
+  RefreshIn rin{};
+  if (NEXT) rin = refresh_load<T>(a.next, si, act, l, lane, wave);
   const uint4 pi = a.pos4[si];
   const RepInfo &R = a.reps[pi.w >> 8];
   const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
@@ -661,6 +790,10 @@ __global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
   };
   int ja = row(0), jb = row(1);
   asm volatile("" ::: "memory");
+  if (NEXT) {   // the next step's refresh while the first rows are in flight
+    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, si, act, l, lane, wave);
+    asm volatile("" ::: "memory");
+  }
   Pj qa, qb;
   fetch(qa, ja, 0);
   int g = 0;
@@ -709,15 +842,44 @@ int force_num_waves(int n, int lanes) {
   return (n + A - 1) / A;
 }
 
+template <int T, int U, bool STATS, bool TAIL, bool SHIFT, bool NEXT>
+static void launch_mixed_n(int grid, int block, bool multi, const ForceArgs &a,
+                           hipStream_t st) {
+  if (multi)
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, true, SHIFT, NEXT>), dim3(grid),
+                       dim3(block), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, false, SHIFT, NEXT>), dim3(grid),
+                       dim3(block), 0, st, a);
+}
+
 template <int T, int U, bool STATS, bool TAIL, bool SHIFT>
 static void launch_mixed_s(int grid, int block, bool multi, const ForceArgs &a,
                            hipStream_t st) {
-  if (multi)
-    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, true, SHIFT>), dim3(grid),
-                       dim3(block), 0, st, a);
+  if (a.next.active)
+    launch_mixed_n<T, U, STATS, TAIL, SHIFT, true>(grid, block, multi, a, st);
   else
-    hipLaunchKernelGGL((k_force_mixed<T, U, STATS, TAIL, false, SHIFT>), dim3(grid),
-                       dim3(block), 0, st, a);
+    launch_mixed_n<T, U, STATS, TAIL, SHIFT, false>(grid, block, multi, a, st);
+}
+
+template <int T, bool STATS, bool TAIL, bool MULTI>
+static void launch_fp64_n(int grid, int block, bool piped, const ForceArgs &a,
+                          hipStream_t st) {
+  if (piped) {
+    if (a.next.active)
+      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, MULTI, true>), dim3(grid),
+                         dim3(block), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, MULTI, false>), dim3(grid),
+                         dim3(block), 0, st, a);
+  } else {
+    if (a.next.active)
+      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, MULTI, true>), dim3(grid),
+                         dim3(block), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, MULTI, false>), dim3(grid),
+                         dim3(block), 0, st, a);
+  }
 }
 
 // the energy shift only enters the statistics
@@ -741,20 +903,9 @@ static void launch_force_t(int precision, int unroll, int block,
     if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
     else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);
     else launch_mixed_u<T, 4, STATS, TAIL>(grid, block, multi, a, st);
-  } else if (unroll == 1) {   // the FP64 form in the pipelined structure
-    if (multi)
-      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, true>), dim3(grid),
-                         dim3(block), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_force_fp64_piped<T, STATS, TAIL, false>), dim3(grid),
-                         dim3(block), 0, st, a);
-  } else {
-    if (multi)
-      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, true>), dim3(grid),
-                         dim3(block), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_force_fp64<T, STATS, TAIL, false>), dim3(grid),
-                         dim3(block), 0, st, a);
+  } else {   // unroll 1: the FP64 form in the pipelined structure
+    if (multi) launch_fp64_n<T, STATS, TAIL, true>(grid, block, unroll == 1, a, st);
+    else launch_fp64_n<T, STATS, TAIL, false>(grid, block, unroll == 1, a, st);
   }
 }
 

@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
     const float *pos, int stride, const int *types, const int *rep_of,
     const RepInfo *reps, const float *rc_table, int n_types, int *start,
     uint4 *pos4, uint4 *backup, uint32_t *inv_perm, float4 *res4, float *rcut,
-    int *tail_idx, const float *rc_atom) {
+    int *tail_idx, const float *rc_atom, uint4 *backup_s, const float *anchor,
+    float4 *anchor_s) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > n) return;
   {
@@ -373,6 +374,11 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
   pos4[s] = v;
   backup[i] = v;
   inv_perm[i] = (uint32_t)s;
+  if (backup_s) backup_s[s] = v;
+  if (anchor_s) {
+    const float *a = anchor + (size_t)i * stride;
+    anchor_s[s] = make_float4(a[0], a[1], a[2], 0.f);
+  }
   if (res4)
     res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
                           quant_residual(y, R.invq[1], R.qd[1]),
@@ -422,11 +428,28 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
                           uint32_t *inv_perm, float4 *res4, float *rcut,
-                          int *tail_idx, const float *rc_atom, hipStream_t st) {
+                          int *tail_idx, const float *rc_atom, hipStream_t st,
+                          uint4 *backup_s, const float *anchor, float4 *anchor_s) {
   hipLaunchKernelGGL(k_gather_sorted, dim3((n + 1 + 255) / 256), dim3(256), 0,
                      st, n, n_cells, keys, perm, pos, stride, types, rep_of,
                      reps, rc_table, n_types, start, pos4, backup, inv_perm,
-                     res4, rcut, tail_idx, rc_atom);
+                     res4, rcut, tail_idx, rc_atom, backup_s, anchor,
+                     anchor ? anchor_s : nullptr);
+}
+
+__global__ void k_gather_anchor(int n, const uint32_t *perm, const float *anchor,
+                                int stride, float4 *anchor_s) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float *a = anchor + (size_t)perm[s] * stride;
+  anchor_s[s] = make_float4(a[0], a[1], a[2], 0.f);
+}
+
+void launch_gather_anchor(int n, const uint32_t *perm, const float *anchor, int stride,
+                          float4 *anchor_s, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_anchor, dim3((n + 255) / 256), dim3(256), 0, st, n, perm,
+                     anchor, stride, anchor_s);
 }
 
 // ------------------------------------------------------------------ fills --

@@ -172,7 +172,7 @@ __host__ __device__ inline int rc_bin(float x, float L, int n) {
 // Lea and Flood, "Fast splittable pseudorandom number generators", OOPSLA
 // 2014) hashes followed by the Box-Muller transform, so a perturbation does
 // not depend on launch shape or on the order atoms are visited in.  Two
-// 64-bit hashes give four 24-bit uniforms in (0, 1]; the logarithm and the
+// 64-bit hashes per atom give four 24-bit uniforms in (0, 1]; the logarithm and the
 // sine and cosine are the hardware's (v_log_f32 is log2; v_sin_f32 and
 // v_cos_f32 take their argument in turns), accurate to a few ulp, which a
 // synthetic thermal displacement does not need more of.
@@ -188,9 +188,17 @@ __device__ inline float u24(uint32_t bits24) {
   return ((float)bits24 + 1.0f) * (1.0f / 16777216.0f);
 }
 
+// The step's key, the same for every atom (so computed once per wavefront,
+// on the scalar unit); atom i's Box-Muller pair p (p = 0: components 0 and
+// 1, p = 1: component 2) draws its two uniforms from one hash of
+// key ^ (2i + p).
+__device__ __forceinline__ uint64_t gauss_key(uint64_t seed, uint64_t step) {
+  return splitmix64(seed ^ splitmix64(step * 0x632BE59BD9B4E019ull));
+}
+
 __device__ inline float3 gauss3(uint64_t seed, uint64_t step, uint64_t i) {
-  const uint64_t base = splitmix64(seed ^ splitmix64(step * 0x632BE59BD9B4E019ull ^ i));
-  const uint64_t h1 = splitmix64(base + 1), h2 = splitmix64(base + 2);
+  const uint64_t key = gauss_key(seed, step);
+  const uint64_t h1 = splitmix64(key ^ (i << 1)), h2 = splitmix64(key ^ (i << 1 | 1));
   const float u1 = u24((uint32_t)(h1 >> 40)), u2 = u24((uint32_t)(h1 >> 8) & 0xffffffu);
   const float u3 = u24((uint32_t)(h2 >> 40)), u4 = u24((uint32_t)(h2 >> 8) & 0xffffffu);
   // sqrt(-2 ln u) = sqrt(-2 ln2 log2 u)
@@ -198,6 +206,38 @@ __device__ inline float3 gauss3(uint64_t seed, uint64_t step, uint64_t i) {
   const float r2 = __builtin_sqrtf(-1.3862943611f * __builtin_amdgcn_logf(u3));
   return make_float3(r1 * __builtin_amdgcn_cosf(u2), r1 * __builtin_amdgcn_sinf(u2),
                      r2 * __builtin_amdgcn_cosf(u4));
+}
+
+// Component c of gauss3 (key = gauss_key(seed, step)), with the same
+// operations.
+__device__ inline float gauss_component(uint64_t key, uint64_t i, int c) {
+  const uint64_t h = splitmix64(key ^ (i << 1 | (uint64_t)(c >> 1)));
+  const float ua = u24((uint32_t)(h >> 40)), ub = u24((uint32_t)(h >> 8) & 0xffffffu);
+  const float r = __builtin_sqrtf(-1.3862943611f * __builtin_amdgcn_logf(ua));
+  return c == 1 ? r * __builtin_amdgcn_sinf(ub) : r * __builtin_amdgcn_cosf(ub);
+}
+
+// Cross-lane moves without LDS (DPP): the value of lane k of this lane's
+// quad, and the maximum of a non-negative float over the wavefront (read on
+// every lane; all 64 lanes must be active).
+template <int K>
+__device__ __forceinline__ float quad_lane(float v) {
+  // quad_perm [K, K, K, K]
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), K * 0x55, 0xF, 0xF,
+                                                    false));
+}
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  // non-negative floats order as their bit patterns; quad, half row, row,
+  // then rows 0 -> 1 and 2 -> 3, then row 1 -> rows 2 and 3: lane 63 ends
+  // with the maximum
+  int b = __float_as_int(v);
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0xB1, 0xF, 0xF, false));   // quad [1,0,3,2]
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0x4E, 0xF, 0xF, false));   // quad [2,3,0,1]
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0x140, 0xF, 0xF, false));  // row_mirror
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  b = max(b, __builtin_amdgcn_update_dpp(b, b, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(b, 63));
 }
 
 __device__ __forceinline__ float disp(uint32_t uj, uint32_t ui, float q) {
@@ -281,7 +321,14 @@ void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
                           const RepInfo *reps, const float *rc_table,
                           int n_types, int *start, uint4 *pos4, uint4 *backup,
                           uint32_t *inv_perm, float4 *res4, float *rcut,
-                          int *tail_idx, const float *rc_atom, hipStream_t st);
+                          int *tail_idx, const float *rc_atom, hipStream_t st,
+                          uint4 *backup_s = nullptr, const float *anchor = nullptr,
+                          float4 *anchor_s = nullptr);
+// The same rebuild-time data in cell order for the next-step refresh of
+// force launches: backup_s[s] (the gather above) and anchor_s[s] =
+// anchor[perm[s]] (x, y, z, 0), also after a new anchor.
+void launch_gather_anchor(int n, const uint32_t *perm, const float *anchor, int stride,
+                          float4 *anchor_s, hipStream_t st);
 // rc_atom[i] (caller order) from the table at the atom's current position,
 // also into rcut[inv_perm[i]] when inv_perm is given
 void launch_assign_rc(int n, const float *pos, int stride, const int *rep_of,
@@ -318,6 +365,14 @@ void launch_rebuild_info(int n_build, const int2 *build_part,
                          unsigned *log_idx = nullptr, int log_cap = 0);
 void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                        uint32_t *tmp, hipStream_t st);
+// Box geometry the position refresh needs; passed by value when all atoms
+// share one replica, so that no load of the replica table precedes the
+// atom's loads.
+struct Geom {
+  float L[3], q[3];
+  double invq[3], qd[3];
+};
+
 // step_ctr (optional) is advanced by one thread after the positions are read
 // Optional perturbation folded into the position refresh:
 // pos = (anchor ? anchor : pos) + sigma * gauss3(seed, *step_ctr, atom).
@@ -379,8 +434,13 @@ void launch_md_finish(int n, const double *part, MDState *state, float *vel,
                       hipStream_t st);
 // Position refresh + skin test: each wavefront keeps the largest squared
 // displacement it has seen since the last rebuild in skin_part[wave]
-// (skin_part_count(n) entries).
+// (skin_part_count(n) entries); a force launch that refreshes the next step
+// keeps one per force tile (force_tile_count entries).
 inline int skin_part_count(int n) { return (n + kWave - 1) / kWave; }
+inline int force_tile_count(int n, int lanes) {
+  const int A = kWave / lanes;
+  return (n + A - 1) / A;
+}
 // backup and inv_perm are in caller order (written by the sorted gather)
 // host_rep0: the host copy of the only replica's geometry (NULL with several)
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
@@ -388,6 +448,30 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              uint4 *pos4, const uint4 *backup, float4 *res4,
                              float *skin_part, const PerturbArgs *pert,
                              const DriftArgs *drift, hipStream_t st);
+
+// The next step's position refresh folded into a force launch (perturbation
+// mode, no integrator): after its forces, each tile's wavefront refreshes
+// the tile's atoms into a second coordinate buffer -- the same arithmetic
+// as k_update_positions with a perturbation at step *ctr_in + 1 -- so that
+// the launch of step s also produces the coordinates of step s + 1 while
+// its own gathers read those of step s.  Each tile keeps its running
+// maximum squared displacement in skin_part[tile] (no atomics).
+struct NextRefresh {
+  int active;
+  const uint32_t *perm;        // sorted slot -> caller index
+  float *pos;                  // caller positions (the new ones written back)
+  int stride;
+  const float4 *anchor_s;      // positions perturbed about, cell order, or NULL
+                               // (then pos itself, caller order)
+  const uint4 *backup_s;       // rebuild-time lattice positions (cell order)
+  uint4 *pos4;                 // the next step's lattice positions (sorted)
+  float4 *res4;                // FP64 form: the next step's residuals, or NULL
+  float *skin_part;            // running maximum per tile
+  float sigma;
+  uint64_t seed;
+  Geom g1;                     // the geometry when one_rep
+  int one_rep;
+};
 
 struct ForceArgs {
   int n;
@@ -409,13 +493,18 @@ struct ForceArgs {
   double *fxd, *fyd, *fzd;     // FP64 output (sorted order)
   double *atom_stats;          // optional per-atom (E, Wxx, Wyy, Wzz)
   unsigned *err_flag;
-  uint64_t *step_ctr;          // perturbation step counter advanced here, or NULL
+  // perturbation step counter: one thread writes *ctr_out = *ctr_in + 1
+  // (ctr_in == ctr_out, or two slots when the launch refreshes the next
+  // step from *ctr_in); NULL = none
+  const uint64_t *ctr_in;
+  uint64_t *ctr_out;
   const int2 *wave_len;        // per force wavefront: (longest list, total) from a
                                // list build of the same tiling, or NULL
   int xcd_remap;               // 1: contiguous block ranges per XCD
   int has_shift;               // some pair type has a nonzero energy shift
   int persist_waves;           // mixed form: > 0 = persistent launch of that many
                                // wavefronts, each taking a contiguous chunk of tiles
+  NextRefresh next;            // the next step's refresh (perturbation mode)
 };
 // unroll: list rows per group of the mixed kernel (1, 2 or 4)
 void launch_force(int precision, int lanes, int unroll, int block, bool stats,
