@@ -464,11 +464,8 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->sb.val_b, N));
   HIP_TRY(dalloc(&e->sb.tkey, N));
   HIP_TRY(dalloc(&e->sb.tval, N));
-  // slots for the refresh's wavefronts and for the force tiles of any lane
-  // count (a graph step that refreshes the next step keeps one per tile)
-  const int skin_cap = std::max(skin_part_count(N), force_tile_count(N, 16));
-  HIP_TRY(dalloc(&e->d_skin_part, skin_cap));
-  launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_cap, e->stream);
+  HIP_TRY(dalloc(&e->d_skin_part, skin_part_count(N)));
+  launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_part_count(N), e->stream);
   HIP_TRY(dalloc(&e->d_pos4_alt, N));
   HIP_TRY(dalloc(&e->d_backup_s, N));
   if (e->precision != SRE_PREC_MIXED) HIP_TRY(dalloc(&e->d_res4_alt, N));
@@ -493,13 +490,6 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
 static sre_status md_update_total_mass(sre_engine *e);
 
 // The host geometry of the only replica (position refresh argument), or NULL
-// Skin partial slots in use: the refresh's wavefronts and the force tiles of
-// the list's lane count.
-static int skin_slots(const sre_engine *e) {
-  const int T = e->list_lanes > 0 ? e->list_lanes : e->tune.force_lanes;
-  return std::max(skin_part_count(e->n_atoms), force_tile_count(e->n_atoms, T));
-}
-
 static const RepInfo *rep0(const sre_engine *e) {
   return e->n_rep == 1 ? &e->reps[0] : nullptr;
 }
@@ -947,7 +937,7 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);
   enqueue_build(e, st);
-  launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_slots(e),
+  launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_part_count(e->n_atoms),
                       e->d_skin_part, 1, reinterpret_cast<unsigned *>(e->d_counters + 2),
                       st, e->d_log, e->d_log_idx, kRebuildLog);
 }
@@ -1067,7 +1057,7 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   HIP_TRY(hipGetLastError());
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
-    launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
+    launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
                         e->d_skin_part, 0, e->d_flags, e->stream);
     unsigned bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
@@ -1522,19 +1512,10 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
                       t->build_mode != e->tune.build_mode ||
                       t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
   if (memcmp(&e->tune, t, sizeof *t) != 0) e->graph_epoch++;  // captured launches
-  const bool lanes = t->force_lanes != e->tune.force_lanes;
   e->tune = *t;
   if (layout) {
     e->built = false;
     e->cap = 0;
-  }
-  if (lanes && e->d_skin_part) {
-    // the tile count changes: every skin slot starts from zero again
-    launch_fill_u32((uint32_t *)e->d_skin_part, 0u,
-                    (size_t)std::max(skin_part_count(e->n_atoms),
-                                     force_tile_count(e->n_atoms, 16)),
-                    e->stream);
-    HIP_TRY(hipGetLastError());
   }
   return SRE_OK;
 }
@@ -1783,6 +1764,8 @@ static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
   refresh_tail_values(e);
   NextRefresh nx{};
   nx.active = 1;
+  // one thread per atom, blocks of the force launch's size, a multiple of 8
+  nx.n_blocks = ((e->n_atoms + e->tune.block - 1) / e->tune.block + 7) / 8 * 8;
   nx.perm = e->sb.val_a;
   nx.pos = const_cast<float *>(e->d_pos);
   nx.stride = e->pos_stride;
@@ -2015,7 +1998,7 @@ extern "C" sre_status sre_check_skin_flag(sre_engine *e) {
   if (!e) return fail(SRE_ERR_INVALID, "null engine");
   HIP_TRY(hipSetDevice(e->device));
   if (!e->d_skin_part) return fail(SRE_ERR_INVALID, "system not set");
-  launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
+  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
                       e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
@@ -2255,7 +2238,7 @@ extern "C" sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in) {
 // the force-range flag, in one readback.  A skin violation is reported first:
 // missed pairs are the likelier cause of a range error that follows it.
 static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
-  launch_rebuild_info(0, nullptr, nullptr, skin_slots(e),
+  launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
                       e->d_skin_part, 0, e->d_flags, e->stream);
   unsigned bits[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(bits, e->d_flags, 2 * sizeof(unsigned),

@@ -126,115 +126,75 @@ __global__ __launch_bounds__(256) void k_update_positions(
 
 // ---------------------------------------- next-step refresh in a force launch --
 //
-// The atom at sorted slot si of a force tile is refreshed by its group of T
-// lanes: with T >= 4 lane c < 3 takes component c (one Gaussian each) and
-// lane 3 the type/replica word, so the 16-byte record is written by four
-// lanes; with T < 4 the group's first lane does all of it.  The inputs are
-// loaded when the launch starts (their latency hides behind the pair loop).
-struct RefreshIn {
-  int i;          // caller index
-  uint4 b;        // rebuild-time lattice position
-  float o[3];     // positions perturbed about (the lane's components)
-  float prev;     // the tile's running maximum (first lane of the wave)
-};
-
-template <int T>
-__device__ __forceinline__ RefreshIn refresh_load(const NextRefresh &nx, int si, bool act,
-                                                  int l, int lane, int tile) {
-  RefreshIn r;
-  r.i = act ? (int)nx.perm[si] : 0;   // waited for only where stored to
-  r.b = make_uint4(0u, 0u, 0u, 0u);
-  r.o[0] = r.o[1] = r.o[2] = 0.f;
-  if (act) {
-    r.b = nx.backup_s[si];
-    // anchored: the cell-order copy (no dependent load); free walk: the
-    // caller-order positions of this step
-    const float *o = nx.anchor_s ? reinterpret_cast<const float *>(nx.anchor_s + si)
-                                 : nx.pos + (size_t)r.i * nx.stride;
-    if (T >= 4) {
-      if (l < 3) r.o[0] = o[l];
-    } else if (l == 0) {
-      r.o[0] = o[0];
-      r.o[1] = o[1];
-      r.o[2] = o[2];
-    }
-  }
-  r.prev = lane == 0 ? nx.skin_part[tile] : 0.f;
-  return r;
-}
-
-__device__ __forceinline__ uint32_t comp_of(const uint4 &v, int c) {
-  return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
-}
-
-// Writes the refreshed atom and folds its squared displacement into the
-// tile's running maximum (the same operations as k_update_positions).
-template <int T>
-__device__ __forceinline__ void refresh_store(const NextRefresh &nx, const RepInfo *reps,
-                                              uint64_t step, const RefreshIn &r, int si,
-                                              bool act, int l, int lane, int tile) {
+// The first nx.n_blocks blocks of a force launch that refreshes the next
+// step do only that, one atom per thread in cell order (slot s = block *
+// blockDim + thread): the arithmetic of k_update_positions with an
+// anchored or free perturbation, reading the cell-order copies of the
+// rebuild-time lattice positions and of the anchor (coalesced) and writing
+// the next coordinate buffer (coalesced) and the caller's positions.  They
+// run beside the force wavefronts and fill their idle issue slots; each
+// wavefront keeps its running maximum in skin_part[s / 64].
+__device__ __forceinline__ void refresh_block(const NextRefresh &nx, const RepInfo *reps,
+                                              int n, uint64_t step, int rb) {
+  const int s = rb * (int)blockDim.x + (int)threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = s / kWave;
+  const bool wslot = w * kWave < n;
+  const float prev = (lane == 0 && wslot) ? nx.skin_part[w] : 0.f;
   float d2 = 0.f;
-  if (T >= 4) {
-    float dmine = 0.f;
-    if (act && l < 4) {
-      uint32_t word = r.b.w;
-      float res = 0.f;
-      if (l < 3) {
-        // this lane's axis of the geometry (constant indices only, so that
-        // nothing is indexed in private memory)
-        float q;
-        double invq, qd;
-        if (nx.one_rep) {
-          q = l == 0 ? nx.g1.q[0] : (l == 1 ? nx.g1.q[1] : nx.g1.q[2]);
-          invq = l == 0 ? nx.g1.invq[0] : (l == 1 ? nx.g1.invq[1] : nx.g1.invq[2]);
-          qd = l == 0 ? nx.g1.qd[0] : (l == 1 ? nx.g1.qd[1] : nx.g1.qd[2]);
-        } else {
-          const RepInfo &R = reps[r.b.w >> 8];
-          q = l == 0 ? R.q[0] : (l == 1 ? R.q[1] : R.q[2]);
-          invq = l == 0 ? R.invq[0] : (l == 1 ? R.invq[1] : R.invq[2]);
-          qd = l == 0 ? R.qd[0] : (l == 1 ? R.qd[1] : R.qd[2]);
-        }
-        const float x = r.o[0] + nx.sigma * gauss_component(gauss_key(nx.seed, step),
-                                                             (uint64_t)r.i, l);
-        nx.pos[(size_t)r.i * nx.stride + l] = x;
-        word = quantize(x, invq);
-        if (nx.res4) res = quant_residual(x, invq, qd);
-        dmine = disp(word, comp_of(r.b, l), q);
-      }
-      reinterpret_cast<uint32_t *>(nx.pos4 + si)[l] = word;
-      if (nx.res4) reinterpret_cast<float *>(nx.res4 + si)[l] = res;
+  if (s < n) {
+    const uint32_t i = nx.perm[s];
+    const uint4 b = nx.backup_s[s];
+    float ox, oy, oz;
+    if (nx.anchor_s) {
+      const float4 o = nx.anchor_s[s];
+      ox = o.x;
+      oy = o.y;
+      oz = o.z;
+    } else {
+      const float *o = nx.pos + (size_t)i * nx.stride;
+      ox = o[0];
+      oy = o[1];
+      oz = o[2];
     }
-    // the group's first quad holds the three components
-    const float dx = quad_lane<0>(dmine), dy = quad_lane<1>(dmine), dz = quad_lane<2>(dmine);
-    d2 = act && l < 4 ? dist2(dx, dy, dz) : 0.f;
-  } else if (act && l == 0) {
-    const RepInfo &R = reps[r.b.w >> 8];
-    Geom G;
-    for (int d = 0; d < 3; ++d) {
-      G.q[d] = nx.one_rep ? nx.g1.q[d] : R.q[d];
-      G.invq[d] = nx.one_rep ? nx.g1.invq[d] : R.invq[d];
-      G.qd[d] = nx.one_rep ? nx.g1.qd[d] : R.qd[d];
-    }
-    const float3 g = gauss3(nx.seed, step, (uint64_t)r.i);
-    const float x = r.o[0] + nx.sigma * g.x;
-    const float y = r.o[1] + nx.sigma * g.y;
-    const float z = r.o[2] + nx.sigma * g.z;
-    float *p = nx.pos + (size_t)r.i * nx.stride;
+    const float3 g = gauss3(nx.seed, step, (uint64_t)i);
+    const float x = ox + nx.sigma * g.x;
+    const float y = oy + nx.sigma * g.y;
+    const float z = oz + nx.sigma * g.z;
+    float *p = nx.pos + (size_t)i * nx.stride;
     p[0] = x;
     p[1] = y;
     p[2] = z;
+    const Geom G = nx.one_rep ? nx.g1 : geom_of(reps[b.w >> 8]);
     const uint32_t ux = quantize(x, G.invq[0]);
     const uint32_t uy = quantize(y, G.invq[1]);
     const uint32_t uz = quantize(z, G.invq[2]);
-    nx.pos4[si] = make_uint4(ux, uy, uz, r.b.w);
+    nx.pos4[s] = make_uint4(ux, uy, uz, b.w);
     if (nx.res4)
-      nx.res4[si] = make_float4(quant_residual(x, G.invq[0], G.qd[0]),
-                                quant_residual(y, G.invq[1], G.qd[1]),
-                                quant_residual(z, G.invq[2], G.qd[2]), 0.f);
-    d2 = dist2(disp(ux, r.b.x, G.q[0]), disp(uy, r.b.y, G.q[1]), disp(uz, r.b.z, G.q[2]));
+      nx.res4[s] = make_float4(quant_residual(x, G.invq[0], G.qd[0]),
+                               quant_residual(y, G.invq[1], G.qd[1]),
+                               quant_residual(z, G.invq[2], G.qd[2]), 0.f);
+    d2 = dist2(disp(ux, b.x, G.q[0]), disp(uy, b.y, G.q[1]), disp(uz, b.z, G.q[2]));
   }
   d2 = wave_max_nonneg(d2);
-  if (lane == 0 && d2 > r.prev) nx.skin_part[tile] = d2;
+  if (lane == 0 && wslot && d2 > prev) nx.skin_part[w] = d2;
+}
+
+// Block index and count of the force part of a launch (the refresh blocks,
+// a multiple of 8 so that the XCD map is unchanged, come first).
+template <bool NEXT>
+__device__ __forceinline__ bool force_block(const ForceArgs &a, int *bid, int *nb) {
+  *bid = blockIdx.x;
+  *nb = gridDim.x;
+  if (NEXT) {
+    if (*bid < a.next.n_blocks) {
+      refresh_block(a.next, a.reps, a.n, *a.ctr_in + 1, *bid);
+      return false;
+    }
+    *bid -= a.next.n_blocks;
+    *nb -= a.next.n_blocks;
+  }
+  return true;
 }
 
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
@@ -377,7 +337,7 @@ __device__ __forceinline__ int wave_max_int(int v) {
 //
 // One wave-tile (64/T consecutive sorted atoms) of the force evaluation;
 // `wave` is the tile index.
-template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT, bool NEXT>
+template <int T, int U, bool STATS, bool TAIL, bool MULTI, bool SHIFT>
 __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *sp,
                                            const int wave) {
   constexpr int A = kWave / T;
@@ -389,11 +349,6 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   if (s0 >= a.n) return;  // whole wave beyond the atoms
 
   const int si = act ? s : s0;
-  // the next step's refresh of this tile: inputs loaded first (with the
-  // loads below); its arithmetic and stores run while the loop's first list
-  // rows are in flight (it needs nothing this launch computes)
-  RefreshIn rin{};
-  if (NEXT) rin = refresh_load<T>(a.next, si, act, l, lane, wave);
   const uint4 pi = a.pos4[si];
   const int rep = (int)(pi.w >> 8);
   const RepInfo &R = a.reps[rep];
@@ -500,10 +455,6 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
   // the gathers of g and the rows of g+2), so that the loop waits only for
   // the loads it is about to use
   asm volatile("" ::: "memory");
-  if (NEXT) {
-    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, si, act, l, lane, wave);
-    asm volatile("" ::: "memory");
-  }
   prefetch(pa, ja, 0);
   // the accumulators' FP64 adds round toward zero inside the pair loop
   // (FP32 rounding is a separate field of the mode register, untouched)
@@ -581,17 +532,19 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
       sp[t] = a.params[t];
     __syncthreads();
   }
-  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  int bid, nb;
+  if (!force_block<NEXT>(a, &bid, &nb)) return;
+  const int blk = a.xcd_remap ? xcd_block(bid, nb) : bid;
   const int wave = __builtin_amdgcn_readfirstlane((blk * blockDim.x + threadIdx.x) / kWave);
   if (a.persist_waves <= 0) {
-    force_tile<T, U, STATS, TAIL, MULTI, SHIFT, NEXT>(a, sp, wave);
+    force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, wave);
     return;
   }
   constexpr int A = kWave / T;
   const int n_tiles = (a.n + A - 1) / A;
   const int per = (n_tiles + a.persist_waves - 1) / a.persist_waves;
   const int t0 = wave * per, t1 = min(n_tiles, t0 + per);
-  for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT, NEXT>(a, sp, t);
+  for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, t);
 }
 
 // --------------------------------------------------------------- FP64 ----
@@ -608,27 +561,19 @@ __global__ __launch_bounds__(256) void k_force_fp64(ForceArgs a) {
   constexpr int A = kWave / T;
   const int lane = threadIdx.x & (kWave - 1);
   const int l = lane % T;
-  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  int bid, nb;
+  if (!force_block<NEXT>(a, &bid, &nb)) return;
+  const int blk = a.xcd_remap ? xcd_block(bid, nb) : bid;
   const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + lane / T;
   const bool act = s < a.n;
   const int s0 = wave * A;
   if (s0 >= a.n) return;
-  // the next step's refresh of this tile (inputs first, then this atom's
-  // loads, then the refresh while those are in flight; outside the per-atom
-  // branch, as it reduces over the wavefront)
-  RefreshIn rin{};
-  if (NEXT) rin = refresh_load<T>(a.next, act ? s : s0, act, l, lane, wave);
   const int sa = act ? s : s0;
   const uint4 pi = a.pos4[sa];
   const float4 ri = a.res4[sa];
   const float rc = a.rcut[sa];
   const int nn = act ? a.nnb[sa] : 0;
-  if (NEXT) {
-    asm volatile("" ::: "memory");
-    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, sa, act, l, lane, wave);
-    asm volatile("" ::: "memory");
-  }
 
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
   if (act) {
@@ -721,15 +666,15 @@ __global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
   constexpr int A = kWave / T;
   const int lane = threadIdx.x & (kWave - 1);
   const int l = lane % T;
-  const int blk = a.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  int bid, nb;
+  if (!force_block<NEXT>(a, &bid, &nb)) return;
+  const int blk = a.xcd_remap ? xcd_block(bid, nb) : bid;
   const int wave = (blk * blockDim.x + threadIdx.x) / kWave;
   const int s = wave * A + lane / T;
   const bool act = s < a.n;
   const int s0 = wave * A;
   if (s0 >= a.n) return;
   const int si = act ? s : s0;
-  RefreshIn rin{};
-  if (NEXT) rin = refresh_load<T>(a.next, si, act, l, lane, wave);
   const uint4 pi = a.pos4[si];
   const RepInfo &R = a.reps[pi.w >> 8];
   const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
@@ -790,10 +735,6 @@ __global__ __launch_bounds__(256) void k_force_fp64_piped(ForceArgs a) {
   };
   int ja = row(0), jb = row(1);
   asm volatile("" ::: "memory");
-  if (NEXT) {   // the next step's refresh while the first rows are in flight
-    refresh_store<T>(a.next, a.reps, *a.ctr_in + 1, rin, si, act, l, lane, wave);
-    asm volatile("" ::: "memory");
-  }
   Pj qa, qb;
   fetch(qa, ja, 0);
   int g = 0;
@@ -899,6 +840,8 @@ static void launch_force_t(int precision, int unroll, int block,
   int grid = (waves * kWave + block - 1) / block;
   const bool multi = a.n_types > 1;
   if (precision == 0 && a.persist_waves > 0) grid = a.persist_waves * kWave / block;
+  // the next step's refresh: blocks of its own in front (a multiple of 8)
+  if (a.next.active) grid += a.next.n_blocks;
   if (precision == 0) {
     if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
     else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);

@@ -434,13 +434,8 @@ void launch_md_finish(int n, const double *part, MDState *state, float *vel,
                       hipStream_t st);
 // Position refresh + skin test: each wavefront keeps the largest squared
 // displacement it has seen since the last rebuild in skin_part[wave]
-// (skin_part_count(n) entries); a force launch that refreshes the next step
-// keeps one per force tile (force_tile_count entries).
+// (skin_part_count(n) entries).
 inline int skin_part_count(int n) { return (n + kWave - 1) / kWave; }
-inline int force_tile_count(int n, int lanes) {
-  const int A = kWave / lanes;
-  return (n + A - 1) / A;
-}
 // backup and inv_perm are in caller order (written by the sorted gather)
 // host_rep0: the host copy of the only replica's geometry (NULL with several)
 void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
@@ -450,14 +445,15 @@ void launch_update_positions(int n, const uint32_t *inv_perm, float *pos,
                              const DriftArgs *drift, hipStream_t st);
 
 // The next step's position refresh folded into a force launch (perturbation
-// mode, no integrator): after its forces, each tile's wavefront refreshes
-// the tile's atoms into a second coordinate buffer -- the same arithmetic
-// as k_update_positions with a perturbation at step *ctr_in + 1 -- so that
-// the launch of step s also produces the coordinates of step s + 1 while
-// its own gathers read those of step s.  Each tile keeps its running
-// maximum squared displacement in skin_part[tile] (no atomics).
+// mode, no integrator): n_blocks blocks in front of the force blocks
+// refresh every atom into a second coordinate buffer -- the arithmetic of
+// k_update_positions with a perturbation at step *ctr_in + 1 -- so that the
+// launch of step s also produces the coordinates of step s + 1 while its
+// force wavefronts read those of step s.  Each refresh wavefront keeps its
+// running maximum squared displacement in skin_part (no atomics).
 struct NextRefresh {
   int active;
+  int n_blocks;                // refresh blocks (a multiple of 8)
   const uint32_t *perm;        // sorted slot -> caller index
   float *pos;                  // caller positions (the new ones written back)
   int stride;
@@ -466,7 +462,7 @@ struct NextRefresh {
   const uint4 *backup_s;       // rebuild-time lattice positions (cell order)
   uint4 *pos4;                 // the next step's lattice positions (sorted)
   float4 *res4;                // FP64 form: the next step's residuals, or NULL
-  float *skin_part;            // running maximum per tile
+  float *skin_part;            // running maximum per 64 cell-order atoms
   float sigma;
   uint64_t seed;
   Geom g1;                     // the geometry when one_rep
