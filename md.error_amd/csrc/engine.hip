@@ -53,7 +53,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2, 0};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2, 0, 0};
   int n_simd = 1024;               // SIMDs of the device (4 per CU)
 
   // system
@@ -109,6 +109,8 @@ struct sre_engine {
   // cells and sort
   SortBufs sb{};
   size_t counts_cap = 0;
+  uint32_t *d_cell_count = nullptr;  // counting sort: zero between rebuilds
+  size_t cell_count_cap = 0;
   int key_bits = 1;
   int n_cells = 0;
   int *d_start = nullptr;
@@ -234,7 +236,7 @@ sre_engine::~sre_engine() {
   dfree(d_tail_entry_rep); dfree(d_tail_atom); dfree(d_pos_own);
   dfree(sb.key_a); dfree(sb.val_a); dfree(sb.key_b); dfree(sb.val_b);
   dfree(sb.tkey); dfree(sb.tval); dfree(sb.counts); dfree(sb.offsets);
-  dfree(sb.scan_tmp); dfree(d_start); dfree(d_pos4); dfree(d_backup);
+  dfree(sb.scan_tmp); dfree(d_cell_count); dfree(d_start); dfree(d_pos4); dfree(d_backup);
   dfree(d_inv_perm);
   dfree(d_rcut); dfree(d_rc_atom); dfree(d_tail_idx); dfree(d_list); dfree(d_nnb); dfree(d_res4);
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
@@ -769,6 +771,13 @@ static sre_status prepare_tables(sre_engine *e) {
     e->start_cap = e->n_cells + 1;
     e->graph_epoch++;
   }
+  if (cell_count_capacity(e->n_cells) > e->cell_count_cap) {
+    const size_t m = cell_count_capacity(e->n_cells);
+    dfree(e->d_cell_count);
+    HIP_TRY(dalloc(&e->d_cell_count, m));
+    launch_fill_u32(e->d_cell_count, 0u, m, e->stream);
+    e->cell_count_cap = m;
+  }
   const size_t cc = radix_counts_capacity(e->n_atoms, e->key_bits);
   if (cc > e->counts_cap) {
     dfree(e->sb.counts);
@@ -884,7 +893,7 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
     v.insert(v.end(), c, c + n);
   };
   const int ints[] = {e->n_atoms, e->key_bits, e->n_cells, e->n_cols, e->n_rep,
-                      e->cap, e->tune.build_mode, e->tune.build_lanes,
+                      e->cap, e->tune.build_mode, e->tune.build_lanes, e->tune.sort_mode,
                       e->tune.force_lanes, e->n_types, e->pos_stride, n_parts,
                       (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) ? 1 : 0,
                       e->rc_explicit ? 1 : 0};
@@ -892,7 +901,8 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
   put(&e->skin, sizeof e->skin);
   const void *ptrs[] = {e->d_pos, e->d_rep_of, e->d_reps, e->sb.key_a, e->sb.key_b,
                         e->sb.val_a, e->sb.val_b, e->sb.tkey, e->sb.tval,
-                        e->sb.counts, e->sb.offsets, e->sb.scan_tmp, e->d_start,
+                        e->sb.counts, e->sb.offsets, e->sb.scan_tmp, e->d_cell_count,
+                        e->d_start,
                         e->d_types, e->d_rc, e->d_pos4, e->d_backup,
                         e->d_inv_perm, e->d_res4,
                         e->d_rcut, e->d_rc_atom, e->d_tail_idx, e->d_tail_atom,
@@ -920,19 +930,29 @@ static void enqueue_build(sre_engine *e, hipStream_t st) {
 }
 
 // The device part of a rebuild (capturable, no host decisions): cell keys,
-// radix sort, cell offsets, gather into cell order with rc assignment, exact
-// tail, list build, and the reduction of the longest/total list length and
-// of the segment's largest displacement (whose partials it clears).
+// binning by cell (counting or radix sort), cell offsets, gather into cell
+// order with rc assignment, exact tail, list build, and the reduction of the
+// longest/total list length and of the segment's largest displacement (whose
+// partials it clears).
 static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   const int N = e->n_atoms;
-  launch_radix_sort(N, e->key_bits, e->d_pos, e->pos_stride, e->d_rep_of,
-                    e->d_reps, e->sb, st);
-  launch_gather_sorted(N, e->n_cells, e->sb.key_a, e->sb.val_a, e->d_pos,
-                       e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
-                       e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
-                       e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
-                       e->rc_explicit ? e->d_rc_atom : nullptr, st, e->d_backup_s,
-                       e->pert_anchored ? e->d_anchor : nullptr, e->d_anchor_s);
+  const float *anchor = e->pert_anchored ? e->d_anchor : nullptr;
+  const float *rc_atom = e->rc_explicit ? e->d_rc_atom : nullptr;
+  if (e->tune.sort_mode == 0) {
+    launch_cell_sort_gather(N, e->n_cells, e->d_pos, e->pos_stride, e->d_types,
+                            e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->sb,
+                            e->d_cell_count, e->d_start, e->d_pos4, e->d_backup,
+                            e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
+                            rc_atom, e->d_backup_s, anchor, e->d_anchor_s, st);
+  } else {
+    launch_radix_sort(N, e->key_bits, e->d_pos, e->pos_stride, e->d_rep_of,
+                      e->d_reps, e->sb, st);
+    launch_gather_sorted(N, e->n_cells, e->sb.key_a, e->sb.val_a, e->d_pos,
+                         e->pos_stride, e->d_types, e->d_rep_of, e->d_reps,
+                         e->d_rc, e->n_types, e->d_start, e->d_pos4, e->d_backup,
+                         e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
+                         rc_atom, st, e->d_backup_s, anchor, e->d_anchor_s);
+  }
   if (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT)
     launch_tail_exact(N, e->n_types, e->d_pos4, e->d_rcut, e->d_reps,
                       e->d_paramsd, e->d_rho, e->d_tail_atom, st);
@@ -1507,6 +1527,8 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "force_unroll must be 1, 2 or 4");
   if (t->persistent < 0 || t->persistent > 16)
     return fail(SRE_ERR_INVALID, "persistent must be in [0, 16]");
+  if (t->sort_mode != 0 && t->sort_mode != 1)
+    return fail(SRE_ERR_INVALID, "sort_mode must be 0 or 1");
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
@@ -1597,7 +1619,7 @@ extern "C" sre_status sre_autotune(sre_engine *e, unsigned flags,
         if (!bar) continue;
         *bar = 0;
         if (strcmp(line, sig) == 0) {
-          sre_tuning t;
+          sre_tuning t = e->tune;
           if (sscanf(bar + 1, "%d %d %d %f %f %d %d %d %d", &t.force_lanes,
                      &t.build_lanes, &t.block, &t.cell_xy, &t.cell_z,
                      &t.build_mode, &t.xcd_remap, &t.force_unroll, &t.persistent) == 9) {

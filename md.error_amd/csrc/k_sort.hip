@@ -1,18 +1,21 @@
-// k_sort.hip -- cell keys, atomic-free LSD radix sort, cell offsets, and the
-// gather of per-atom data into cell order.
+// k_sort.hip -- cell keys, binning of the atoms by cell, cell offsets, and
+// the gather of per-atom data into cell order.
 //
 // The reference bins atoms with fixed-capacity cells and atomicInc slot
 // insertion (H/src/NeighborList.cu:84-167), whose in-cell order depends on
-// scheduling.  Here binning is a stable LSD radix sort of the cell key over
-// digits of at most 8 bits, each pass built from
-//   (1) a per-block stable ranking of 1024 keys: lanes holding the same digit
-//       find each other with one ballot per digit bit (their rank is the
-//       number of such lanes below them), per-wave digit counts go through
-//       LDS, and the keys are written in (digit, original position) order;
-//   (2) a digit-major exclusive scan of the per-block digit counts;
-//   (3) a stable scatter to the global positions.
-// No atomics are used, so the cell order (and everything derived from it) is
-// a pure function of the input.
+// scheduling.  Here the cell order is always the stable order (cell, caller
+// index), a pure function of the input, produced two ways:
+//   * a counting sort by cell (the rebuild's default, described below at
+//     k_cell_count), whose atomics only place atoms in a cell's slots before
+//     a rank by caller index fixes their order;
+//   * an atomic-free LSD radix sort of the cell key over digits of at most 8
+//     bits (the allocation-free path and a tuning option), each pass built
+//     from (1) a per-block stable ranking of 1024 keys: lanes holding the
+//     same digit find each other with one ballot per digit bit (their rank is
+//     the number of such lanes below them), per-wave digit counts go through
+//     LDS, and the keys are written in (digit, original position) order;
+//     (2) a digit-major exclusive scan of the per-block digit counts; (3) a
+//     stable scatter to the global positions.
 #include "sre_internal.h"
 
 namespace sre {
@@ -336,15 +339,50 @@ int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
 
 // ------------------------------------------------------------- cell start --
 
-// One thread per sorted slot s (plus one): gathers atom i = perm[s] into
-// cell order -- lattice coordinates, type/replica word, FP64 sub-lattice
-// residual, rc from the table (the reference's AssignRCut rule) and
-// tail-table index -- records inv_perm[i] = s and the rebuild-time lattice
-// position backup[i] (caller order, read by the position refresh), and fills
-// the cell offsets: start[c] = first sorted
-// position with key >= c, thread s covering the cells from key[s-1] + 1 to
-// key[s] (thread n: the cells after the last key), so every entry of
-// start[0..n_cells] is written exactly once.
+// Atom i into sorted slot d: lattice coordinates, type/replica word, FP64
+// sub-lattice residual, rc from the table (the reference's AssignRCut rule)
+// and tail-table index; records inv_perm[i] = d and the rebuild-time lattice
+// position backup[i] (caller order, read by the position refresh) and, when
+// given, its cell-order copy and the cell-order anchor.
+__device__ __forceinline__ void gather_atom(
+    int d, int i, const float *pos, int stride, const int *types,
+    const int *rep_of, const RepInfo *reps, const float *rc_table, int n_types,
+    uint4 *pos4, uint4 *backup, uint32_t *inv_perm, float4 *res4, float *rcut,
+    int *tail_idx, const float *rc_atom, uint4 *backup_s, const float *anchor,
+    float4 *anchor_s) {
+  const int r = rep_of[i];
+  const RepInfo &R = reps[r];
+  const float *p = pos + (size_t)i * stride;
+  const float x = p[0], y = p[1], z = p[2];
+  const int ty = types[i];
+  uint4 v = make_uint4(quantize(x, R.invq[0]), quantize(y, R.invq[1]),
+                       quantize(z, R.invq[2]),
+                       (uint32_t)ty | ((uint32_t)r << 8));
+  pos4[d] = v;
+  backup[i] = v;
+  inv_perm[i] = (uint32_t)d;
+  if (backup_s) backup_s[d] = v;
+  if (anchor_s) {
+    const float *a = anchor + (size_t)i * stride;
+    anchor_s[d] = make_float4(a[0], a[1], a[2], 0.f);
+  }
+  if (res4)
+    res4[d] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
+                          quant_residual(y, R.invq[1], R.qd[1]),
+                          quant_residual(z, R.invq[2], R.qd[2]), 0.f);
+  const int ix = rc_bin(x, R.L[0], R.rc_n[0]);
+  const int iy = rc_bin(y, R.L[1], R.rc_n[1]);
+  const int iz = rc_bin(z, R.L[2], R.rc_n[2]);
+  rcut[d] = rc_atom ? rc_atom[i]
+                    : rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
+  tail_idx[d] = (R.tail_off >= 0) ? (R.tail_off + iz) * n_types + ty : -1;
+}
+
+// One thread per sorted slot s (plus one): gathers atom perm[s] into slot s
+// and fills the cell offsets: start[c] = first sorted position with key >= c,
+// thread s covering the cells from key[s-1] + 1 to key[s] (thread n: the
+// cells after the last key), so every entry of start[0..n_cells] is written
+// exactly once.
 __global__ __launch_bounds__(256) void k_gather_sorted(
     int n, int n_cells, const uint32_t *keys, const uint32_t *perm,
     const float *pos, int stride, const int *types, const int *rep_of,
@@ -362,33 +400,188 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
     for (int c = lo; c <= hi; ++c) start[c] = s;
   }
   if (s == n) return;
-  const int i = (int)perm[s];
-  const int r = rep_of[i];
-  const RepInfo &R = reps[r];
-  const float *p = pos + (size_t)i * stride;
-  const float x = p[0], y = p[1], z = p[2];
-  const int ty = types[i];
-  uint4 v = make_uint4(quantize(x, R.invq[0]), quantize(y, R.invq[1]),
-                       quantize(z, R.invq[2]),
-                       (uint32_t)ty | ((uint32_t)r << 8));
-  pos4[s] = v;
-  backup[i] = v;
-  inv_perm[i] = (uint32_t)s;
-  if (backup_s) backup_s[s] = v;
-  if (anchor_s) {
-    const float *a = anchor + (size_t)i * stride;
-    anchor_s[s] = make_float4(a[0], a[1], a[2], 0.f);
+  gather_atom(s, (int)perm[s], pos, stride, types, rep_of, reps, rc_table, n_types,
+              pos4, backup, inv_perm, res4, rcut, tail_idx, rc_atom, backup_s,
+              anchor, anchor_s);
+}
+
+// ------------------------------------------------------ cell counting sort --
+//
+// The rebuild's default binning.  Keys are cell indices, so one count per
+// cell fits in memory and a counting sort needs one histogram, one scan and
+// one scatter instead of a radix pass per 8 key bits:
+//   (1) k_cell_count: the key of atom i (caller order), count[key] += 1;
+//   (2) the exclusive scan of the counts, start[c] = first slot of cell c;
+//   (3) the scatter: cursor[key] += 1 gives each atom a distinct slot of its
+//       cell, in an order that depends on scheduling;
+//   (4) k_gather_cells: the atom in a slot of cell c moves to slot
+//       start[c] + (number of atoms of c with a smaller caller index), which
+//       is the stable order (cell, caller index) -- the radix sort's order --
+//       so the result is a pure function of the input; it also sets count
+//       and cursor of its cell back to zero for the next rebuild (every cell
+//       with a non-zero entry holds an atom).
+// Up to kPlaceCells cells, (2) and (3) are one launch (k_cell_place): every
+// block of 1024 atoms scans all the counts into LDS itself (16 per thread,
+// up to 60 KB read from L2 per block) instead of waiting for a separate scan
+// launch; larger grids use k_cell_scan and k_cell_scatter.  A cell holds a
+// few atoms (its volume is a fraction of (rc + skin)^3 and the atoms
+// repel), so the rank loop of (4) is short.
+
+// cells scanned by one 1024-thread block; its LDS holds the offsets with one
+// pad word per 16 (thread t writes words 17 t + e, so the 64 lanes of a wave
+// spread over the banks), 16320 words of the 16384 a block may hold
+constexpr int kPlaceCells = 15360;
+__device__ __forceinline__ int place_idx(int c) { return c + (c >> 4); }
+
+__global__ __launch_bounds__(256) void k_cell_count(int n, int n_cells, const float *pos,
+                                                    int stride, const int *rep_of,
+                                                    const RepInfo *reps, uint32_t *keys,
+                                                    uint32_t *count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // keys are < n_cells by construction; the clamp keeps the counts in bounds
+  const uint32_t k = min(cell_key(pos, stride, rep_of, reps, i), (uint32_t)(n_cells - 1));
+  keys[i] = k;
+  atomicAdd(&count[k], 1u);
+}
+
+// One tile of the scan of counts [base, base + 16384) (count holds zeros up
+// to a multiple of 16 past m): thread t takes 16 consecutive counts (four
+// 16-byte loads); returns the tile total, writes the exclusive prefix plus
+// carry through out(c, value).
+template <typename Out>
+__device__ __forceinline__ uint32_t scan_tile16(int m, int base, const uint32_t *count,
+                                                uint32_t carry, uint32_t *tmp, Out out) {
+  constexpr int E = 16;
+  const int i0 = base + (int)threadIdx.x * E;
+  uint32_t v[E];
+  if (i0 < m) {
+    const uint4 *c4 = reinterpret_cast<const uint4 *>(count + i0);
+#pragma unroll
+    for (int q = 0; q < E / 4; ++q) {
+      const uint4 w = c4[q];
+      v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = 0u;
   }
-  if (res4)
-    res4[s] = make_float4(quant_residual(x, R.invq[0], R.qd[0]),
-                          quant_residual(y, R.invq[1], R.qd[1]),
-                          quant_residual(z, R.invq[2], R.qd[2]), 0.f);
-  const int ix = rc_bin(x, R.L[0], R.rc_n[0]);
-  const int iy = rc_bin(y, R.L[1], R.rc_n[1]);
-  const int iz = rc_bin(z, R.L[2], R.rc_n[2]);
-  rcut[s] = rc_atom ? rc_atom[i]
-                    : rc_table[R.rc_off + iz + R.rc_n[2] * (iy + R.rc_n[1] * ix)];
-  tail_idx[s] = (R.tail_off >= 0) ? (R.tail_off + iz) * n_types + ty : -1;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) sum += v[e];
+  uint32_t tot = 0;
+  uint32_t ex = block_exscan_1024(sum, tmp, &tot) + carry;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (i0 + e < m) out(i0 + e, ex);
+    ex += v[e];
+  }
+  return tot;
+}
+
+// start[c] for c < m (any m), by one block, tile after tile.
+__global__ __launch_bounds__(kSortT) void k_cell_scan(int m, const uint32_t *count,
+                                                      int *start) {
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  uint32_t carry = 0;
+  for (int base = 0; base < m; base += kSortT * 16)
+    carry += scan_tile16(m, base, count, carry, tmp,
+                         [&](int c, uint32_t v) { start[c] = (int)v; });
+}
+
+__global__ __launch_bounds__(256) void k_cell_scatter(int n, int n_cells,
+                                                      const uint32_t *keys,
+                                                      int *start, uint32_t *cursor,
+                                                      uint32_t *skey, uint32_t *sval) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) start[n_cells] = n;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const int slot = start[k] + (int)atomicAdd(&cursor[k], 1u);  // in [start[k], start[k+1])
+  if ((unsigned)slot < (unsigned)n) {
+    skey[slot] = k;
+    sval[slot] = (uint32_t)i;
+  }
+}
+
+// (2) + (3) for m <= kPlaceCells: block b scans all counts into LDS, block 0
+// also writes them to start[0..m], then each thread places one atom.
+__global__ __launch_bounds__(kSortT) void k_cell_place(int n, int m, const uint32_t *keys,
+                                                       const uint32_t *count,
+                                                       uint32_t *cursor, int *start,
+                                                       uint32_t *skey, uint32_t *sval) {
+  __shared__ int s_start[kPlaceCells + kPlaceCells / 16];
+  __shared__ uint32_t tmp[kSortT / 64 + 1];
+  const bool first = blockIdx.x == 0;
+  scan_tile16(m, 0, count, 0u, tmp, [&](int c, uint32_t v) {
+    s_start[place_idx(c)] = (int)v;
+    if (first) start[c] = (int)v;
+  });
+  if (first && threadIdx.x == 0) start[m] = n;
+  __syncthreads();
+  const int i = blockIdx.x * kSortT + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];     // < m (clamped when counted)
+  const int slot = s_start[place_idx((int)k)] + (int)atomicAdd(&cursor[k], 1u);
+  if ((unsigned)slot < (unsigned)n) {
+    skey[slot] = k;
+    sval[slot] = (uint32_t)i;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_cells(
+    int n, const uint32_t *skey, const uint32_t *sval, const int *start,
+    uint32_t *count, uint32_t *cursor, uint32_t *perm, const float *pos,
+    int stride, const int *types, const int *rep_of, const RepInfo *reps,
+    const float *rc_table, int n_types, uint4 *pos4, uint4 *backup,
+    uint32_t *inv_perm, float4 *res4, float *rcut, int *tail_idx,
+    const float *rc_atom, uint4 *backup_s, const float *anchor, float4 *anchor_s) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t k = skey[s], i = sval[s];
+  count[k] = 0u;
+  cursor[k] = 0u;
+  const int lo = start[k], hi = min(start[k + 1], n);
+  int d = lo;
+  for (int t = lo; t < hi; ++t) d += sval[t] < i ? 1 : 0;
+  if ((unsigned)d >= (unsigned)n) return;       // only for inconsistent offsets
+  perm[d] = i;
+  gather_atom(d, (int)i, pos, stride, types, rep_of, reps, rc_table, n_types, pos4,
+              backup, inv_perm, res4, rcut, tail_idx, rc_atom, backup_s, anchor,
+              anchor_s);
+}
+
+static size_t cell_count_half(int n_cells) {
+  return ((size_t)n_cells + 15) / 16 * 16 + 16;
+}
+
+size_t cell_count_capacity(int n_cells) { return 2 * cell_count_half(n_cells); }
+
+void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
+                             const int *types, const int *rep_of,
+                             const RepInfo *reps, const float *rc_table,
+                             int n_types, const SortBufs &b, uint32_t *count,
+                             int *start, uint4 *pos4, uint4 *backup,
+                             uint32_t *inv_perm, float4 *res4, float *rcut,
+                             int *tail_idx, const float *rc_atom, uint4 *backup_s,
+                             const float *anchor, float4 *anchor_s, hipStream_t st) {
+  if (n <= 0) return;
+  uint32_t *cursor = count + cell_count_half(n_cells);
+  const dim3 g((n + 255) / 256), t(256);
+  hipLaunchKernelGGL(k_cell_count, g, t, 0, st, n, n_cells, pos, stride, rep_of, reps,
+                     b.tkey, count);
+  if (n_cells <= kPlaceCells) {
+    hipLaunchKernelGGL(k_cell_place, dim3((n + kSortT - 1) / kSortT), dim3(kSortT), 0, st,
+                       n, n_cells, b.tkey, count, cursor, start, b.key_b, b.val_b);
+  } else {
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(kSortT), 0, st, n_cells, count, start);
+    hipLaunchKernelGGL(k_cell_scatter, g, t, 0, st, n, n_cells, b.tkey, start, cursor,
+                       b.key_b, b.val_b);
+  }
+  hipLaunchKernelGGL(k_gather_cells, g, t, 0, st, n, b.key_b, b.val_b, start, count,
+                     cursor, b.val_a, pos, stride, types, rep_of, reps, rc_table,
+                     n_types, pos4, backup, inv_perm, res4, rcut, tail_idx, rc_atom,
+                     backup_s, anchor, anchor ? anchor_s : nullptr);
 }
 
 // AssignRCut::assign between rebuilds (H/src/AssignRCut.cu:77-117): every

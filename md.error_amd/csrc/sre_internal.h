@@ -313,6 +313,21 @@ int launch_radix_sort(int n, int key_bits, const float *pos, int stride,
                       const int *rep_of, const RepInfo *reps, const SortBufs &b,
                       hipStream_t st);
 size_t radix_counts_capacity(int n, int key_bits);
+// Counting sort by cell and gather into cell order (+ rc assignment): the
+// same permutation (val_a), cell offsets start[0..n_cells] and gathered data
+// as the radix sort followed by launch_gather_sorted.  count (the cell
+// counts and the placement cursors) holds cell_count_capacity(n_cells) zeros
+// before the call and again after it; uses b.tkey, b.key_b, b.val_b as
+// scratch.
+size_t cell_count_capacity(int n_cells);
+void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
+                             const int *types, const int *rep_of,
+                             const RepInfo *reps, const float *rc_table,
+                             int n_types, const SortBufs &b, uint32_t *count,
+                             int *start, uint4 *pos4, uint4 *backup,
+                             uint32_t *inv_perm, float4 *res4, float *rcut,
+                             int *tail_idx, const float *rc_atom, uint4 *backup_s,
+                             const float *anchor, float4 *anchor_s, hipStream_t st);
 // Gather into cell order (+ rc assignment) fused with the cell offsets
 // start[0..n_cells] computed from the sorted keys.
 void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
