@@ -241,8 +241,10 @@ typedef struct {
                            atoms (flat launch over all replicas); k = 1..16: a
                            persistent launch of k wavefronts per SIMD, each taking a
                            contiguous chunk of tiles (same sums) */
-  int sort_mode;        /* binning at a rebuild: 0 = counting sort by cell, 1 = LSD
-                           radix sort of the cell key (same cell order) */
+  int sort_mode;        /* binning at a rebuild (same cell order in every mode):
+                           0 = counting sort by cell up to 15360 cells, radix sort
+                           above; 1 = LSD radix sort of the cell key; 2 = counting
+                           sort at any cell count */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */

@@ -173,6 +173,7 @@ struct sre_engine {
   uint4 *d_backup_s = nullptr;
   float4 *d_anchor_s = nullptr;
   int2 *d_build_part = nullptr;    // per build wavefront: (longest, total)
+  int2 *d_info_slices = nullptr;   // scratch of the two-stage partial reduction
   int build_part_cap = 0;
   float *d_anchor = nullptr;       // anchored perturbation: positions at set-up
   size_t anchor_cap = 0, anchor_count = 0;
@@ -246,7 +247,7 @@ sre_engine::~sre_engine() {
   dfree(d_log_idx);
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
   dfree(d_atom_stats); dfree(d_tmp_out); dfree(d_flags); dfree(d_step_ctr);
-  dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part);
+  dfree(d_anchor); dfree(d_skin_part); dfree(d_build_part); dfree(d_info_slices);
   dfree(d_pos4_alt); dfree(d_res4_alt); dfree(d_backup_s); dfree(d_anchor_s);
   dfree(d_adj); dfree(d_adj_tw);
   dfree(d_mass); dfree(d_vel); dfree(d_fcall); dfree(d_md_part); dfree(d_md);
@@ -467,6 +468,7 @@ extern "C" sre_status sre_set_system(sre_engine *e, int n_replicas,
   HIP_TRY(dalloc(&e->sb.tkey, N));
   HIP_TRY(dalloc(&e->sb.tval, N));
   HIP_TRY(dalloc(&e->d_skin_part, skin_part_count(N)));
+  if (!e->d_info_slices) HIP_TRY(dalloc(&e->d_info_slices, 2 * kInfoSlices));
   launch_fill_u32((uint32_t *)e->d_skin_part, 0u, skin_part_count(N), e->stream);
   HIP_TRY(dalloc(&e->d_pos4_alt, N));
   HIP_TRY(dalloc(&e->d_backup_s, N));
@@ -910,6 +912,7 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
                         e->d_rho, e->d_list, e->d_nnb, e->d_build_part,
                         e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp,
                         e->d_counters, e->d_skin_part, e->d_log, e->d_log_idx,
+                        e->d_info_slices,
                         e->d_backup_s, e->pert_anchored ? e->d_anchor : nullptr,
                         e->d_anchor_s};
   put(ptrs, sizeof ptrs);
@@ -938,7 +941,12 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   const int N = e->n_atoms;
   const float *anchor = e->pert_anchored ? e->d_anchor : nullptr;
   const float *rc_atom = e->rc_explicit ? e->d_rc_atom : nullptr;
-  if (e->tune.sort_mode == 0) {
+  // counting sort while one block scans all cell counts (one placement
+  // launch); beyond that the radix sort's passes are cheaper than the
+  // counting sort's atomic round trips over many cells
+  const bool counting = e->tune.sort_mode == 2 ||
+                        (e->tune.sort_mode == 0 && e->n_cells <= kPlaceCells);
+  if (counting) {
     launch_cell_sort_gather(N, e->n_cells, e->d_pos, e->pos_stride, e->d_types,
                             e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->sb,
                             e->d_cell_count, e->d_start, e->d_pos4, e->d_backup,
@@ -959,7 +967,7 @@ static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
   enqueue_build(e, st);
   launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, skin_part_count(e->n_atoms),
                       e->d_skin_part, 1, reinterpret_cast<unsigned *>(e->d_counters + 2),
-                      st, e->d_log, e->d_log_idx, kRebuildLog);
+                      st, e->d_log, e->d_log_idx, kRebuildLog, e->d_info_slices);
 }
 
 // Queues one launch of the captured rebuild; returns its rebuild-log entry
@@ -1050,7 +1058,7 @@ static sre_status do_rebuild(sre_engine *e) {
     if (s != SRE_OK) return s;
     enqueue_build(e, st);
     launch_rebuild_info(n_parts, e->d_build_part, e->d_counters, 0, e->d_skin_part,
-                        0, nullptr, st);
+                        0, nullptr, st, nullptr, nullptr, 0, e->d_info_slices);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(e->h_info, e->d_counters, 2 * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, st));
@@ -1078,7 +1086,8 @@ extern "C" sre_status sre_update_positions(sre_engine *e, int *need_rebuild) {
   e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
   if (need_rebuild) {
     launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
-                        e->d_skin_part, 0, e->d_flags, e->stream);
+                        e->d_skin_part, 0, e->d_flags, e->stream, nullptr, nullptr, 0,
+                        e->d_info_slices);
     unsigned bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
                            hipMemcpyDeviceToHost, e->stream));
@@ -1527,8 +1536,8 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "force_unroll must be 1, 2 or 4");
   if (t->persistent < 0 || t->persistent > 16)
     return fail(SRE_ERR_INVALID, "persistent must be in [0, 16]");
-  if (t->sort_mode != 0 && t->sort_mode != 1)
-    return fail(SRE_ERR_INVALID, "sort_mode must be 0 or 1");
+  if (t->sort_mode < 0 || t->sort_mode > 2)
+    return fail(SRE_ERR_INVALID, "sort_mode must be 0, 1 or 2");
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
@@ -2021,7 +2030,8 @@ extern "C" sre_status sre_check_skin_flag(sre_engine *e) {
   HIP_TRY(hipSetDevice(e->device));
   if (!e->d_skin_part) return fail(SRE_ERR_INVALID, "system not set");
   launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
-                      e->d_skin_part, 0, e->d_flags, e->stream);
+                      e->d_skin_part, 0, e->d_flags, e->stream, nullptr, nullptr, 0,
+                      e->d_info_slices);
   unsigned bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, e->d_flags, sizeof(unsigned),
                          hipMemcpyDeviceToHost, e->stream));
@@ -2261,7 +2271,8 @@ extern "C" sre_status sre_set_md_state(sre_engine *e, const sre_md_state *in) {
 // missed pairs are the likelier cause of a range error that follows it.
 static sre_status md_skin_ok(sre_engine *e, bool *ok, float *disp2 = nullptr) {
   launch_rebuild_info(0, nullptr, nullptr, skin_part_count(e->n_atoms),
-                      e->d_skin_part, 0, e->d_flags, e->stream);
+                      e->d_skin_part, 0, e->d_flags, e->stream, nullptr, nullptr, 0,
+                      e->d_info_slices);
   unsigned bits[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(bits, e->d_flags, 2 * sizeof(unsigned),
                          hipMemcpyDeviceToHost, e->stream));

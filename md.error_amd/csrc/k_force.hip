@@ -280,11 +280,73 @@ __global__ __launch_bounds__(1024) void k_rebuild_info(
   }
 }
 
+// First stage for many partials: block b reduces slice b of the build and
+// skin partials (clearing the skin partials when asked) into part2[b] and
+// skin2[b], which k_rebuild_info then reduces in one block.  A slice total is
+// at most per_b waves x 64 atoms x the list capacity, below 2^32.
+__global__ __launch_bounds__(1024) void k_rebuild_slices(
+    int n_build, const int2 *__restrict__ build_part, int n_skin,
+    float *__restrict__ skin_part, int clear_skin, int per_b, int per_s,
+    int2 *part2, float *skin2) {
+  __shared__ unsigned s_tot[1024 / kWave];
+  __shared__ int s_max[1024 / kWave];
+  __shared__ float s_d2[1024 / kWave];
+  const int b = blockIdx.x;
+  int mx = 0;
+  unsigned tot = 0;
+  float d2 = 0.f;
+  const int b1 = min(n_build, (b + 1) * per_b), s1 = min(n_skin, (b + 1) * per_s);
+  for (int k = b * per_b + (int)threadIdx.x; k < b1; k += blockDim.x) {
+    const int2 v = build_part[k];
+    mx = max(mx, v.x);
+    tot += (unsigned)v.y;
+  }
+  for (int k = b * per_s + (int)threadIdx.x; k < s1; k += blockDim.x) {
+    d2 = fmaxf(d2, skin_part[k]);
+    if (clear_skin) skin_part[k] = 0.f;
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mx = max(mx, __shfl_xor(mx, o, kWave));
+    tot += __shfl_xor(tot, o, kWave);
+    d2 = fmaxf(d2, __shfl_xor(d2, o, kWave));
+  }
+  const int wv = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_max[wv] = mx;
+    s_tot[wv] = tot;
+    s_d2[wv] = d2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / kWave); ++k) {
+      mx = max(mx, s_max[k]);
+      tot += s_tot[k];
+      d2 = fmaxf(d2, s_d2[k]);
+    }
+    part2[b] = make_int2(mx, (int)tot);
+    skin2[b] = d2;
+  }
+}
+
 void launch_rebuild_info(int n_build, const int2 *build_part,
                          unsigned long long *counters, int n_skin,
                          float *skin_part, int clear_skin, unsigned *skin_bits,
                          hipStream_t st, unsigned long long *log,
-                         unsigned *log_idx, int log_cap) {
+                         unsigned *log_idx, int log_cap, int2 *slices) {
+  const int most = n_build > n_skin ? n_build : n_skin;
+  if (slices && most > kInfoSliceLen) {
+    int g = (most + kInfoSliceLen - 1) / kInfoSliceLen;
+    g = g < kInfoSlices ? g : kInfoSlices;
+    const int per_b = (n_build + g - 1) / g, per_s = (n_skin + g - 1) / g;
+    float *skin2 = reinterpret_cast<float *>(slices + kInfoSlices);
+    hipLaunchKernelGGL(k_rebuild_slices, dim3(g), dim3(1024), 0, st, n_build, build_part,
+                       n_skin, skin_part, clear_skin, per_b, per_s, slices, skin2);
+    hipLaunchKernelGGL(k_rebuild_info, dim3(1), dim3(1024), 0, st, n_build > 0 ? g : 0,
+                       slices, counters, n_skin > 0 ? g : 0, skin2, 0, skin_bits, log,
+                       log_idx, log_cap);
+    return;
+  }
   hipLaunchKernelGGL(k_rebuild_info, dim3(1), dim3(1024), 0, st, n_build,
                      build_part, counters, n_skin, skin_part, clear_skin,
                      skin_bits, log, log_idx, log_cap);

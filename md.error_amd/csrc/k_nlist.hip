@@ -25,8 +25,8 @@
 
 namespace sre {
 
-constexpr int kBuildCols = 8;     // columns per lane per chunk
-constexpr int kBuildUnroll = 4;   // candidates per lane per round
+constexpr int kBuildCols = 4;     // columns per lane per chunk
+constexpr int kBuildUnroll = 6;   // candidates per lane per round
 
 __device__ __forceinline__ int wave_max_i(int v) {
 #pragma unroll
@@ -167,65 +167,67 @@ __global__ __launch_bounds__(256) void k_build_list(
       send[slot0 + r] = run;
     }
     __builtin_amdgcn_wave_barrier();
-    // stream the flat sequence: lane l takes v = l, l + TB, ...
+    // stream the flat sequence: in round k0 lane l takes the U consecutive
+    // candidates v = (k0 TB + l) U + u
     const int rounds = __builtin_amdgcn_readfirstlane(wave_max_i((T + TB * U - 1) / (TB * U)));
-    // cursor over the runs: run cur covers flat indices [beg, end) and
-    // starts at sorted atom p0c (kept in registers; LDS is read only when
-    // the cursor moves to the next run)
-    int cur = 0, beg = 0, end = send[0], p0c = sp0[0];
+    // cursor over the runs: run cur covers flat indices [send[cur-1],
+    // end) and candidate v of it is sorted atom off + v (kept in registers;
+    // LDS is read only when the cursor moves to the next run)
+    int cur = 0, end = send[0], off = sp0[0];
     // candidates of round k0 and their gathers (a lane past the sequence
-    // takes j = -1 and gathers its own atom)
+    // takes its own atom si, which the self test below rejects)
     auto load = [&](int k0, int *jj, uint4 *cj) {
+      const int v0 = (k0 * TB + l) * U;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int v = l + TB * (k0 * U + u);
-        int j = -1;
+        const int v = v0 + u;
+        int j = si;
         if (v < T) {
           while (v >= end) {
             ++cur;
-            beg = end;
+            const int b = end;
             end = send[cur];
-            p0c = sp0[cur];
+            off = sp0[cur] - b;
           }
-          j = p0c + (v - beg);
+          j = off + v;
         }
         jj[u] = j;
-        cj[u] = gather_pos_idx<false>(prs, j >= 0 ? j : si);
+        cj[u] = gather_pos_idx<false>(prs, j);
       }
       asm volatile("" ::: "memory");
     };
     // the accepted candidates of a round, placed in flat-sequence order
-    // (v = l + TB (k0 U + u): for each u the group's lanes in turn), so that
-    // consecutive list entries are consecutive candidates -- neighbouring
-    // atoms of one cell column -- and the force kernel's lanes of one atom
-    // gather from the same few cache lines
+    // (lane by lane, each lane's U candidates in turn: an exclusive prefix
+    // of the lanes' accepted counts over the group), so that consecutive
+    // list entries are consecutive candidates -- neighbouring atoms of one
+    // cell column -- and the force kernel's lanes of one atom gather from
+    // the same few cache lines
     auto accept = [&](const int *jj, const uint4 *cj) {
-      const int gl = lane & ~(TB - 1);             // first lane of the group
-      const unsigned lt_mask = (1u << l) - 1u;
-      int base = count;
+      bool ok[U];
+      int c = 0;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int j = jj[u];
         const float dx = disp(cj[u].x, pi.x, qx);
         const float dy = disp(cj[u].y, pi.y, qy);
         const float dz2 = disp(cj[u].z, pi.z, qz);
-        const bool ok = (j >= 0) & (j != s) & (dist2(dx, dy, dz2) < rl2);
-        const unsigned long long wb = __ballot(ok);
-        constexpr unsigned gmask = TB >= 32 ? 0xffffffffu : (1u << (TB & 31)) - 1u;
-        const unsigned grp = TB == 64 ? 0u : (unsigned)(wb >> gl) & gmask;
-        if (TB == 64) {
-          // the whole wave is one group: lanes below this one from the mask
-          const unsigned long long below = l == 0 ? 0ull : (wb & (~0ull >> (64 - l)));
-          const int k = base + __popcll(below);
-          if (ok && k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = j;
-          base += __popcll(wb);
-        } else {
-          const int k = base + __popc(grp & lt_mask);
-          if (ok && k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = j;
-          base += __popc(grp);
-        }
+        // (an inactive group, s >= n, has no candidates: all its lanes
+        // hold j = si)
+        ok[u] = (jj[u] != si) & (dist2(dx, dy, dz2) < rl2);
+        c += ok[u] ? 1 : 0;
       }
-      count = base;
+      int incl = c;
+#pragma unroll
+      for (int o = 1; o < TB; o <<= 1) {
+        const int t = __shfl_up(incl, o, TB);
+        if (l >= o) incl += t;
+      }
+      int k = count + incl - c;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (ok[u] && k < cap) lrow[(unsigned)(((k >> lt) << 6) + (k & (TF - 1)))] = jj[u];
+        k += ok[u] ? 1 : 0;
+      }
+      count += __shfl(incl, TB - 1, TB);
     };
     // software pipeline: the gathers of round k0 + 1 are in flight while
     // round k0 is tested; whole pairs of rounds in the loop (no branch in its

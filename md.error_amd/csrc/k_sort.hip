@@ -427,10 +427,11 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
 // few atoms (its volume is a fraction of (rc + skin)^3 and the atoms
 // repel), so the rank loop of (4) is short.
 
-// cells scanned by one 1024-thread block; its LDS holds the offsets with one
-// pad word per 16 (thread t writes words 17 t + e, so the 64 lanes of a wave
-// spread over the banks), 16320 words of the 16384 a block may hold
-constexpr int kPlaceCells = 15360;
+// kPlaceCells: cells scanned by one 1024-thread block; its LDS holds the
+// offsets with one pad word per 16 (thread t writes words 17 t + e, so the
+// 64 lanes of a wave spread over the banks), 16320 words of the 16384 a
+// block may hold
+static_assert(kPlaceCells + kPlaceCells / 16 <= 16384, "placement offsets exceed 64 KB");
 __device__ __forceinline__ int place_idx(int c) { return c + (c >> 4); }
 
 __global__ __launch_bounds__(256) void k_cell_count(int n, int n_cells, const float *pos,

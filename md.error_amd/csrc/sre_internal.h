@@ -320,6 +320,8 @@ size_t radix_counts_capacity(int n, int key_bits);
 // before the call and again after it; uses b.tkey, b.key_b, b.val_b as
 // scratch.
 size_t cell_count_capacity(int n_cells);
+// up to this many cells the scan of the counts is fused into the placement
+constexpr int kPlaceCells = 15360;
 void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
                              const int *types, const int *rep_of,
                              const RepInfo *reps, const float *rc_table,
@@ -369,15 +371,21 @@ void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
 // counters[0] = longest list, counters[1] = total list length over the
 // n_build partials (skipped when n_build == 0); *skin_bits = the largest
 // squared displacement (float bits) over the n_skin partials, which are
-// zeroed after reading when clear_skin (skipped when n_skin == 0).  One block,
-// fixed order: no atomics.  With a log (device-mapped host memory of log_cap
+// zeroed after reading when clear_skin (skipped when n_skin == 0).  Fixed
+// order: no atomics.  With a log (device-mapped host memory of log_cap
 // entries of three words), the three values (longest, total, displacement^2
 // bits) also go to entry (*log_idx % log_cap), and *log_idx is advanced.
+// With slices (2 kInfoSlices int2 of scratch) and more than kInfoSliceLen
+// partials of either kind, slices of kInfoSliceLen are first reduced by one
+// block each (same results: max and sum are exact in any order).
+constexpr int kInfoSliceLen = 16384;
+constexpr int kInfoSlices = 1024;     // enough for 2^24 partials
 void launch_rebuild_info(int n_build, const int2 *build_part,
                          unsigned long long *counters, int n_skin,
                          float *skin_part, int clear_skin, unsigned *skin_bits,
                          hipStream_t st, unsigned long long *log = nullptr,
-                         unsigned *log_idx = nullptr, int log_cap = 0);
+                         unsigned *log_idx = nullptr, int log_cap = 0,
+                         int2 *slices = nullptr);
 void device_exscan_u32(size_t m, const uint32_t *in, uint32_t *out,
                        uint32_t *tmp, hipStream_t st);
 // Box geometry the position refresh needs; passed by value when all atoms
