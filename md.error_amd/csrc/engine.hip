@@ -206,6 +206,14 @@ struct sre_engine {
   // rebuild graph: the device part of a rebuild, cached by its arguments
   hipGraphExec_t rebuild_exec = nullptr;
   std::vector<unsigned char> rebuild_sig;
+  // segment graphs: m x [steps graph steps + rebuild] as one graph for
+  // m = 1, 2, 4, ..., kSegGraphs / 2 (slot log2 m), all captured together for
+  // one (flags, steps) and the step and rebuild arguments
+  static constexpr int kSegGraphs = 5;
+  hipGraphExec_t seg_exec[kSegGraphs] = {};
+  unsigned seg_flags = 0;
+  int seg_steps = 0, seg_epoch = -1;
+  std::vector<unsigned char> seg_sig;
 
   // MD integrator (caller order, one replica)
   std::vector<float> mass;         // per type (default 1)
@@ -232,6 +240,8 @@ sre_engine::~sre_engine() {
   for (StepGraph &g : step_graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (rebuild_exec) (void)hipGraphExecDestroy(rebuild_exec);
+  for (hipGraphExec_t &g : seg_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   dfree(d_reps); dfree(d_rep_of); dfree(d_types); dfree(d_params);
   dfree(d_paramsd); dfree(d_rc); dfree(d_rho); dfree(d_tail);
   dfree(d_tail_entry_rep); dfree(d_tail_atom); dfree(d_pos_own);
@@ -1836,6 +1846,31 @@ static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
 // Captures K steps of [refresh (+ perturbation) + force] into one graph, or
 // returns the cached one; the graph reads the engine's buffers by address, so
 // every reallocation or layout change bumps graph_epoch and invalidates it.
+// The kernels of K <= kMaxGraphSteps steps (the contents of a step graph),
+// queued on the engine stream.
+static sre_status enqueue_steps(sre_engine *e, unsigned flags, int K) {
+  const bool pert = (flags & SRE_PERTURB) != 0;
+  PerturbArgs pa{e->pert_sigma, e->pert_seed,
+                 e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
+  const bool md = (flags & kGraphMD) != 0;
+  if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s))
+    return enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa);
+  sre_status s = SRE_OK;
+  for (int k = 0; k < K && s == SRE_OK; ++k) {
+    if (md) {
+      s = enqueue_md_step(e, k == K - 1 ? flags : flags & ~SRE_STATS);
+      continue;
+    }
+    launch_update_positions(e->n_atoms, e->d_inv_perm, const_cast<float *>(e->d_pos),
+                            e->pos_stride, e->d_reps, rep0(e), e->d_pos4, e->d_backup,
+                            e->d_res4, e->d_skin_part, pert ? &pa : nullptr, nullptr,
+                            e->stream);
+    s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
+                       pert ? e->d_step_ctr : nullptr);
+  }
+  return s;
+}
+
 static sre_status step_graph(sre_engine *e, unsigned flags, int K,
                              hipGraphExec_t *out) {
   sre_engine::StepGraph *slot = nullptr, *lru = &e->step_graphs[0];
@@ -1855,27 +1890,7 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
       return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    PerturbArgs pa{e->pert_sigma, e->pert_seed,
-                   e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
-    const bool md = (flags & kGraphMD) != 0;
-    sre_status s = SRE_OK;
-    if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s)) {
-      s = enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa);
-    } else {
-      for (int k = 0; k < K && s == SRE_OK; ++k) {
-        if (md) {
-          s = enqueue_md_step(e, k == K - 1 ? flags : flags & ~SRE_STATS);
-          continue;
-        }
-        launch_update_positions(e->n_atoms, e->d_inv_perm,
-                                const_cast<float *>(e->d_pos), e->pos_stride,
-                                e->d_reps, rep0(e), e->d_pos4, e->d_backup, e->d_res4,
-                                e->d_skin_part, pert ? &pa : nullptr, nullptr,
-                                e->stream);
-        s = launch_compute(e, flags & (SRE_STATS | SRE_TAIL | SRE_FCORR),
-                           pert ? e->d_step_ctr : nullptr);
-      }
-    }
+    sre_status s = enqueue_steps(e, flags, K);
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (s != SRE_OK) return s;
     if (r != hipSuccess)
@@ -1895,17 +1910,28 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
   return SRE_OK;
 }
 
-extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) {
-  if (!e || n_steps < 0) return fail(SRE_ERR_INVALID, "bad arguments");
-  if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
-  HIP_TRY(hipSetDevice(e->device));
+// Checks before queued steps; uploads changed tail values.
+static sre_status steps_ready(sre_engine *e, unsigned flags) {
   refresh_tail_values(e);
+  if ((flags & SRE_PERTURB) && !e->d_step_ctr)
+    return fail(SRE_ERR_INVALID, "SRE_PERTURB needs sre_set_perturbation first");
   if ((flags & SRE_PERTURB) && e->pert_anchored &&
       (e->anchor_stride != e->pos_stride ||
        e->anchor_count != (size_t)e->n_atoms * e->pos_stride))
     return fail(SRE_ERR_INVALID,
                 "perturbation anchor does not match the bound positions "
                 "(call sre_set_perturbation again)");
+  return SRE_OK;
+}
+
+extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) {
+  if (!e || n_steps < 0) return fail(SRE_ERR_INVALID, "bad arguments");
+  if (!e->built) return fail(SRE_ERR_NOT_READY, "graph before the first rebuild");
+  HIP_TRY(hipSetDevice(e->device));
+  {
+    sre_status s = steps_ready(e, flags);
+    if (s != SRE_OK) return s;
+  }
   // whole graphs of kMaxGraphSteps steps, then one graph for the remainder
   int left = n_steps;
   while (left > 0) {
@@ -1930,13 +1956,69 @@ extern "C" sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps) 
 // rebuild logs its longest list, total length and the largest displacement
 // of the segment before it (the rebuild log, host memory), checked once the
 // queued segments have run.
-static sre_status run_segment_queue(sre_engine *e, unsigned flags, int g0, int n_seg,
-                                    int steps, int64_t *n_list, float *d2_out) {
-  std::vector<const uint64_t *> info(n_seg);
-  for (int g = 0; g < n_seg; ++g) {
-    sre_status s = sre_graph_run(e, flags, steps);
+// Segments of `steps` steps, each followed by the device part of a
+// rebuild, captured m at a time as one graph: between graph launches the GPU
+// idles ~8 us (measured at 64k atoms), between the kernels of one graph it
+// does not.  The graphs for every m = 2^k (k < kSegGraphs) are captured
+// together, so that any later count of segments is replayed from them
+// (largest first) without a capture.
+static sre_status segment_graphs(sre_engine *e, unsigned flags, int steps, int n_parts) {
+  if (e->seg_exec[0] && e->seg_flags == flags && e->seg_steps == steps &&
+      e->seg_epoch == e->graph_epoch && e->seg_sig == e->rebuild_sig)
+    return SRE_OK;
+  for (hipGraphExec_t &g : e->seg_exec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  for (int k = 0; k < sre_engine::kSegGraphs; ++k) {
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    sre_status s = SRE_OK;
+    for (int seg = 0; seg < (1 << k) && s == SRE_OK; ++seg) {
+      for (int left = steps; left > 0 && s == SRE_OK; left -= kMaxGraphSteps)
+        s = enqueue_steps(e, flags, std::min(left, kMaxGraphSteps));
+      if (s == SRE_OK) enqueue_rebuild(e, n_parts, e->stream);
+    }
+    hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (s != SRE_OK) return s;
-    HIP_TRY(launch_rebuild(e, &info[g]));
+    if (r != hipSuccess)
+      return fail(SRE_ERR_HIP, "segment graph capture: %s", hipGetErrorString(r));
+    r = hipGraphInstantiate(&e->seg_exec[k], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (r != hipSuccess) {
+      e->seg_exec[k] = nullptr;
+      return fail(SRE_ERR_HIP, "segment graph instantiate: %s", hipGetErrorString(r));
+    }
+  }
+  e->seg_flags = flags;
+  e->seg_steps = steps;
+  e->seg_epoch = e->graph_epoch;
+  e->seg_sig = e->rebuild_sig;
+  return SRE_OK;
+}
+
+static sre_status run_segment_queue(sre_engine *e, unsigned flags, int g0, int n_seg,
+                                    int steps, int n_parts, int64_t *n_list,
+                                    float *d2_out) {
+  std::vector<const uint64_t *> info(n_seg);
+  {
+    sre_status s = steps_ready(e, flags);
+    if (s != SRE_OK) return s;
+  }
+  {
+    sre_status s = segment_graphs(e, flags, steps, n_parts);
+    if (s != SRE_OK) return s;
+  }
+  for (int g = 0; g < n_seg;) {
+    int k = sre_engine::kSegGraphs - 1;
+    while ((1 << k) > n_seg - g) --k;
+    HIP_TRY(hipGraphLaunch(e->seg_exec[k], e->stream));
+    // each rebuild of the graph writes the next rebuild-log entry
+    for (int q = 0; q < (1 << k); ++q, ++g) {
+      info[g] = e->h_log + 3 * (size_t)(e->log_launches % kRebuildLog);
+      ++e->log_launches;
+    }
   }
   HIP_TRY(hipStreamSynchronize(e->stream));
   e->built = true;
@@ -2008,7 +2090,7 @@ extern "C" sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segm
   for (int g0 = 0; g0 < n_segments; g0 += kRebuildLog) {
     const int m = std::min(kRebuildLog, n_segments - g0);
     float d2 = 0.f;
-    sre_status s = run_segment_queue(e, flags, g0, m, steps,
+    sre_status s = run_segment_queue(e, flags, g0, m, steps, n_parts,
                                      n_list ? n_list + g0 : nullptr, &d2);
     if (s != SRE_OK) return s;
     d2max = std::max(d2max, d2);

@@ -343,10 +343,12 @@ def test_overlapping_atoms_raise_force_range(torch_cuda):
         e.stats()
 
 
-def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab):
-    """sre_run_segments (segments and rebuilds queued, checks at the end) does
-    what graph_run + rebuild per segment does: the same positions, lists and
-    forces bit for bit, the same list lengths and largest displacement."""
+@pytest.mark.parametrize("n_seg,steps", [(3, 5), (19, 2)])
+def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps):
+    """sre_run_segments (segments and rebuilds queued as graphs of up to 16
+    segments, checks at the end) does what graph_run + rebuild per segment
+    does: the same positions, lists and forces bit for bit, the same list
+    lengths and largest displacement."""
     import torch
     res = []
     for queued in (False, True):
@@ -358,11 +360,11 @@ def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab):
         e.rebuild()
         e.set_perturbation(0.01, 21, 0, anchored=True)
         if queued:
-            nls, disp = e.run_segments(3, 5, stats=True, tail=True, perturb=True)
+            nls, disp = e.run_segments(n_seg, steps, stats=True, tail=True, perturb=True)
         else:
             nls, disp = [], 0.0
-            for _ in range(3):
-                e.graph_run(5, stats=True, tail=True, perturb=True)
+            for _ in range(n_seg):
+                e.graph_run(steps, stats=True, tail=True, perturb=True)
                 e.rebuild()
                 nls.append(e.neighbor_count(in_rc=False)[0])
                 disp = max(disp, e.last_rebuild_displacement())
