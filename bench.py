@@ -11,8 +11,9 @@ the per-atom tail correction.  The neighbor list is rebuilt every
 the displacement recorded on the device is checked at every rebuild, so a run
 in which any atom moved more than skin/2 between rebuilds fails instead of
 reporting.  The energy and virial are reduced and read back every 100 steps,
-like the reference driver's thermo output.  Steps between rebuilds are
-replayed as one hipGraph.
+like the reference driver's thermo output.  The segments between two
+read-backs, each of its steps and its closing rebuild, are replayed as
+hipGraphs of several segments each.
 
 Prints one JSON line:  value = pair evaluations per second (list entries
 processed by the force kernel per second of whole-job wall time, rebuilds
@@ -79,9 +80,12 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
         and their rebuilds are queued with one synchronisation
         (sre_run_segments, which checks every rebuild's list capacity and
         displacement at the end).  A block at a read-back ends with the
-        read-back and a rebuild; a run that ends between two read-backs ends
-        with the queued segments' own check (or the skin-flag check of a
-        trailing partial segment)."""
+        read-back of its last step's energy and virial, after the queued
+        segments (their closing rebuild leaves those values), or, when the
+        block does not end on a segment, after its partial segment and
+        before a rebuild; a run that ends between two read-backs ends with
+        the queued segments' own check (or the skin-flag check of a trailing
+        partial segment)."""
         done = 0
         pairs = 0
         rebuilds = 0
@@ -91,8 +95,6 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
             block = min(to_thermo, n_steps - done)
             thermo = block == to_thermo
             nfull, rem = divmod(block, every)
-            if thermo and rem == 0:     # the read-back and rebuild end the last segment
-                nfull, rem = nfull - 1, every
             if nfull > 0:
                 nls, disp = e.run_segments(nfull, every, stats=True, tail=True,
                                            perturb=True)
@@ -101,6 +103,11 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
                     pairs += nl * every
                     nl = int(nls[k])
                 rebuilds += nfull
+                if thermo and rem == 0:
+                    # the energy and virial of the block's last step (the
+                    # segment's closing rebuild leaves them); raises on a
+                    # force overflow
+                    e.stats()
             if rem > 0:
                 e.graph_run(rem, stats=True, tail=True, perturb=True)
                 pairs += nl * rem

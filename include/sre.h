@@ -272,7 +272,10 @@ sre_status sre_graph_run(sre_engine *e, unsigned flags, int n_steps);
  * regrown and the lists rebuilt, and SRE_ERR_LIST_OVERFLOW reports those
  * segments as invalid.  If an atom moved more than skin/2 within a segment,
  * SRE_ERR_SKIN names the first such segment (its forces may have missed
- * pairs); the outputs are filled in either case and the lists are current. */
+ * pairs); the outputs are filled in either case and the lists are current.
+ * With SRE_STATS, sre_get_stats afterwards returns the energy and virial of
+ * the last step (the final rebuild does not change them); forces and
+ * per-atom statistics need a new compute. */
 sre_status sre_run_segments(sre_engine *e, unsigned flags, int n_segments, int steps,
                             int64_t *n_list, float *max_disp);
 /* Perturbation of the bound positions applied by SRE_PERTURB replays, one

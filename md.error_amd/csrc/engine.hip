@@ -2023,7 +2023,12 @@ static sre_status run_segment_queue(sre_engine *e, unsigned flags, int g0, int n
   HIP_TRY(hipStreamSynchronize(e->stream));
   e->built = true;
   e->md_since_rebuild = 0;
-  e->stats_valid = e->forces_valid = e->atom_stats_valid = false;
+  // the per-atom statistics of the last step are still those of the atoms'
+  // slots before the final rebuild, and every replica keeps its slot range,
+  // so their totals stay readable; the forces and per-atom values are in
+  // the old order and are not
+  e->stats_valid = e->stats_pending = (flags & SRE_STATS) != 0;
+  e->forces_valid = e->atom_stats_valid = false;
   float d2max = 0.f;
   int worst = -1;
   std::vector<float> d2(n_seg);

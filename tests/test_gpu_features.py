@@ -361,18 +361,21 @@ def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps):
         e.set_perturbation(0.01, 21, 0, anchored=True)
         if queued:
             nls, disp = e.run_segments(n_seg, steps, stats=True, tail=True, perturb=True)
+            st = e.stats().copy()     # the last step's, read after the final rebuild
         else:
             nls, disp = [], 0.0
             for _ in range(n_seg):
                 e.graph_run(steps, stats=True, tail=True, perturb=True)
+                st = e.stats().copy()
                 e.rebuild()
                 nls.append(e.neighbor_count(in_rc=False)[0])
                 disp = max(disp, e.last_rebuild_displacement())
         e.graph_run(5, stats=True, tail=True, perturb=True)
         res.append((list(nls), disp, e.forces_fixed(), e.export_pairs(),
-                    pos.cpu().numpy()))
+                    pos.cpu().numpy(), st))
         e.close()
-    (n0, d0, f0, p0, x0), (n1, d1, f1, p1, x1) = res
+    (n0, d0, f0, p0, x0, s0), (n1, d1, f1, p1, x1, s1) = res
     assert n0 == n1 and d0 == d1 and 0.0 < d1 <= 0.5 * slab.skin
+    np.testing.assert_array_equal(s0, s1)
     assert np.array_equal(x0, x1) and np.array_equal(f0, f1)
     assert np.array_equal(p0[0], p1[0]) and np.array_equal(p0[1], p1[1])
