@@ -421,16 +421,16 @@ __global__ __launch_bounds__(256) void k_gather_sorted(
 //       and cursor of its cell back to zero for the next rebuild (every cell
 //       with a non-zero entry holds an atom).
 // Up to kPlaceCells cells, (2) and (3) are one launch (k_cell_place): every
-// block of 1024 atoms scans all the counts into LDS itself (16 per thread,
-// up to 60 KB read from L2 per block) instead of waiting for a separate scan
-// launch; larger grids use k_cell_scan and k_cell_scatter.  A cell holds a
+// block of 1024 atoms scans all the counts in LDS itself (up to 60 KB read
+// from L2 per block) instead of waiting for a separate scan launch; larger
+// grids use k_cell_scan and k_cell_scatter.  A cell holds a
 // few atoms (its volume is a fraction of (rc + skin)^3 and the atoms
 // repel), so the rank loop of (4) is short.
 
 // kPlaceCells: cells scanned by one 1024-thread block; its LDS holds the
-// offsets with one pad word per 16 (thread t writes words 17 t + e, so the
-// 64 lanes of a wave spread over the banks), 16320 words of the 16384 a
-// block may hold
+// offsets with one pad word per 16 (thread t reads and writes words 17 t + e,
+// so the 64 lanes of a wave spread over the banks), 16320 words of the 16384
+// a block may hold
 static_assert(kPlaceCells + kPlaceCells / 16 <= 16384, "placement offsets exceed 64 KB");
 __device__ __forceinline__ int place_idx(int c) { return c + (c >> 4); }
 
@@ -505,21 +505,38 @@ __global__ __launch_bounds__(256) void k_cell_scatter(int n, int n_cells,
   }
 }
 
-// (2) + (3) for m <= kPlaceCells: block b scans all counts into LDS, block 0
-// also writes them to start[0..m], then each thread places one atom.
+// (2) + (3) for m <= kPlaceCells: block b scans all counts in LDS (loaded
+// and, by block 0, written to start[0..m] with coalesced accesses; each
+// thread scans 16 consecutive counts in between), then each thread places
+// one atom.
 __global__ __launch_bounds__(kSortT) void k_cell_place(int n, int m, const uint32_t *keys,
                                                        const uint32_t *count,
                                                        uint32_t *cursor, int *start,
                                                        uint32_t *skey, uint32_t *sval) {
+  constexpr int E = 16;
   __shared__ int s_start[kPlaceCells + kPlaceCells / 16];
   __shared__ uint32_t tmp[kSortT / 64 + 1];
-  const bool first = blockIdx.x == 0;
-  scan_tile16(m, 0, count, 0u, tmp, [&](int c, uint32_t v) {
-    s_start[place_idx(c)] = (int)v;
-    if (first) start[c] = (int)v;
-  });
-  if (first && threadIdx.x == 0) start[m] = n;
+  for (int c = threadIdx.x; c < m; c += kSortT) s_start[place_idx(c)] = (int)count[c];
   __syncthreads();
+  const int i0 = (int)threadIdx.x * E;
+  uint32_t v[E];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    v[e] = i0 + e < m ? (uint32_t)s_start[place_idx(i0 + e)] : 0u;
+    sum += v[e];
+  }
+  uint32_t ex = block_exscan_1024(sum, tmp, nullptr);   // ends with a barrier
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (i0 + e < m) s_start[place_idx(i0 + e)] = (int)ex;
+    ex += v[e];
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int c = threadIdx.x; c < m; c += kSortT) start[c] = s_start[place_idx(c)];
+    if (threadIdx.x == 0) start[m] = n;
+  }
   const int i = blockIdx.x * kSortT + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = keys[i];     // < m (clamped when counted)
