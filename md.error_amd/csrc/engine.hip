@@ -1044,6 +1044,7 @@ static sre_status do_rebuild(sre_engine *e) {
       e->rebuild_exec = nullptr;
       return fail(SRE_ERR_HIP, "rebuild instantiate: %s", hipGetErrorString(r));
     }
+    HIP_TRY(hipGraphUpload(e->rebuild_exec, st));
     e->rebuild_sig = std::move(sig);
   }
   const uint64_t *info = nullptr;
@@ -1901,6 +1902,7 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
       slot->exec = nullptr;
       return fail(SRE_ERR_HIP, "graph instantiate: %s", hipGetErrorString(r));
     }
+    HIP_TRY(hipGraphUpload(slot->exec, e->stream));
     slot->flags = flags;
     slot->steps = K;
     slot->epoch = e->graph_epoch;
@@ -1990,6 +1992,8 @@ static sre_status segment_graphs(sre_engine *e, unsigned flags, int steps, int n
       e->seg_exec[k] = nullptr;
       return fail(SRE_ERR_HIP, "segment graph instantiate: %s", hipGetErrorString(r));
     }
+    // upload now: the first launch of a graph not uploaded carries the upload
+    HIP_TRY(hipGraphUpload(e->seg_exec[k], e->stream));
   }
   e->seg_flags = flags;
   e->seg_steps = steps;
