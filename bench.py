@@ -159,6 +159,17 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
         e.time_compute(20, stats=True, tail=True)
         k_piped_ms = e.time_compute(200, stats=True, tail=True)
         e.set_tuning(**plain)
+    k_cells_ms = None
+    if precision == "mixed":
+        # the cell-window form without a list (LDS-staged candidates, bitwise
+        # the same forces): needs a rebuild for its tile map, then one back
+        plain = e.tuning()
+        e.set_tuning(force_mode=1)
+        e.rebuild()
+        e.time_compute(5, stats=True, tail=True)
+        k_cells_ms = e.time_compute(20, stats=True, tail=True)
+        e.set_tuning(**plain)
+        e.rebuild()
     dist = force_time_distribution(e, torch, stream)
     rb_ms = e.time_rebuild(10)
     e.compute(stats=True, tail=True)
@@ -168,7 +179,7 @@ def run_form(w, precision, steps, warmup, every, torch, Engine, autotune=False):
     return dict(engine=e, pos=pos, wall_s=wall, gpu_ms=gpu_ms, pairs=pairs, warm=warm,
                 rebuilds=rebuilds, force_ms=k_ms, n_list=nl, n_in_rc=nin, max_list=mx,
                 force_us_dist=dist, rebuild_ms=rb_ms, autotune_s=tune_s,
-                force_piped_ms=k_piped_ms,
+                force_piped_ms=k_piped_ms, force_cells_ms=k_cells_ms,
                 pair_evals_per_s=pairs / wall, ms_per_step=1e3 * wall / steps)
 
 
@@ -578,6 +589,8 @@ def main():
                    "tuning": mixed["engine"].tuning()},
         "force_kernel_us": round(mixed["force_ms"] * 1e3, 3),
         "force_kernel_us_per_launch": mixed["force_us_dist"],
+        # the same forces (bitwise) from the cell-window kernel without a list
+        "force_kernel_cells_us": round(mixed["force_cells_ms"] * 1e3, 3),
         "rebuild_us": round(mixed["rebuild_ms"] * 1e3, 1),
         "rebuild_us_per_step": round(mixed["rebuild_ms"] * 1e3 / args.rebuild_every, 2),
         "force_pair_evals_per_s": round(L / (mixed["force_ms"] * 1e-3), 1),

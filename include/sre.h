@@ -245,6 +245,10 @@ typedef struct {
                            0 = counting sort by cell up to 15360 cells, radix sort
                            above; 1 = LSD radix sort of the cell key; 2 = counting
                            sort at any cell count */
+  int force_mode;       /* mixed form: 0 = neighbor-list kernel; 1 = cell-window
+                           kernel without a list (one wavefront per 64-atom tile of
+                           a cell column, candidates staged in LDS and read by all
+                           lanes; bitwise the same forces, measured slower) */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */

@@ -53,7 +53,7 @@ struct sre_engine {
   int n_types = 1;
   float skin = 0.3f;
   int precision = SRE_PREC_MIXED;
-  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2, 0, 0};
+  sre_tuning tune{4, 4, 256, 1.0f, 0.35f, 0, 1, 2, 0, 0, 0};
   int n_simd = 1024;               // SIMDs of the device (4 per CU)
 
   // system
@@ -906,6 +906,7 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
   };
   const int ints[] = {e->n_atoms, e->key_bits, e->n_cells, e->n_cols, e->n_rep,
                       e->cap, e->tune.build_mode, e->tune.build_lanes, e->tune.sort_mode,
+                      e->tune.force_mode,
                       e->tune.force_lanes, e->n_types, e->pos_stride, n_parts,
                       (e->tail_bins > 0 && e->tail_mode == SRE_TAIL_EXACT) ? 1 : 0,
                       e->rc_explicit ? 1 : 0};
@@ -931,6 +932,10 @@ static std::vector<unsigned char> rebuild_signature(const sre_engine *e,
 
 static void enqueue_build(sre_engine *e, hipStream_t st) {
   const int N = e->n_atoms;
+  // the cell-window force form reads the tile map of the tile build
+  if (e->tune.force_mode == 1 && e->tune.build_mode != 1)
+    launch_tile_map(e->n_cols, e->n_rep, e->d_reps, e->d_start, e->d_tiles_per_col,
+                    e->d_tile_off, e->d_tile_tmp, st);
   if (e->tune.build_mode == 1)
     launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
                        e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
@@ -1162,8 +1167,16 @@ static sre_status launch_compute(sre_engine *e, unsigned flags,
   ForceArgs a = force_args(e, stats);
   a.ctr_in = step_ctr;
   a.ctr_out = step_ctr;
-  launch_force(e->precision, T, e->tune.force_unroll, e->tune.block, stats, tail,
-               a, e->stream);
+  if (e->tune.force_mode == 1) {
+    if (e->precision != SRE_PREC_MIXED)
+      return fail(SRE_ERR_INVALID, "the cell-window force form is mixed precision only");
+    CellWindow c{e->n_cols, e->n_rep, e->d_tile_off, e->d_tiles_per_col, e->d_start,
+                 e->d_backup_s, e->skin};
+    launch_force_cells(stats, tail, a, c, e->stream);
+  } else {
+    launch_force(e->precision, T, e->tune.force_unroll, e->tune.block, stats, tail,
+                 a, e->stream);
+  }
   if ((flags & SRE_FCORR) && e->fc_cells > 0)
     launch_add_force_grid(e->n_atoms, e->d_pos, e->pos_stride, e->d_backup,
                           e->d_inv_perm, e->d_reps, e->d_fc, e->d_fx, e->d_fy,
@@ -1549,9 +1562,13 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "persistent must be in [0, 16]");
   if (t->sort_mode < 0 || t->sort_mode > 2)
     return fail(SRE_ERR_INVALID, "sort_mode must be 0, 1 or 2");
+  if (t->force_mode != 0 && t->force_mode != 1)
+    return fail(SRE_ERR_INVALID, "force_mode must be 0 or 1");
+  // (the cell-window force form reads a tile map made at the rebuild)
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||
                       t->build_mode != e->tune.build_mode ||
+                      t->force_mode != e->tune.force_mode ||
                       t->cell_xy != e->tune.cell_xy || t->cell_z != e->tune.cell_z;
   if (memcmp(&e->tune, t, sizeof *t) != 0) e->graph_epoch++;  // captured launches
   e->tune = *t;
@@ -1854,7 +1871,8 @@ static sre_status enqueue_steps(sre_engine *e, unsigned flags, int K) {
   PerturbArgs pa{e->pert_sigma, e->pert_seed,
                  e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
   const bool md = (flags & kGraphMD) != 0;
-  if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s))
+  if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s) &&
+      e->tune.force_mode == 0)
     return enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa);
   sre_status s = SRE_OK;
   for (int k = 0; k < K && s == SRE_OK; ++k) {

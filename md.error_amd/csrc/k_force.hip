@@ -609,6 +609,214 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
   for (int t = t0; t < t1; ++t) force_tile<T, U, STATS, TAIL, MULTI, SHIFT>(a, sp, t);
 }
 
+// ------------------------------------------------- cell-window form ----
+//
+// The mixed force evaluation without a neighbor list: one wavefront per
+// tile of up to 64 consecutive sorted atoms of one cell column (the tiles of
+// the tile list build), one atom per lane.  The candidates are the cell runs
+// within rc_max + skin of the tile's rebuild-time cells -- the tile build's
+// window, which holds every pair the list holds -- staged 64 at a time in
+// LDS and read by all lanes at once (one broadcast read per candidate).
+// Each lane masks |d| < rc_i at the current positions with the list
+// kernel's per-pair arithmetic, so the pair set and every pair component are
+// the list kernel's and the fixed-point forces are bitwise the same; the
+// FP32 virial and energy partial sums run in another order.  Rebuild-time
+// cells come from the rebuild-time lattice copy in cell order (backup_s).
+template <bool STATS, bool TAIL, bool MULTI, bool SHIFT>
+__global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c) {
+  if (a.ctr_out && blockIdx.x == 0 && threadIdx.x == 0) *a.ctr_out = *a.ctr_in + 1;
+  __shared__ uint4 cand[kWave];
+  __shared__ PairParam sp[MULTI ? kMaxPairTypes : 1];
+  if (MULTI) {
+    for (int t = threadIdx.x; t < a.n_types * a.n_types; t += blockDim.x)
+      sp[t] = a.params[t];
+    __syncthreads();
+  }
+  const int tile = blockIdx.x;
+  const int n_tiles = (int)(c.tile_off[c.n_cols - 1] + c.tiles_per_col[c.n_cols - 1]);
+  if (tile >= n_tiles) return;   // the grid is an upper bound of the tile count
+  int lo = 0, hi = c.n_cols - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int)c.tile_off[mid] <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const int col_g = lo;
+  int rlo = 0, rhi = c.n_rep - 1;
+  while (rlo < rhi) {
+    const int mid = (rlo + rhi + 1) >> 1;
+    if (a.reps[mid].col_off <= col_g) rlo = mid;
+    else rhi = mid - 1;
+  }
+  const RepInfo &R = a.reps[rlo];
+  const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
+  const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
+  const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  const int col = col_g - R.col_off;
+  const int cx = col / ncy, cy = col - cx * ncy;
+  const int cbase = R.cell_off + col * ncz;
+  const int a0 = c.start[cbase] + (tile - (int)c.tile_off[col_g]) * kWave;
+  const int a1 = min(c.start[cbase + ncz], a0 + kWave);
+  const int lane = threadIdx.x;
+  const int s = a0 + lane;
+  const bool act = s < a1;
+  const int si = act ? s : a0;
+  const uint4 pi = a.pos4[si];
+  const float rc = a.rcut[si];
+  const float rc2 = rc * rc;
+  const int ti = (int)(pi.w & 0xffu) * a.n_types;
+  float rlmax = act ? rc + c.skin : 0.f;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) rlmax = fmaxf(rlmax, __shfl_xor(rlmax, o, kWave));
+  const float rlm = rlmax * 1.0001f + 1e-4f;
+  const float rlm2 = rlm * rlm;
+  const int cz_lo = cell_of(c.backup_s[a0].z, ncz), cz_hi = cell_of(c.backup_s[a1 - 1].z, ncz);
+  const int kx = (int)ceilf(rlm / hx), ky = (int)ceilf(rlm / hy), kz = (int)ceilf(rlm / hz);
+  const bool fullx = 2 * kx + 1 >= ncx, fully = 2 * ky + 1 >= ncy;
+  const int ox0 = fullx ? -cx : -kx, ox1 = fullx ? ncx - 1 - cx : kx;
+  const int oy0 = fully ? -cy : -ky, oy1 = fully ? ncy - 1 - cy : ky;
+  PairParam P = a.params[0];
+  double ax = kAccBase, ay = kAccBase, az = kAccBase;
+  float e = 0.f, wx = 0.f, wy = 0.f, wz = 0.f, fsabs = 0.f;
+  constexpr bool SPLIT_E = STATS && !MULTI && !SHIFT;
+  float e6 = 0.f;
+  round_f64_toward_zero();
+  for (int ox = ox0; ox <= ox1; ++ox) {
+    const float gx = fullx ? 0.f : fmaxf(0.f, (float)(abs(ox) - 1) * hx);
+    if (gx * gx >= rlm2) continue;
+    int cxw = cx + ox;
+    cxw += cxw < 0 ? ncx : 0;
+    cxw -= cxw >= ncx ? ncx : 0;
+    for (int oy = oy0; oy <= oy1; ++oy) {
+      const float gy = fully ? 0.f : fmaxf(0.f, (float)(abs(oy) - 1) * hy);
+      if (gx * gx + gy * gy >= rlm2) continue;
+      int cyw = cy + oy;
+      cyw += cyw < 0 ? ncy : 0;
+      cyw -= cyw >= ncy ? ncy : 0;
+      const int colbase = R.cell_off + (cxw * ncy + cyw) * ncz;
+      int ra0 = 0, rb0 = ncz - 1, ra1 = 0, rb1 = -1;
+      const int zl = cz_lo - kz, zh = cz_hi + kz;
+      if (zh - zl + 1 < ncz) {
+        if (zl < 0) {
+          ra0 = zl + ncz; rb0 = ncz - 1; ra1 = 0; rb1 = zh;
+        } else if (zh >= ncz) {
+          ra0 = zl; rb0 = ncz - 1; ra1 = 0; rb1 = zh - ncz;
+        } else {
+          ra0 = zl; rb0 = zh;
+        }
+      }
+      for (int run = 0; run < 2; ++run) {
+        const int ca = run ? ra1 : ra0, cb = run ? rb1 : rb0;
+        if (cb < ca) continue;
+        // runs lie inside [0, n) by construction; the clamps keep every
+        // load in bounds whatever the offsets hold
+        const int p0 = max(c.start[colbase + ca], 0);
+        const int p1 = min(c.start[colbase + cb + 1], a.n);
+        for (int p = p0; p < p1; p += kWave) {
+          const int q = p + lane;
+          cand[lane] = a.pos4[q < p1 ? q : p0];
+          __syncthreads();
+          const int m = min(kWave, p1 - p);
+          for (int k = 0; k < m; ++k) {
+            const uint4 cj = cand[k];
+            const float dx = disp(cj.x, pi.x, qx);
+            const float dy = disp(cj.y, pi.y, qy);
+            const float dz = disp(cj.z, pi.z, qz);
+            const float d2 = dist2(dx, dy, dz);
+            const bool in = act && (p + k != s) && (d2 < rc2);
+            if (MULTI) P = sp[ti + (int)(cj.w & 0xffu)];
+            const float ri2 = in ? __builtin_amdgcn_rcpf(d2) : 0.f;
+            const float ri6 = ri2 * ri2 * ri2;
+            const float fs = ri2 * ri6 * __builtin_fmaf(-P.A12, ri6, P.B6);
+            fsabs += fabsf(fs);
+            const float fpx = fs * dx, fpy = fs * dy, fpz = fs * dz;
+            acc_floor(ax, fpx);
+            acc_floor(ay, fpy);
+            acc_floor(az, fpz);
+            if (STATS) {
+              if (SPLIT_E) {
+                e6 += ri6;
+              } else {
+                const float t = __builtin_fmaf(P.A, ri6, -P.B);
+                e += SHIFT ? __builtin_fmaf(ri6, t, in ? P.shift : 0.f) : ri6 * t;
+              }
+              wx = __builtin_fmaf(fpx, dx, wx);
+              wy = __builtin_fmaf(fpy, dy, wy);
+              wz = __builtin_fmaf(fpz, dz, wz);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  round_f64_nearest();
+  // the partial sums pass through an (empty) asm after the mode switch, so
+  // the compiler cannot schedule their FP64 conversions in front of it
+  asm volatile("" : "+v"(e), "+v"(e6), "+v"(wx), "+v"(wy), "+v"(wz), "+v"(fsabs));
+  const int64_t acc0 = __double_as_longlong(kAccBase);
+  int64_t Fx = __double_as_longlong(ax) - acc0;
+  int64_t Fy = __double_as_longlong(ay) - acc0;
+  int64_t Fz = __double_as_longlong(az) - acc0;
+  if (__ballot(act && fsabs * rc >= 1125899906842624.0f) && lane == 0)
+    atomicOr(a.err_flag, 1u);
+  if (!act) return;
+  double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
+  if (STATS) {
+    E = SPLIT_E ? (-((double)wx + (double)wy + (double)wz) * (kFixInv / 12.0) -
+                   0.5 * (double)P.B * (double)e6) * 0.5
+                : (double)e * 0.5;
+    Wx = (double)wx * (0.5 * kFixInv);
+    Wy = (double)wy * (0.5 * kFixInv);
+    Wz = (double)wz * (0.5 * kFixInv);
+  }
+  if (TAIL) {
+    double2 tv = make_double2(0.0, 0.0);
+    if (a.tail_atom) {
+      tv = a.tail_atom[s];
+    } else {
+      const int tix = a.tail_idx[s];
+      if (tix >= 0) tv = a.tail[tix];
+    }
+    Fz += to_fix(tv.x);
+    E += tv.y;
+  }
+  a.fx[s] = Fx;
+  a.fy[s] = Fy;
+  a.fz[s] = Fz;
+  if (STATS) {
+    double2 *d = reinterpret_cast<double2 *>(a.atom_stats + (size_t)s * 4);
+    d[0] = make_double2(E, Wx);
+    d[1] = make_double2(Wy, Wz);
+  }
+}
+
+void launch_force_cells(bool stats, bool tail, const ForceArgs &a, const CellWindow &c,
+                        hipStream_t st) {
+  if (a.n <= 0 || c.n_cols <= 0) return;
+  const int max_tiles = (a.n + kWave - 1) / kWave + c.n_cols;
+  const bool multi = a.n_types > 1, shift = a.has_shift != 0;
+#define SRE_CELLS(S, T, M, H)                                                       \
+  hipLaunchKernelGGL((k_force_cells<S, T, M, H>), dim3(max_tiles), dim3(kWave), 0, st, \
+                     a, c)
+  if (stats) {
+    if (tail) {
+      if (multi) { if (shift) SRE_CELLS(true, true, true, true); else SRE_CELLS(true, true, true, false); }
+      else { if (shift) SRE_CELLS(true, true, false, true); else SRE_CELLS(true, true, false, false); }
+    } else {
+      if (multi) { if (shift) SRE_CELLS(true, false, true, true); else SRE_CELLS(true, false, true, false); }
+      else { if (shift) SRE_CELLS(true, false, false, true); else SRE_CELLS(true, false, false, false); }
+    }
+  } else {
+    if (tail) {
+      if (multi) SRE_CELLS(false, true, true, false); else SRE_CELLS(false, true, false, false);
+    } else {
+      if (multi) SRE_CELLS(false, false, true, false); else SRE_CELLS(false, false, false, false);
+    }
+  }
+#undef SRE_CELLS
+}
+
 // --------------------------------------------------------------- FP64 ----
 
 template <int T, bool STATS, bool TAIL, bool MULTI, bool NEXT>

@@ -441,6 +441,15 @@ __global__ __launch_bounds__(kWave) void k_build_tiles(
   if (lane == 0) part[t] = make_int2(mx, tot);
 }
 
+void launch_tile_map(int n_cols, int n_rep, const RepInfo *reps, const int *start,
+                     uint32_t *tiles_per_col, uint32_t *tile_off, uint32_t *scan_tmp,
+                     hipStream_t st) {
+  if (n_cols <= 0) return;
+  hipLaunchKernelGGL(k_col_tiles, dim3((n_cols + 255) / 256), dim3(256), 0, st,
+                     n_cols, n_rep, reps, start, tiles_per_col);
+  device_exscan_u32((size_t)n_cols, tiles_per_col, tile_off, scan_tmp, st);
+}
+
 void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
                         const uint4 *pos4, const float *rcut,
                         const RepInfo *reps, const int *start, float skin,
@@ -448,9 +457,7 @@ void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
                         uint32_t *tiles_per_col, uint32_t *tile_off,
                         uint32_t *scan_tmp, hipStream_t st) {
   if (n <= 0 || n_cols <= 0) return;
-  hipLaunchKernelGGL(k_col_tiles, dim3((n_cols + 255) / 256), dim3(256), 0, st,
-                     n_cols, n_rep, reps, start, tiles_per_col);
-  device_exscan_u32((size_t)n_cols, tiles_per_col, tile_off, scan_tmp, st);
+  launch_tile_map(n_cols, n_rep, reps, start, tiles_per_col, tile_off, scan_tmp, st);
   // every column adds at most one partial tile
   const int max_tiles = (n + kWave - 1) / kWave + n_cols;
   hipLaunchKernelGGL(k_build_tiles, dim3(max_tiles), dim3(kWave), 0, st, n_cols,

@@ -360,6 +360,11 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
                        float skin, int cap, int *list, int *nnb, int2 *part,
                        hipStream_t st);
+// Tiles of <= 64 atoms inside one cell column: tiles_per_col and tile_off
+// (n_cols each) filled on the device from the cell offsets.
+void launch_tile_map(int n_cols, int n_rep, const RepInfo *reps, const int *start,
+                     uint32_t *tiles_per_col, uint32_t *tile_off, uint32_t *scan_tmp,
+                     hipStream_t st);
 // Tile build: tiles of <= 64 atoms inside one cell column.  tiles_per_col and
 // tile_off (n_cols each) are scratch arrays filled on the device.
 void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
@@ -525,6 +530,19 @@ struct ForceArgs {
                                // wavefronts, each taking a contiguous chunk of tiles
   NextRefresh next;            // the next step's refresh (perturbation mode)
 };
+// The cell-window form of the mixed force evaluation (no list): tiles of
+// up to 64 sorted atoms of one cell column (tile_off / tiles_per_col from
+// launch_tile_map after the rebuild), candidates from the rebuild's cell
+// offsets, rebuild-time cells from backup_s.
+struct CellWindow {
+  int n_cols, n_rep;
+  const uint32_t *tile_off, *tiles_per_col;
+  const int *start;
+  const uint4 *backup_s;
+  float skin;
+};
+void launch_force_cells(bool stats, bool tail, const ForceArgs &a, const CellWindow &c,
+                        hipStream_t st);
 // unroll: list rows per group of the mixed kernel (1, 2 or 4)
 void launch_force(int precision, int lanes, int unroll, int block, bool stats,
                   bool tail,

@@ -44,7 +44,7 @@ class Tuning(C.Structure):
     _fields_ = [("force_lanes", C.c_int), ("build_lanes", C.c_int), ("block", C.c_int),
                 ("cell_xy", C.c_float), ("cell_z", C.c_float), ("build_mode", C.c_int),
                 ("xcd_remap", C.c_int), ("force_unroll", C.c_int), ("persistent", C.c_int),
-                ("sort_mode", C.c_int)]
+                ("sort_mode", C.c_int), ("force_mode", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -18,7 +18,7 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(REPO, "md.error_amd", "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-KERNELS = re.compile(r"^(_ZN3sre\d+k_(force_mixed|adjoint_pairs|adjoint_piped)\S*):")
+KERNELS = re.compile(r"^(_ZN3sre\d+k_(force_mixed|force_cells|adjoint_pairs|adjoint_piped)\S*):")
 
 
 def device_asm(src, out):
