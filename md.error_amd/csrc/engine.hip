@@ -207,8 +207,8 @@ struct sre_engine {
   hipGraphExec_t rebuild_exec = nullptr;
   std::vector<unsigned char> rebuild_sig;
   // segment graphs: m x [steps graph steps + rebuild] as one graph for
-  // m = 1, 2, 4, ..., kSegGraphs / 2 (slot log2 m), all captured together for
-  // one (flags, steps) and the step and rebuild arguments
+  // m = 2^k, k < kSegGraphs (slot k), all captured together for one (flags,
+  // steps) and the step and rebuild arguments
   static constexpr int kSegGraphs = 5;
   hipGraphExec_t seg_exec[kSegGraphs] = {};
   unsigned seg_flags = 0;
