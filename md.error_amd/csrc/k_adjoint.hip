@@ -311,8 +311,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint_piped(AdjointArgs a) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
       p[u] = gather_pos_idx<(NT > 1)>(prs, (g * U + u) * T < lim ? jr[u] : si);
+    const int gr = max(min(g + 2, groups - 1), 0);   // past the last group: re-read
 #pragma unroll
-    for (int u = 0; u < U; ++u) jr[u] = row((g + 2) * U + u);
+    for (int u = 0; u < U; ++u) jr[u] = row(gr * U + u);
     asm volatile("" ::: "memory");
   };
   int ja[U], jb[U];

@@ -389,8 +389,9 @@ __device__ __forceinline__ int wave_max_int(int v) {
 // evaluates its own atom (d = 0) and masks the result.  The loop is software
 // pipelined: while group g is evaluated, the U position gathers of group g+1
 // and the U list rows of group g+2 are in flight, so the gather latency hides
-// behind a group's arithmetic.  Rows past the end of a wave's tile (up to 2U)
-// read the next tile or the list's tail padding; those values are never used.
+// behind a group's arithmetic.  The row loads issued for the two groups past
+// the wave's last one re-read its last group's rows; those values are never
+// used.
 // P.A12 and P.B6 carry the fixed-point scale 2^32 (see upload_params), so a
 // pair component c is already c*2^32 in FP32.
 //
@@ -503,8 +504,11 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const PairParam *
 #pragma unroll
     for (int u = 0; u < U; ++u)
       p[u] = gather_pos_idx<MULTI>(prs, (int)min((unsigned)jr[u], nmax));
+    // (rows of groups past the wave's last one: its last rows again, cache
+    // hits, so no row past the wave's lists is fetched)
+    const int gr = max(min(g + 2, groups - 1), 0);
 #pragma unroll
-    for (int u = 0; u < U; ++u) jr[u] = row((g + 2) * U + u);
+    for (int u = 0; u < U; ++u) jr[u] = row(gr * U + u);
     asm volatile("" ::: "memory");
   };
   int ja[U], jb[U];   // list rows of the groups due in buffers a and b
