@@ -952,21 +952,25 @@ static void enqueue_build(sre_engine *e, hipStream_t st) {
 // order with rc assignment, exact tail, list build, and the reduction of the
 // longest/total list length and of the segment's largest displacement (whose
 // partials it clears).
-static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st) {
+// counting sort while one block scans all cell counts (one placement
+// launch); beyond that the radix sort's passes are cheaper than the counting
+// sort's atomic round trips over many cells
+static bool counting_sort(const sre_engine *e) {
+  return e->tune.sort_mode == 2 || (e->tune.sort_mode == 0 && e->n_cells <= kPlaceCells);
+}
+
+// counted: the cell histogram was made by the force launch just before
+static void enqueue_rebuild(sre_engine *e, int n_parts, hipStream_t st,
+                            bool counted = false) {
   const int N = e->n_atoms;
   const float *anchor = e->pert_anchored ? e->d_anchor : nullptr;
   const float *rc_atom = e->rc_explicit ? e->d_rc_atom : nullptr;
-  // counting sort while one block scans all cell counts (one placement
-  // launch); beyond that the radix sort's passes are cheaper than the
-  // counting sort's atomic round trips over many cells
-  const bool counting = e->tune.sort_mode == 2 ||
-                        (e->tune.sort_mode == 0 && e->n_cells <= kPlaceCells);
-  if (counting) {
+  if (counting_sort(e)) {
     launch_cell_sort_gather(N, e->n_cells, e->d_pos, e->pos_stride, e->d_types,
                             e->d_rep_of, e->d_reps, e->d_rc, e->n_types, e->sb,
                             e->d_cell_count, e->d_start, e->d_pos4, e->d_backup,
                             e->d_inv_perm, e->d_res4, e->d_rcut, e->d_tail_idx,
-                            rc_atom, e->d_backup_s, anchor, e->d_anchor_s, st);
+                            rc_atom, e->d_backup_s, anchor, e->d_anchor_s, st, counted);
   } else {
     launch_radix_sort(N, e->key_bits, e->d_pos, e->pos_stride, e->d_rep_of,
                       e->d_reps, e->sb, st);
@@ -1811,8 +1815,10 @@ static sre_status enqueue_md_step(sre_engine *e, unsigned flags) {
 // graph's first refresh reads).  Same values as the unfused steps: the
 // refresh arithmetic is the same and no launch reads a buffer another
 // writes.
+// count_next: the last launch also makes the cell histogram of the counting
+// sort of a rebuild queued right after these steps (CountFold).
 static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
-                                      const PerturbArgs &pa) {
+                                      const PerturbArgs &pa, bool count_next = false) {
   const bool stats = flags & SRE_STATS;
   const bool tail = (flags & SRE_TAIL) && e->tail_bins > 0;
   auto pos4_of = [&](int j) { return ((K - 1 - j) % 2 == 0) ? e->d_pos4 : e->d_pos4_alt; };
@@ -1852,6 +1858,14 @@ static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
       a.next = nx;
       a.next.pos4 = pos4_of(j + 1);
       a.next.res4 = res4_of(j + 1);
+    } else if (count_next) {
+      a.count.n_blocks = nx.n_blocks;
+      a.count.pos = e->d_pos;
+      a.count.stride = e->pos_stride;
+      a.count.rep_of = e->d_rep_of;
+      a.count.n_cells = e->n_cells;
+      a.count.keys = e->sb.tkey;
+      a.count.count = e->d_cell_count;
     }
     launch_force(e->precision, e->list_lanes, e->tune.force_unroll, e->tune.block, stats,
                  tail, a, e->stream);
@@ -1861,19 +1875,22 @@ static sre_status enqueue_fused_steps(sre_engine *e, unsigned flags, int K,
   return SRE_OK;
 }
 
-// Captures K steps of [refresh (+ perturbation) + force] into one graph, or
-// returns the cached one; the graph reads the engine's buffers by address, so
-// every reallocation or layout change bumps graph_epoch and invalidates it.
 // The kernels of K <= kMaxGraphSteps steps (the contents of a step graph),
-// queued on the engine stream.
-static sre_status enqueue_steps(sre_engine *e, unsigned flags, int K) {
+// queued on the engine stream.  count_next asks the last force launch for the
+// histogram of the counting sort of a rebuild that follows; *counted tells
+// whether it was made (only the fused perturbation steps make it).
+static sre_status enqueue_steps(sre_engine *e, unsigned flags, int K,
+                                bool count_next = false, bool *counted = nullptr) {
   const bool pert = (flags & SRE_PERTURB) != 0;
   PerturbArgs pa{e->pert_sigma, e->pert_seed,
                  e->pert_anchored ? e->d_anchor : nullptr, e->d_step_ctr};
   const bool md = (flags & kGraphMD) != 0;
+  if (counted) *counted = false;
   if (pert && !md && !(flags & SRE_FCORR) && K >= 2 && (!pa.anchor || e->d_anchor_s) &&
-      e->tune.force_mode == 0)
-    return enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa);
+      e->tune.force_mode == 0) {
+    if (counted) *counted = count_next;
+    return enqueue_fused_steps(e, flags & (SRE_STATS | SRE_TAIL), K, pa, count_next);
+  }
   sre_status s = SRE_OK;
   for (int k = 0; k < K && s == SRE_OK; ++k) {
     if (md) {
@@ -1890,6 +1907,9 @@ static sre_status enqueue_steps(sre_engine *e, unsigned flags, int K) {
   return s;
 }
 
+// Captures K steps of [refresh (+ perturbation) + force] into one graph, or
+// returns the cached one; the graph reads the engine's buffers by address, so
+// every reallocation or layout change bumps graph_epoch and invalidates it.
 static sre_status step_graph(sre_engine *e, unsigned flags, int K,
                              hipGraphExec_t *out) {
   sre_engine::StepGraph *slot = nullptr, *lru = &e->step_graphs[0];
@@ -1996,9 +2016,12 @@ static sre_status segment_graphs(sre_engine *e, unsigned flags, int steps, int n
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     sre_status s = SRE_OK;
     for (int seg = 0; seg < (1 << k) && s == SRE_OK; ++seg) {
+      // the last force launch of the segment makes the rebuild's histogram
+      bool counted = false;
       for (int left = steps; left > 0 && s == SRE_OK; left -= kMaxGraphSteps)
-        s = enqueue_steps(e, flags, std::min(left, kMaxGraphSteps));
-      if (s == SRE_OK) enqueue_rebuild(e, n_parts, e->stream);
+        s = enqueue_steps(e, flags, std::min(left, kMaxGraphSteps),
+                          left <= kMaxGraphSteps && counting_sort(e), &counted);
+      if (s == SRE_OK) enqueue_rebuild(e, n_parts, e->stream, counted);
     }
     hipError_t r = hipStreamEndCapture(e->stream, &g);
     if (s != SRE_OK) return s;

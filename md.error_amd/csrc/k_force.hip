@@ -186,6 +186,22 @@ template <bool NEXT>
 __device__ __forceinline__ bool force_block(const ForceArgs &a, int *bid, int *nb) {
   *bid = blockIdx.x;
   *nb = gridDim.x;
+  if (a.count.n_blocks > 0) {   // block-uniform
+    if (*bid < a.count.n_blocks) {
+      const int i = *bid * (int)blockDim.x + (int)threadIdx.x;
+      if (i < a.n) {
+        // keys are < n_cells by construction; the clamp keeps the counts in bounds
+        const uint32_t k = min(cell_key(a.count.pos, a.count.stride, a.count.rep_of,
+                                        a.reps, i),
+                               (uint32_t)(a.count.n_cells - 1));
+        a.count.keys[i] = k;
+        atomicAdd(&a.count.count[k], 1u);
+      }
+      return false;
+    }
+    *bid -= a.count.n_blocks;
+    *nb -= a.count.n_blocks;
+  }
   if (NEXT) {
     if (*bid < a.next.n_blocks) {
       refresh_block(a.next, a.reps, a.n, *a.ctr_in + 1, *bid);
@@ -1114,8 +1130,10 @@ static void launch_force_t(int precision, int unroll, int block,
   int grid = (waves * kWave + block - 1) / block;
   const bool multi = a.n_types > 1;
   if (precision == 0 && a.persist_waves > 0) grid = a.persist_waves * kWave / block;
-  // the next step's refresh: blocks of its own in front (a multiple of 8)
+  // the next step's refresh and the next rebuild's histogram: blocks of their
+  // own in front (multiples of 8, so each force block keeps its XCD)
   if (a.next.active) grid += a.next.n_blocks;
+  grid += a.count.n_blocks;
   if (precision == 0) {
     if (unroll == 1) launch_mixed_u<T, 1, STATS, TAIL>(grid, block, multi, a, st);
     else if (unroll == 2) launch_mixed_u<T, 2, STATS, TAIL>(grid, block, multi, a, st);

@@ -31,18 +31,6 @@ constexpr int kFusedScanBlocks = 128;
 
 // ------------------------------------------------------------------ keys --
 
-// Cell key of atom i: replica cell offset + (cx, cy, cz) with z fastest, so a
-// column of cells is one contiguous range of sorted atoms.
-__device__ __forceinline__ uint32_t cell_key(const float *pos, int stride,
-                                             const int *rep_of,
-                                             const RepInfo *reps, int i) {
-  const RepInfo &R = reps[rep_of[i]];
-  const float *p = pos + (size_t)i * stride;
-  const int cx = cell_of(quantize(p[0], R.invq[0]), R.nc[0]);
-  const int cy = cell_of(quantize(p[1], R.invq[1]), R.nc[1]);
-  const int cz = cell_of(quantize(p[2], R.invq[2]), R.nc[2]);
-  return (uint32_t)(R.cell_off + (cx * R.nc[1] + cy) * R.nc[2] + cz);
-}
 
 // ------------------------------------------------------- block scan 1024 --
 
@@ -582,12 +570,14 @@ void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
                              int *start, uint4 *pos4, uint4 *backup,
                              uint32_t *inv_perm, float4 *res4, float *rcut,
                              int *tail_idx, const float *rc_atom, uint4 *backup_s,
-                             const float *anchor, float4 *anchor_s, hipStream_t st) {
+                             const float *anchor, float4 *anchor_s, hipStream_t st,
+                             bool counted) {
   if (n <= 0) return;
   uint32_t *cursor = count + cell_count_half(n_cells);
   const dim3 g((n + 255) / 256), t(256);
-  hipLaunchKernelGGL(k_cell_count, g, t, 0, st, n, n_cells, pos, stride, rep_of, reps,
-                     b.tkey, count);
+  if (!counted)
+    hipLaunchKernelGGL(k_cell_count, g, t, 0, st, n, n_cells, pos, stride, rep_of, reps,
+                       b.tkey, count);
   if (n_cells <= kPlaceCells) {
     hipLaunchKernelGGL(k_cell_place, dim3((n + kSortT - 1) / kSortT), dim3(kSortT), 0, st,
                        n, n_cells, b.tkey, count, cursor, start, b.key_b, b.val_b);

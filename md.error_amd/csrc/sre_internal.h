@@ -128,6 +128,19 @@ __host__ __device__ inline int cell_of(uint32_t u, int nc) {
   return (int)(((uint64_t)u * (uint64_t)nc) >> 32);
 }
 
+// Cell key of atom i: replica cell offset + (cx, cy, cz) with z fastest, so a
+// column of cells is one contiguous range of sorted atoms.
+__device__ __forceinline__ uint32_t cell_key(const float *pos, int stride,
+                                             const int *rep_of,
+                                             const RepInfo *reps, int i) {
+  const RepInfo &R = reps[rep_of[i]];
+  const float *p = pos + (size_t)i * stride;
+  const int cx = cell_of(quantize(p[0], R.invq[0]), R.nc[0]);
+  const int cy = cell_of(quantize(p[1], R.invq[1]), R.nc[1]);
+  const int cz = cell_of(quantize(p[2], R.invq[2]), R.nc[2]);
+  return (uint32_t)(R.cell_off + (cx * R.nc[1] + cy) * R.nc[2] + cz);
+}
+
 // Position gathers through a buffer resource over pos4 in index mode: a
 // resource with a 16-byte record stride takes the atom index itself
 // (buffer_load ... idxen), so a gather needs neither a 64-bit address nor a
@@ -322,6 +335,8 @@ size_t radix_counts_capacity(int n, int key_bits);
 size_t cell_count_capacity(int n_cells);
 // up to this many cells the scan of the counts is fused into the placement
 constexpr int kPlaceCells = 15360;
+// counted: the keys (b.tkey) and counts were made by a CountFold of the last
+// force launch, so the histogram launch is skipped.
 void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
                              const int *types, const int *rep_of,
                              const RepInfo *reps, const float *rc_table,
@@ -329,7 +344,8 @@ void launch_cell_sort_gather(int n, int n_cells, const float *pos, int stride,
                              int *start, uint4 *pos4, uint4 *backup,
                              uint32_t *inv_perm, float4 *res4, float *rcut,
                              int *tail_idx, const float *rc_atom, uint4 *backup_s,
-                             const float *anchor, float4 *anchor_s, hipStream_t st);
+                             const float *anchor, float4 *anchor_s, hipStream_t st,
+                             bool counted = false);
 // Gather into cell order (+ rc assignment) fused with the cell offsets
 // start[0..n_cells] computed from the sorted keys.
 void launch_gather_sorted(int n, int n_cells, const uint32_t *keys,
@@ -497,6 +513,20 @@ struct NextRefresh {
   int one_rep;
 };
 
+// The histogram step of the counting sort of the rebuild that follows a
+// force launch, done by blocks in front of that launch when it is the last
+// of a segment (it does not move the atoms, so the keys are those the
+// rebuild would compute): one atom per thread, caller order, keys[i] and
+// count[key] += 1 as k_cell_count.
+struct CountFold {
+  int n_blocks;                // 0 = none; else a multiple of 8
+  const float *pos;            // caller positions
+  int stride;
+  const int *rep_of;
+  int n_cells;
+  uint32_t *keys, *count;
+};
+
 struct ForceArgs {
   int n;
   int n_types;
@@ -529,6 +559,7 @@ struct ForceArgs {
   int persist_waves;           // mixed form: > 0 = persistent launch of that many
                                // wavefronts, each taking a contiguous chunk of tiles
   NextRefresh next;            // the next step's refresh (perturbation mode)
+  CountFold count;             // the next rebuild's cell histogram, or none
 };
 // The cell-window form of the mixed force evaluation (no list): tiles of
 // up to 64 sorted atoms of one cell column (tile_off / tiles_per_col from
