@@ -343,18 +343,14 @@ def test_overlapping_atoms_raise_force_range(torch_cuda):
         e.stats()
 
 
-@pytest.mark.parametrize("n_seg,steps", [(3, 5), (19, 2)])
-def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps):
-    """sre_run_segments (segments and rebuilds queued as graphs of up to 16
-    segments, checks at the end) does what graph_run + rebuild per segment
-    does: the same positions, lists and forces bit for bit, the same list
-    lengths and largest displacement."""
+def _segments_vs_manual(w, n_seg, steps, sort_mode=0):
     import torch
     res = []
     for queued in (False, True):
-        e = Engine(skin=slab.skin)
-        slab.setup(e, tail=True)
-        pos = torch.from_numpy(slab.xyz.copy()).cuda()
+        e = Engine(n_types=w.n_types, skin=w.skin)
+        e.set_tuning(sort_mode=sort_mode)
+        w.setup(e, tail=True)
+        pos = torch.from_numpy(w.xyz.copy()).cuda()
         torch.cuda.synchronize()
         e.bind_positions(pos)
         e.rebuild()
@@ -375,7 +371,26 @@ def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps):
                     pos.cpu().numpy(), st))
         e.close()
     (n0, d0, f0, p0, x0, s0), (n1, d1, f1, p1, x1, s1) = res
-    assert n0 == n1 and d0 == d1 and 0.0 < d1 <= 0.5 * slab.skin
+    assert n0 == n1 and d0 == d1 and 0.0 < d1 <= 0.5 * w.skin
     np.testing.assert_array_equal(s0, s1)
     assert np.array_equal(x0, x1) and np.array_equal(f0, f1)
     assert np.array_equal(p0[0], p1[0]) and np.array_equal(p0[1], p1[1])
+
+
+@pytest.mark.parametrize("n_seg,steps,sort_mode", [(3, 5, 0), (19, 2, 0), (4, 3, 1)])
+def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps, sort_mode):
+    """sre_run_segments (segments and rebuilds queued as graphs of up to 16
+    segments, the cell histogram of each rebuild made by the segment's last
+    force launch, checks at the end) does what graph_run + rebuild per
+    segment does: the same positions, lists and forces bit for bit, the same
+    list lengths, largest displacement and last-step statistics.  sort_mode
+    1 (radix sort): no histogram in the force launch."""
+    _segments_vs_manual(slab, n_seg, steps, sort_mode)
+
+
+def test_run_segments_equals_graph_and_rebuild_species_and_replicas(torch_cuda, binary):
+    """The same with per-type parameters and with several replicas (the
+    histogram blocks compute every replica's cell keys)."""
+    _segments_vs_manual(binary, 5, 4)
+    _segments_vs_manual(configs.replicas(seed=15, count=5, n_lo=1500, n_hi=5000, nz=32),
+                        5, 4)
