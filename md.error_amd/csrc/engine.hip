@@ -2011,6 +2011,7 @@ static sre_status segment_graphs(sre_engine *e, unsigned flags, int steps, int n
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  e->seg_epoch = -1;   // valid again only once every size is captured
   for (int k = 0; k < sre_engine::kSegGraphs; ++k) {
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -2024,7 +2025,10 @@ static sre_status segment_graphs(sre_engine *e, unsigned flags, int steps, int n
       if (s == SRE_OK) enqueue_rebuild(e, n_parts, e->stream, counted);
     }
     hipError_t r = hipStreamEndCapture(e->stream, &g);
-    if (s != SRE_OK) return s;
+    if (s != SRE_OK) {
+      if (r == hipSuccess) (void)hipGraphDestroy(g);
+      return s;
+    }
     if (r != hipSuccess)
       return fail(SRE_ERR_HIP, "segment graph capture: %s", hipGetErrorString(r));
     r = hipGraphInstantiate(&e->seg_exec[k], g, nullptr, nullptr, 0);
