@@ -1931,7 +1931,10 @@ static sre_status step_graph(sre_engine *e, unsigned flags, int K,
     HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     sre_status s = enqueue_steps(e, flags, K);
     hipError_t r = hipStreamEndCapture(e->stream, &g);
-    if (s != SRE_OK) return s;
+    if (s != SRE_OK) {
+      if (r == hipSuccess) (void)hipGraphDestroy(g);
+      return s;
+    }
     if (r != hipSuccess)
       return fail(SRE_ERR_HIP, "graph capture: %s", hipGetErrorString(r));
     r = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
