@@ -246,9 +246,10 @@ typedef struct {
                            above; 1 = LSD radix sort of the cell key; 2 = counting
                            sort at any cell count */
   int force_mode;       /* mixed form: 0 = neighbor-list kernel; 1 = cell-window
-                           kernel without a list (one wavefront per 64-atom tile of
-                           a cell column, candidates staged in LDS and read by all
-                           lanes; bitwise the same forces, measured slower) */
+                           kernel without a list (one wavefront per 16-atom tile of
+                           a cell column, four lanes per atom, candidates staged in
+                           LDS; bitwise the same forces, measured slower; needs
+                           build_mode 0) */
 } sre_tuning;
 sre_status sre_get_tuning(sre_engine *e, sre_tuning *t);
 /* Changing cell sizes or lanes takes effect at the next rebuild. */

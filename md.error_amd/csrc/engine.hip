@@ -934,8 +934,8 @@ static void enqueue_build(sre_engine *e, hipStream_t st) {
   const int N = e->n_atoms;
   // the cell-window force form reads the tile map of the tile build
   if (e->tune.force_mode == 1 && e->tune.build_mode != 1)
-    launch_tile_map(e->n_cols, e->n_rep, e->d_reps, e->d_start, e->d_tiles_per_col,
-                    e->d_tile_off, e->d_tile_tmp, st);
+    launch_tile_map(e->n_cols, e->n_rep, e->d_reps, e->d_start, kCellTile,
+                    e->d_tiles_per_col, e->d_tile_off, e->d_tile_tmp, st);
   if (e->tune.build_mode == 1)
     launch_build_tiles(N, e->n_cols, e->n_rep, e->tune.force_lanes, e->d_pos4,
                        e->d_rcut, e->d_reps, e->d_start, e->skin, e->cap,
@@ -1568,6 +1568,8 @@ extern "C" sre_status sre_set_tuning(sre_engine *e, const sre_tuning *t) {
     return fail(SRE_ERR_INVALID, "sort_mode must be 0, 1 or 2");
   if (t->force_mode != 0 && t->force_mode != 1)
     return fail(SRE_ERR_INVALID, "force_mode must be 0 or 1");
+  if (t->force_mode == 1 && t->build_mode == 1)
+    return fail(SRE_ERR_INVALID, "force_mode 1 needs build_mode 0 (both use the tile map)");
   // (the cell-window force form reads a tile map made at the rebuild)
   const bool layout = t->force_lanes != e->tune.force_lanes ||
                       t->build_lanes != e->tune.build_lanes ||

@@ -632,16 +632,18 @@ __global__ __launch_bounds__(256) void k_force_mixed(ForceArgs a) {
 // ------------------------------------------------- cell-window form ----
 //
 // The mixed force evaluation without a neighbor list: one wavefront per
-// tile of up to 64 consecutive sorted atoms of one cell column (the tiles of
-// the tile list build), one atom per lane.  The candidates are the cell runs
-// within rc_max + skin of the tile's rebuild-time cells -- the tile build's
-// window, which holds every pair the list holds -- staged 64 at a time in
-// LDS and read by all lanes at once (one broadcast read per candidate).
-// Each lane masks |d| < rc_i at the current positions with the list
-// kernel's per-pair arithmetic, so the pair set and every pair component are
-// the list kernel's and the fixed-point forces are bitwise the same; the
-// FP32 virial and energy partial sums run in another order.  Rebuild-time
-// cells come from the rebuild-time lattice copy in cell order (backup_s).
+// tile of up to kCellTile consecutive sorted atoms of one cell column,
+// T = kWave / kCellTile lanes per atom.  The candidates are the cell runs
+// within rc_max + skin of the tile's rebuild-time cells -- a window that
+// holds every pair a list of the tile's atoms would -- staged 64 at a time
+// in LDS; lane l of an atom takes the staged candidates l, l + T, ... (every
+// atom of the wave reads the same LDS words at once).  Each lane masks
+// |d| < rc_i at the current positions with the list kernel's per-pair
+// arithmetic, so the pair set and every pair component are the list
+// kernel's and the fixed-point forces (exact sums, summed over the atom's
+// lanes) are bitwise the same; the FP32 virial and energy partial sums run
+// in another order.  Rebuild-time cells come from the rebuild-time lattice
+// copy in cell order (backup_s).
 template <bool STATS, bool TAIL, bool MULTI, bool SHIFT>
 __global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c) {
   if (a.ctr_out && blockIdx.x == 0 && threadIdx.x == 0) *a.ctr_out = *a.ctr_in + 1;
@@ -672,13 +674,15 @@ __global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c
   const int ncx = R.nc[0], ncy = R.nc[1], ncz = R.nc[2];
   const float hx = R.h[0], hy = R.h[1], hz = R.h[2];
   const float qx = R.q[0], qy = R.q[1], qz = R.q[2];
+  constexpr int T = kWave / kCellTile;
   const int col = col_g - R.col_off;
   const int cx = col / ncy, cy = col - cx * ncy;
   const int cbase = R.cell_off + col * ncz;
-  const int a0 = c.start[cbase] + (tile - (int)c.tile_off[col_g]) * kWave;
-  const int a1 = min(c.start[cbase + ncz], a0 + kWave);
+  const int a0 = c.start[cbase] + (tile - (int)c.tile_off[col_g]) * kCellTile;
+  const int a1 = min(c.start[cbase + ncz], a0 + kCellTile);
   const int lane = threadIdx.x;
-  const int s = a0 + lane;
+  const int l = lane % T;
+  const int s = a0 + lane / T;
   const bool act = s < a1;
   const int si = act ? s : a0;
   const uint4 pi = a.pos4[si];
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c
           cand[lane] = a.pos4[q < p1 ? q : p0];
           __syncthreads();
           const int m = min(kWave, p1 - p);
-          for (int k = 0; k < m; ++k) {
+          for (int k = l; k < m; k += T) {
             const uint4 cj = cand[k];
             const float dx = disp(cj.x, pi.x, qx);
             const float dy = disp(cj.y, pi.y, qy);
@@ -775,21 +779,21 @@ __global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c
   // the compiler cannot schedule their FP64 conversions in front of it
   asm volatile("" : "+v"(e), "+v"(e6), "+v"(wx), "+v"(wy), "+v"(wz), "+v"(fsabs));
   const int64_t acc0 = __double_as_longlong(kAccBase);
-  int64_t Fx = __double_as_longlong(ax) - acc0;
-  int64_t Fy = __double_as_longlong(ay) - acc0;
-  int64_t Fz = __double_as_longlong(az) - acc0;
+  int64_t Fx = group_sum(__double_as_longlong(ax) - acc0, T);
+  int64_t Fy = group_sum(__double_as_longlong(ay) - acc0, T);
+  int64_t Fz = group_sum(__double_as_longlong(az) - acc0, T);
   if (__ballot(act && fsabs * rc >= 1125899906842624.0f) && lane == 0)
     atomicOr(a.err_flag, 1u);
-  if (!act) return;
   double E = 0.0, Wx = 0.0, Wy = 0.0, Wz = 0.0;
   if (STATS) {
-    E = SPLIT_E ? (-((double)wx + (double)wy + (double)wz) * (kFixInv / 12.0) -
-                   0.5 * (double)P.B * (double)e6) * 0.5
-                : (double)e * 0.5;
-    Wx = (double)wx * (0.5 * kFixInv);
-    Wy = (double)wy * (0.5 * kFixInv);
-    Wz = (double)wz * (0.5 * kFixInv);
+    E = SPLIT_E ? group_sum(-((double)wx + (double)wy + (double)wz) * (kFixInv / 12.0) -
+                                0.5 * (double)P.B * (double)e6, T) * 0.5
+                : group_sum((double)e, T) * 0.5;
+    Wx = group_sum((double)wx, T) * (0.5 * kFixInv);
+    Wy = group_sum((double)wy, T) * (0.5 * kFixInv);
+    Wz = group_sum((double)wz, T) * (0.5 * kFixInv);
   }
+  if (!act || l != 0) return;
   if (TAIL) {
     double2 tv = make_double2(0.0, 0.0);
     if (a.tail_atom) {
@@ -814,7 +818,7 @@ __global__ __launch_bounds__(kWave) void k_force_cells(ForceArgs a, CellWindow c
 void launch_force_cells(bool stats, bool tail, const ForceArgs &a, const CellWindow &c,
                         hipStream_t st) {
   if (a.n <= 0 || c.n_cols <= 0) return;
-  const int max_tiles = (a.n + kWave - 1) / kWave + c.n_cols;
+  const int max_tiles = (a.n + kCellTile - 1) / kCellTile + c.n_cols;
   const bool multi = a.n_types > 1, shift = a.has_shift != 0;
 #define SRE_CELLS(S, T, M, H)                                                       \
   hipLaunchKernelGGL((k_force_cells<S, T, M, H>), dim3(max_tiles), dim3(kWave), 0, st, \

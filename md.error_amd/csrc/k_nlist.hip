@@ -319,14 +319,14 @@ __device__ __forceinline__ int find_replica_by_col(const RepInfo *reps, int n_re
 
 __global__ __launch_bounds__(256) void k_col_tiles(int n_cols, int n_rep,
                                                    const RepInfo *reps,
-                                                   const int *start,
+                                                   const int *start, int tile,
                                                    uint32_t *tiles_per_col) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cols) return;
   const RepInfo &R = reps[find_replica_by_col(reps, n_rep, c)];
   const int base = R.cell_off + (c - R.col_off) * R.nc[2];
   const int cnt = start[base + R.nc[2]] - start[base];
-  tiles_per_col[c] = (uint32_t)((cnt + kWave - 1) / kWave);
+  tiles_per_col[c] = (uint32_t)((cnt + tile - 1) / tile);
 }
 
 __global__ __launch_bounds__(kWave) void k_build_tiles(
@@ -442,11 +442,11 @@ __global__ __launch_bounds__(kWave) void k_build_tiles(
 }
 
 void launch_tile_map(int n_cols, int n_rep, const RepInfo *reps, const int *start,
-                     uint32_t *tiles_per_col, uint32_t *tile_off, uint32_t *scan_tmp,
-                     hipStream_t st) {
+                     int tile, uint32_t *tiles_per_col, uint32_t *tile_off,
+                     uint32_t *scan_tmp, hipStream_t st) {
   if (n_cols <= 0) return;
   hipLaunchKernelGGL(k_col_tiles, dim3((n_cols + 255) / 256), dim3(256), 0, st,
-                     n_cols, n_rep, reps, start, tiles_per_col);
+                     n_cols, n_rep, reps, start, tile, tiles_per_col);
   device_exscan_u32((size_t)n_cols, tiles_per_col, tile_off, scan_tmp, st);
 }
 
@@ -457,7 +457,8 @@ void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
                         uint32_t *tiles_per_col, uint32_t *tile_off,
                         uint32_t *scan_tmp, hipStream_t st) {
   if (n <= 0 || n_cols <= 0) return;
-  launch_tile_map(n_cols, n_rep, reps, start, tiles_per_col, tile_off, scan_tmp, st);
+  launch_tile_map(n_cols, n_rep, reps, start, kWave, tiles_per_col, tile_off, scan_tmp,
+                  st);
   // every column adds at most one partial tile
   const int max_tiles = (n + kWave - 1) / kWave + n_cols;
   hipLaunchKernelGGL(k_build_tiles, dim3(max_tiles), dim3(kWave), 0, st, n_cols,

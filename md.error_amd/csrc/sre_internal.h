@@ -376,11 +376,11 @@ void launch_build_list(int lanes, int force_lanes, int n, const uint4 *pos4,
                        const float *rcut, const RepInfo *reps, const int *start,
                        float skin, int cap, int *list, int *nnb, int2 *part,
                        hipStream_t st);
-// Tiles of <= 64 atoms inside one cell column: tiles_per_col and tile_off
+// Tiles of <= tile atoms inside one cell column: tiles_per_col and tile_off
 // (n_cols each) filled on the device from the cell offsets.
 void launch_tile_map(int n_cols, int n_rep, const RepInfo *reps, const int *start,
-                     uint32_t *tiles_per_col, uint32_t *tile_off, uint32_t *scan_tmp,
-                     hipStream_t st);
+                     int tile, uint32_t *tiles_per_col, uint32_t *tile_off,
+                     uint32_t *scan_tmp, hipStream_t st);
 // Tile build: tiles of <= 64 atoms inside one cell column.  tiles_per_col and
 // tile_off (n_cols each) are scratch arrays filled on the device.
 void launch_build_tiles(int n, int n_cols, int n_rep, int force_lanes,
@@ -562,9 +562,11 @@ struct ForceArgs {
   CountFold count;             // the next rebuild's cell histogram, or none
 };
 // The cell-window form of the mixed force evaluation (no list): tiles of
-// up to 64 sorted atoms of one cell column (tile_off / tiles_per_col from
-// launch_tile_map after the rebuild), candidates from the rebuild's cell
-// offsets, rebuild-time cells from backup_s.
+// up to kCellTile sorted atoms of one cell column (tile_off / tiles_per_col
+// from launch_tile_map after the rebuild), kWave / kCellTile lanes per atom,
+// candidates from the rebuild's cell offsets, rebuild-time cells from
+// backup_s.
+constexpr int kCellTile = 16;
 struct CellWindow {
   int n_cols, n_rep;
   const uint32_t *tile_off, *tiles_per_col;

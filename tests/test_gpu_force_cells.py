@@ -1,6 +1,6 @@
 """The cell-window form of the mixed force evaluation (sre_tuning.force_mode
-= 1: no neighbor list; one wavefront per 64-atom tile of a cell column, the
-candidates of the tile's cell window staged in LDS and read by every lane)
+= 1: no neighbor list; one wavefront per 16-atom tile of a cell column, four
+lanes per atom, the candidates of the tile's cell window staged in LDS)
 evaluates the same pairs with the same per-pair arithmetic as the list
 kernel, so its fixed-point forces are bitwise the list kernel's; energy and
 virial sum the same FP32 terms in another order.  Checked at the rebuild
