@@ -137,6 +137,8 @@ struct sre_engine {
   // [0] longest list, [1] total length, [2] low word: max displacement^2
   unsigned long long *d_counters = nullptr;
   uint64_t *h_info = nullptr;               // pinned readback (4 words)
+  double *h_stats = nullptr;                // pinned: 4 per replica + the force-range flag
+  size_t h_stats_cap = 0;
   // rebuild log: every launch of the captured rebuild writes its (longest,
   // total, displacement^2 bits) to entry (launch number % kRebuildLog) of
   // this pinned host array, mapped for the device (d_log); d_log_idx is the
@@ -253,6 +255,7 @@ sre_engine::~sre_engine() {
   dfree(d_tiles_per_col); dfree(d_tile_off); dfree(d_tile_tmp);
   dfree(d_counters); dfree(d_fx); dfree(d_fy); dfree(d_fz); dfree(d_fxd);
   if (h_info) (void)hipHostFree(h_info);
+  if (h_stats) (void)hipHostFree(h_stats);
   if (h_log) (void)hipHostFree(h_log);
   dfree(d_log_idx);
   dfree(d_fyd); dfree(d_fzd); dfree(d_nin); dfree(d_stats); dfree(d_stat_part);
@@ -1293,10 +1296,24 @@ extern "C" sre_status sre_get_stats(sre_engine *e, double *out) {
   HIP_TRY(hipSetDevice(e->device));
   sre_status rs = reduce_stats(e);
   if (rs != SRE_OK) return rs;
-  HIP_TRY(hipMemcpyAsync(out, e->d_stats, sizeof(double) * 4 * e->n_rep,
+  // the totals and the force-range flag into pinned memory, one synchronisation
+  const size_t m = (size_t)4 * e->n_rep;
+  if (e->h_stats_cap < m + 1) {
+    if (e->h_stats) HIP_TRY(hipHostFree(e->h_stats));
+    e->h_stats = nullptr;
+    e->h_stats_cap = 0;
+    HIP_TRY(hipHostMalloc((void **)&e->h_stats, sizeof(double) * (m + 1), hipHostMallocDefault));
+    e->h_stats_cap = m + 1;
+  }
+  unsigned *flag = reinterpret_cast<unsigned *>(e->h_stats + m);
+  HIP_TRY(hipMemcpyAsync(e->h_stats, e->d_stats, sizeof(double) * m,
                          hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipMemcpyAsync(flag, e->d_flags + 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                         e->stream));
   HIP_TRY(hipStreamSynchronize(e->stream));
-  return check_errors(e);
+  memcpy(out, e->h_stats, sizeof(double) * m);
+  if (*flag & 1u) return report_force_range(e);
+  return SRE_OK;
 }
 
 extern "C" sre_status sre_get_atom_stats(sre_engine *e, double *out) {
