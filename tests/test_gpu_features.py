@@ -343,11 +343,11 @@ def test_overlapping_atoms_raise_force_range(torch_cuda):
         e.stats()
 
 
-def _segments_vs_manual(w, n_seg, steps, sort_mode=0):
+def _segments_vs_manual(w, n_seg, steps, sort_mode=0, prec="mixed"):
     import torch
     res = []
     for queued in (False, True):
-        e = Engine(n_types=w.n_types, skin=w.skin)
+        e = Engine(n_types=w.n_types, skin=w.skin, precision=prec)
         e.set_tuning(sort_mode=sort_mode)
         w.setup(e, tail=True)
         pos = torch.from_numpy(w.xyz.copy()).cuda()
@@ -367,8 +367,8 @@ def _segments_vs_manual(w, n_seg, steps, sort_mode=0):
                 nls.append(e.neighbor_count(in_rc=False)[0])
                 disp = max(disp, e.last_rebuild_displacement())
         e.graph_run(5, stats=True, tail=True, perturb=True)
-        res.append((list(nls), disp, e.forces_fixed(), e.export_pairs(),
-                    pos.cpu().numpy(), st))
+        F = e.forces_fixed() if prec == "mixed" else e.forces()
+        res.append((list(nls), disp, F, e.export_pairs(), pos.cpu().numpy(), st))
         e.close()
     (n0, d0, f0, p0, x0, s0), (n1, d1, f1, p1, x1, s1) = res
     assert n0 == n1 and d0 == d1 and 0.0 < d1 <= 0.5 * w.skin
@@ -388,9 +388,11 @@ def test_run_segments_equals_graph_and_rebuild(torch_cuda, slab, n_seg, steps, s
     _segments_vs_manual(slab, n_seg, steps, sort_mode)
 
 
-def test_run_segments_equals_graph_and_rebuild_species_and_replicas(torch_cuda, binary):
-    """The same with per-type parameters and with several replicas (the
-    histogram blocks compute every replica's cell keys)."""
+def test_run_segments_equals_graph_and_rebuild_species_and_replicas(torch_cuda, binary, slab):
+    """The same with per-type parameters, with several replicas (the
+    histogram blocks compute every replica's cell keys) and in the FP64 form
+    (its force launches carry the same refresh and histogram blocks)."""
     _segments_vs_manual(binary, 5, 4)
+    _segments_vs_manual(slab, 3, 4, prec="fp64")
     _segments_vs_manual(configs.replicas(seed=15, count=5, n_lo=1500, n_hi=5000, nz=32),
                         5, 4)
