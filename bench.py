@@ -507,6 +507,9 @@ def main():
                          "1, 2, 4, 8 wavefronts per SIMD against the flat launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--force-kernel-only", action="store_true",
+                    help="only time the mixed force kernel back to back (the launches "
+                         "the roofline figure averages), for a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
@@ -532,6 +535,19 @@ def main():
         return
     w = configs.WORKLOADS[args.workload]()
     log(f"[bench] workload {w.name}: N={w.n}, {w.notes}")
+    if args.force_kernel_only:
+        e = Engine(n_types=w.n_types, skin=w.skin)
+        w.setup(e, tail=True)
+        e.rebuild()
+        e.time_compute(20, stats=True, tail=True)
+        k_ms = e.time_compute(200, stats=True, tail=True)
+        nl, _, _ = e.neighbor_count(in_rc=False)
+        b = 4.0 * nl + 84.0 * w.n
+        print(json.dumps({"workload": w.name, "force_kernel_us": round(k_ms * 1e3, 3),
+                          "launches": 220, "bytes_per_launch": b,
+                          "achieved_GBs": round(b / (k_ms * 1e-3) / 1e9, 1)}))
+        e.close()
+        return
     mixed = run_form(w, "mixed", args.steps, args.warmup, args.rebuild_every, torch, Engine,
                      autotune=args.autotune)
     log(f"[bench] mixed: {mixed['pair_evals_per_s']:.4g} pair evals/s, "
